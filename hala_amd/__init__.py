@@ -1,0 +1,350 @@
+"""hala_amd -- HALA's sparse iterative-solve path on MI355X (gfx950).
+
+Python mirror of the reference's engine/matrix/ILU/solver interface
+(gpu/hala_gpu_engine.hpp, gpu/hala_rocm_sparse_general.hpp,
+gpu/hala_gpu_ilu.hpp, hex/solvers/*.hpp), used by the tests, the benchmark
+and the autograd harness.  Device memory comes from torch tensors; every
+computation is a call into libhala_amd.so through its C ABI.  The C++ drop-in
+for user code calling hala:: lives in include/hala/.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import CsrInfo, HalaError, IluInfo, call, lib  # noqa: F401
+from . import stencils  # noqa: F401
+
+__all__ = [
+    "HalaError", "gpu_engine", "stop_criteria", "make_sparse_matrix", "sparse_gemv", "make_ilu",
+    "make_triangular_matrix", "solve_cg", "solve_cg_ilu", "solve_gmres", "solve_gmres_ilu",
+    "solve_gmres_ir", "dot", "norm2", "axpy", "scal", "vcopy", "stencils",
+]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class stop_criteria:
+    """hex/solvers/hala_solvers_core.hpp:52-71: absolute tolerance and/or iteration cap."""
+
+    def __init__(self, tol=0.0, max_iter=2**31 - 1):
+        self.tol = float(tol)
+        self.max_iter = int(max_iter)
+
+
+class gpu_engine:
+    """gpu/hala_gpu_engine.hpp:167-307: one device, one stream (torch's current stream by default)."""
+
+    def __init__(self, device=0, stream=None):
+        torch = _torch()
+        self._device = int(device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self._device)
+        self.stream = stream
+        h = C.c_void_p()
+        call("hala_ctx_create", self._device, C.c_void_p(stream.cuda_stream), C.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and lib() is not None:
+            lib().hala_ctx_destroy(self.handle)
+            self.handle = None
+
+    def device(self):
+        return self._device
+
+    def synchronize(self):
+        call("hala_ctx_sync", self.handle)
+
+    def load(self, arr, dtype=None):
+        torch = _torch()
+        a = np.ascontiguousarray(arr if dtype is None else np.asarray(arr, dtype=dtype))
+        return torch.from_numpy(a).to(f"cuda:{self._device}")
+
+    @staticmethod
+    def unload(t):
+        return t.detach().cpu().numpy()
+
+    def vector(self, n, value=0.0, dtype=None):
+        torch = _torch()
+        dtype = dtype or torch.float64
+        t = torch.empty(int(n), dtype=dtype, device=f"cuda:{self._device}")
+        if n:
+            if dtype == torch.float64:
+                call("hala_fill_f64", self.handle, _ptr(t), int(n), float(value))
+            elif dtype == torch.float32:
+                call("hala_fill_f32", self.handle, _ptr(t), int(n), float(value))
+            else:
+                call("hala_fill_i32", self.handle, _ptr(t), int(n), int(value))
+        return t
+
+
+def _dt(t):
+    torch = _torch()
+    return {torch.float64: 64, torch.float32: 32}[t.dtype]
+
+
+class gpu_sparse_matrix:
+    """gpu/hala_rocm_sparse_general.hpp:74-169: non-owning view of a device CSR (kept alive here)."""
+
+    def __init__(self, engine, nrows, ncols, pntr, indx, vals):
+        self.engine = engine
+        self.pntr, self.indx, self.vals = pntr, indx, vals
+        self.nrows, self.ncols = int(nrows), int(ncols)
+        h = C.c_void_p()
+        call("hala_csr_create", engine.handle, self.nrows, self.ncols, int(indx.numel()), _ptr(pntr), _ptr(indx),
+             C.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None:
+            lib().hala_csr_destroy(self.handle)
+            self.handle = None
+
+    def info(self):
+        inf = CsrInfo()
+        call("hala_csr_info", self.handle, C.byref(inf))
+        return {k: getattr(inf, k) for k, _ in CsrInfo._fields_}
+
+    def set_kernel(self, k):
+        call("hala_csr_set_kernel", self.handle, int(k))
+
+    def gemv(self, trans, alpha, x, beta, y, vals=None):
+        vals = self.vals if vals is None else vals
+        t = trans.encode() if isinstance(trans, str) else trans
+        if _dt(vals) == 64:
+            call("hala_spmv_f64", self.handle, t, float(alpha), _ptr(vals), _ptr(x), float(beta), _ptr(y))
+        elif _dt(x) == 32:
+            call("hala_spmv_f32", self.handle, t, float(alpha), _ptr(vals), _ptr(x), float(beta), _ptr(y))
+        else:
+            assert t in (b"N", b"n"), "FP32-value product with FP64 vectors is 'N' only"
+            call("hala_spmv_f32x64", self.handle, float(alpha), _ptr(vals), _ptr(x), float(beta), _ptr(y))
+        return y
+
+
+def make_sparse_matrix(engine, ncols, pntr, indx, vals):
+    """make_sparse_matrix(gpu_engine, ncols, pntr, indx, vals): gpu/hala_rocm_sparse_general.hpp:179-185."""
+    return gpu_sparse_matrix(engine, int(pntr.numel()) - 1, ncols, pntr, indx, vals)
+
+
+def sparse_gemv(engine, trans, M, N, alpha, pntr, indx, vals, x, beta, y):
+    """sparse_gemv(gpu_engine, ...): gpu/hala_rocm_sparse_general.hpp:188-199."""
+    return gpu_sparse_matrix(engine, M, N, pntr, indx, vals).gemv(trans, alpha, x, beta, y)
+
+
+class gpu_ilu:
+    """gpu/hala_gpu_ilu.hpp:45-185: ILU(0) factor owned on the device, pattern aliased."""
+
+    def __init__(self, engine, pntr, indx, vals, policy="N", want_f32=False):
+        self.engine = engine
+        self.pntr, self.indx = pntr, indx
+        self.n = int(pntr.numel()) - 1
+        self.nnz = int(indx.numel())
+        h = C.c_void_p()
+        call("hala_ilu_create", engine.handle, self.n, self.nnz, _ptr(pntr), _ptr(indx), C.byref(h))
+        self.handle = h
+        self.policy = policy
+        if vals is not None:
+            self.factor(vals, want_f32)
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None:
+            lib().hala_ilu_destroy(self.handle)
+            self.handle = None
+
+    def factor(self, vals, want_f32=False):
+        call("hala_ilu_factor_f64", self.handle, _ptr(vals), int(bool(want_f32)))
+
+    def info(self):
+        inf = IluInfo()
+        call("hala_ilu_info", self.handle, C.byref(inf))
+        return {k: getattr(inf, k) for k, _ in IluInfo._fields_}
+
+    def apply(self, x, r, num_rhs=1):
+        assert num_rhs == 1
+        if _dt(x) == 64:
+            call("hala_ilu_apply_f64", self.handle, _ptr(x), _ptr(r))
+        else:
+            call("hala_ilu_apply_f32", self.handle, _ptr(x), _ptr(r))
+        return r
+
+    def apply_f32x64(self, x, r):
+        call("hala_ilu_apply_f32x64", self.handle, _ptr(x), _ptr(r))
+        return r
+
+    def apply_t(self, x, r):
+        call("hala_ilu_apply_t_f64", self.handle, _ptr(x), _ptr(r))
+        return r
+
+    def symbolic(self, upper=False):
+        torch = _torch()
+        dev = self.pntr.device
+        inf = self.info()
+        nlev = inf["nlev_upper" if upper else "nlev_lower"]
+        diag = torch.empty(self.n, dtype=torch.int32, device=dev)
+        lev = torch.empty(self.n, dtype=torch.int32, device=dev)
+        lptr = torch.empty(nlev + 1, dtype=torch.int32, device=dev)
+        perm = torch.empty(self.n, dtype=torch.int32, device=dev)
+        call("hala_ilu_get_symbolic", self.handle, int(bool(upper)), _ptr(diag), _ptr(lev), _ptr(lptr), _ptr(perm))
+        return diag, lev, lptr, perm
+
+    def get_factor(self):
+        torch = _torch()
+        f = torch.empty(self.nnz, dtype=torch.float64, device=self.pntr.device)
+        call("hala_ilu_get_factor_f64", self.handle, _ptr(f))
+        return f
+
+
+def make_ilu(engine, pntr, indx, vals, policy="N", want_f32=False):
+    """make_ilu(gpu_engine, pntr, indx, vals, policy): gpu/hala_gpu_ilu.hpp:191-195."""
+    return gpu_ilu(engine, pntr, indx, vals, policy, want_f32)
+
+
+class gpu_triangular_matrix:
+    """gpu/hala_rocm_sparse_triangular.hpp:29-178 with sparse_trsv_array semantics."""
+
+    def __init__(self, engine, uplo, diag, pntr, indx, vals):
+        self.engine = engine
+        self.pntr, self.indx, self.vals = pntr, indx, vals
+        self.n = int(pntr.numel()) - 1
+        h = C.c_void_p()
+        call("hala_tri_create", engine.handle, uplo.encode(), diag.encode(), self.n, int(indx.numel()), _ptr(pntr),
+             _ptr(indx), C.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None:
+            lib().hala_tri_destroy(self.handle)
+            self.handle = None
+
+    def trsv(self, trans, alpha, b, x):
+        call("hala_trsv_f64", self.handle, trans.encode(), float(alpha), _ptr(self.vals), _ptr(b), _ptr(x))
+        return x
+
+
+def make_triangular_matrix(engine, uplo, diag, pntr, indx, vals):
+    return gpu_triangular_matrix(engine, uplo, diag, pntr, indx, vals)
+
+
+# ---- BLAS-1 on the engine (gpu/hala_gpu_blas1.hpp) ---------------------------
+def dot(engine, x, y):
+    if _dt(x) == 64:
+        r = C.c_double()
+        call("hala_dot_f64", engine.handle, int(x.numel()), _ptr(x), _ptr(y), C.byref(r))
+    else:
+        r = C.c_float()
+        call("hala_dot_f32", engine.handle, int(x.numel()), _ptr(x), _ptr(y), C.byref(r))
+    return r.value
+
+
+def norm2(engine, x):
+    if _dt(x) == 64:
+        r = C.c_double()
+        call("hala_nrm2_f64", engine.handle, int(x.numel()), _ptr(x), C.byref(r))
+    else:
+        r = C.c_float()
+        call("hala_nrm2_f32", engine.handle, int(x.numel()), _ptr(x), C.byref(r))
+    return r.value
+
+
+def axpy(engine, alpha, x, y):
+    f = "hala_axpy_f64" if _dt(x) == 64 else "hala_axpy_f32"
+    call(f, engine.handle, int(x.numel()), float(alpha), _ptr(x), _ptr(y))
+    return y
+
+
+def scal(engine, alpha, x):
+    f = "hala_scal_f64" if _dt(x) == 64 else "hala_scal_f32"
+    call(f, engine.handle, int(x.numel()), float(alpha), _ptr(x))
+    return x
+
+
+def vcopy(engine, x, y):
+    f = "hala_copy_f64" if _dt(x) == 64 else "hala_copy_f32"
+    call(f, engine.handle, int(x.numel()), _ptr(x), _ptr(y))
+    return y
+
+
+# ---- solvers (hex/solvers) ---------------------------------------------------
+def _stop(stop):
+    return stop if isinstance(stop, stop_criteria) else stop_criteria(stop)
+
+
+def solve_cg(engine, stop, pntr, indx, vals, precon, b, x, hist=None, matrix=None):
+    """solve_cg(engine, stop, pntr, indx, vals, precon, b, x): hex/solvers/hala_solvers_cg.hpp:230-244.
+
+    precon: None (identity copy), a gpu_ilu, or ("jacobi", diagonal_tensor).
+    Returns the iteration count (operator applications, counted from 1).
+    """
+    stop = _stop(stop)
+    A = matrix or gpu_sparse_matrix(engine, int(pntr.numel()) - 1, int(pntr.numel()) - 1, pntr, indx, vals)
+    it = C.c_int()
+    if precon is None:
+        kind, ilu, dg = 0, None, None
+    elif isinstance(precon, gpu_ilu):
+        kind, ilu, dg = 1, precon, None
+    else:
+        kind, ilu, dg = 2, None, precon[1]
+    nh = int(hist.numel()) if hist is not None else 0
+    call("hala_cg_f64", A.handle, _ptr(vals), kind, ilu.handle if ilu else C.c_void_p(0), _ptr(dg), _ptr(b),
+         _ptr(x), stop.tol, stop.max_iter, C.byref(it), _ptr(hist), nh)
+    return it.value
+
+
+def solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x, hist=None, matrix=None):
+    """solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x): hex/solvers/hala_solvers_cg.hpp:312-326."""
+    if ilu is None:
+        ilu = make_ilu(engine, pntr, indx, vals)
+    return solve_cg(engine, stop, pntr, indx, vals, ilu, b, x, hist=hist, matrix=matrix)
+
+
+def solve_cg_t(engine, stop, matrix, vals, ilu, b, x, hist=None):
+    """Adjoint solve A^T x = b preconditioned with (LU)^-T."""
+    stop = _stop(stop)
+    it = C.c_int()
+    nh = int(hist.numel()) if hist is not None else 0
+    call("hala_cg_t_f64", matrix.handle, _ptr(vals), ilu.handle, _ptr(b), _ptr(x), stop.tol, stop.max_iter,
+         C.byref(it), _ptr(hist), nh)
+    return it.value
+
+
+def solve_gmres(engine, stop, restart, pntr, indx, vals, precon, b, x, hist=None, matrix=None):
+    """solve_gmres(engine, stop, restart, ...): hex/solvers/hala_solvers_gmres.hpp:125-230.
+
+    precon: None (identity) or a gpu_ilu.  stop.max_iter counts restart cycles.
+    Returns (preconditioner applications, number of Givens estimates recorded).
+    """
+    stop = _stop(stop)
+    A = matrix or gpu_sparse_matrix(engine, int(pntr.numel()) - 1, int(pntr.numel()) - 1, pntr, indx, vals)
+    tot, nh = C.c_int(), C.c_int()
+    kind = 0 if precon is None else 1
+    mh = int(hist.numel()) if hist is not None else 0
+    call("hala_gmres_f64", A.handle, _ptr(vals), kind, precon.handle if precon is not None else C.c_void_p(0),
+         _ptr(b), _ptr(x), stop.tol, int(restart), stop.max_iter, C.byref(tot), _ptr(hist), mh, C.byref(nh))
+    return tot.value, nh.value
+
+
+def solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x, hist=None, matrix=None):
+    """solve_gmres_ilu: hex/solvers/hala_solvers_gmres.hpp:250-267."""
+    if ilu is None:
+        ilu = make_ilu(engine, pntr, indx, vals)
+    return solve_gmres(engine, stop, restart, pntr, indx, vals, ilu, b, x, hist=hist, matrix=matrix)
+
+
+def solve_gmres_ir(engine, matrix, vals, vals32, ilu, b, x, rtol=1e-10, inner_rtol=1e-4, restart=30,
+                   max_refine=50, hist=None):
+    """FP32 ILU(0) + FP32 product inside GMRES(restart), FP64 refinement to ||b-Ax|| < rtol ||b||.
+
+    Returns (inner preconditioner applications, outer residuals recorded).
+    """
+    tot, nh = C.c_int(), C.c_int()
+    mh = int(hist.numel()) if hist is not None else 0
+    call("hala_gmres_ir_f64", matrix.handle, _ptr(vals), _ptr(vals32), ilu.handle, _ptr(b), _ptr(x), float(rtol),
+         float(inner_rtol), int(restart), int(max_refine), C.byref(tot), _ptr(hist), mh, C.byref(nh))
+    return tot.value, nh.value

@@ -1,0 +1,175 @@
+// ctx.hip -- engine (device + stream), device memory, fills and conversions.
+#include "common.hpp"
+
+namespace hala_amd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+template <class T>
+__global__ void fill_kernel(T *__restrict__ x, int64_t n, T v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) x[i] = v;
+}
+
+template <class A, class B>
+__global__ void convert_kernel(const A *__restrict__ x, B *__restrict__ y, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) y[i] = (B)x[i];
+}
+
+template <class T>
+static void fill(hala_ctx c, T *x, int64_t n, T v) {
+    HALA_REQUIRE(c && n >= 0 && (x || n == 0), HALA_ERR_ARG, "fill: bad argument");
+    if (n == 0) return;
+    set_device(c);
+    int g = grid_for(n, 256 * 4, 8192);
+    fill_kernel<T><<<g, 256, 0, c->stream>>>(x, n, v);
+    HALA_CHECK_LAUNCH();
+}
+
+template <class A, class B>
+static void convert(hala_ctx c, int64_t n, const A *x, B *y) {
+    HALA_REQUIRE(c && n >= 0 && ((x && y) || n == 0), HALA_ERR_ARG, "convert: bad argument");
+    if (n == 0) return;
+    set_device(c);
+    int g = grid_for(n, 256 * 4, 8192);
+    convert_kernel<A, B><<<g, 256, 0, c->stream>>>(x, y, n);
+    HALA_CHECK_LAUNCH();
+}
+
+}  // namespace hala_amd
+
+using namespace hala_amd;
+
+extern "C" {
+
+const char *hala_last_error(void) { return g_last_error.c_str(); }
+
+int hala_version(int *major, int *minor) {
+    if (major) *major = 1;
+    if (minor) *minor = 0;
+    return HALA_OK;
+}
+
+int hala_device_count(int *count) {
+    return guard([&] {
+        HALA_REQUIRE(count, HALA_ERR_ARG, "hala_device_count: null count");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        *count = (e == hipSuccess) ? c : 0;
+    });
+}
+
+int hala_ctx_create(int device, void *stream, hala_ctx *out) {
+    return guard([&] {
+        HALA_REQUIRE(out, HALA_ERR_ARG, "hala_ctx_create: null output");
+        *out = nullptr;
+        int count = 0;
+        HALA_HIP(hipGetDeviceCount(&count));
+        HALA_REQUIRE(device >= 0 && device < count, HALA_ERR_ARG,
+                     "hala_ctx_create: device " + std::to_string(device) + " out of range");
+        HALA_HIP(hipSetDevice(device));
+        auto *c = new hala_ctx_s();
+        c->device = device;
+        c->stream = (hipStream_t)stream;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+        try {
+            c->scal.reserve(S_TOTAL * sizeof(double));
+            HALA_HIP(hipMemsetAsync(c->scal.p, 0, S_TOTAL * sizeof(double), c->stream));
+            c->red.reserve(64 * 1024 * sizeof(double));
+            HALA_HIP(hipHostMalloc((void **)&c->host_pinned, 1024 * sizeof(double), hipHostMallocDefault));
+            HALA_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int hala_ctx_destroy(hala_ctx c) {
+    return guard([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+        delete c;
+    });
+}
+
+int hala_ctx_set_stream(hala_ctx c, void *stream) {
+    return guard([&] {
+        HALA_REQUIRE(c, HALA_ERR_ARG, "null context");
+        set_device(c);
+        HALA_HIP(hipStreamSynchronize(c->stream));
+        c->stream = (hipStream_t)stream;
+    });
+}
+
+int hala_ctx_sync(hala_ctx c) {
+    return guard([&] {
+        HALA_REQUIRE(c, HALA_ERR_ARG, "null context");
+        set_device(c);
+        HALA_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hala_ctx_device(hala_ctx c, int *device) {
+    return guard([&] {
+        HALA_REQUIRE(c && device, HALA_ERR_ARG, "null argument");
+        *device = c->device;
+    });
+}
+
+int hala_malloc(hala_ctx c, size_t bytes, void **ptr) {
+    return guard([&] {
+        HALA_REQUIRE(c && ptr, HALA_ERR_ARG, "hala_malloc: null argument");
+        set_device(c);
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        hipError_t e = hipMalloc(ptr, bytes);
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            throw status_error{HALA_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        }
+    });
+}
+
+int hala_free(hala_ctx c, void *ptr) {
+    return guard([&] {
+        HALA_REQUIRE(c, HALA_ERR_ARG, "hala_free: null context");
+        if (!ptr) return;
+        set_device(c);
+        HALA_HIP(hipStreamSynchronize(c->stream));
+        HALA_HIP(hipFree(ptr));
+    });
+}
+
+int hala_memcpy(hala_ctx c, void *dst, const void *src, size_t bytes, int kind) {
+    return guard([&] {
+        HALA_REQUIRE(c && kind >= 0 && kind <= 2, HALA_ERR_ARG, "hala_memcpy: bad argument");
+        if (bytes == 0) return;
+        HALA_REQUIRE(dst && src, HALA_ERR_ARG, "hala_memcpy: null pointer");
+        set_device(c);
+        hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        HALA_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+        HALA_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hala_fill_f64(hala_ctx c, double *x, int64_t n, double v) { return guard([&] { fill<double>(c, x, n, v); }); }
+int hala_fill_f32(hala_ctx c, float *x, int64_t n, float v) { return guard([&] { fill<float>(c, x, n, v); }); }
+int hala_fill_i32(hala_ctx c, int *x, int64_t n, int v) { return guard([&] { fill<int>(c, x, n, v); }); }
+int hala_convert_f64_f32(hala_ctx c, int64_t n, const double *x, float *y) {
+    return guard([&] { convert<double, float>(c, n, x, y); });
+}
+int hala_convert_f32_f64(hala_ctx c, int64_t n, const float *x, double *y) {
+    return guard([&] { convert<float, double>(c, n, x, y); });
+}
+
+}  // extern "C"

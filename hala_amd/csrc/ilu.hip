@@ -1,0 +1,275 @@
+// ilu.hip -- ILU(0) on the device: symbolic analysis, numeric factorisation
+// level by level, and the triangular applies (replacing rocsparse_?csrilu0
+// and rocsparse_?csrsv of gpu/hala_gpu_ilu.hpp:81-172).
+#include <climits>
+#include <cmath>
+
+#include "ilu.hpp"
+
+namespace hala_amd {
+
+// diag[i] = first position in row i with column >= i (sparse/hala_sparse_core.hpp:45-60)
+__global__ void diag_kernel(int n, const int *__restrict__ pntr, const int *__restrict__ indx, int *__restrict__ diag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = pntr[i];
+    while (d < pntr[i + 1] && indx[d] < i) d++;
+    diag[i] = d;
+}
+
+// IKJ ILU(0) of the rows of one level: for each strictly lower entry (j,i) in
+// column order, a_ji /= u_ii, then a_jk -= a_ji * u_ik over the columns k > i
+// both rows hold (merge of the sorted rows).  Rows i of earlier levels are
+// final, so each entry sees exactly the operation sequence of the reference's
+// KIJ loop (sparse/hala_sparse_utils.hpp:229-254).
+__global__ void ilu_level_kernel(const int *__restrict__ rows, int count, const int *__restrict__ pntr,
+                                 const int *__restrict__ indx, const int *__restrict__ diag, double *f) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int j = rows[t];
+    const int je = pntr[j + 1];
+    const int dj = diag[j];
+    for (int jc = pntr[j]; jc < dj; jc++) {
+        const int i = indx[jc];
+        const double l = f[jc] / f[diag[i]];
+        f[jc] = l;
+        int ik = diag[i] + 1, jk = jc + 1;
+        const int ie = pntr[i + 1];
+        while (ik < ie && jk < je) {
+            const int ci = indx[ik], cj = indx[jk];
+            if (ci == cj) {
+                f[jk] = f[jk] - l * f[ik];
+                ik++;
+                jk++;
+            } else if (ci < cj) {
+                ik++;
+            } else {
+                jk++;
+            }
+        }
+    }
+}
+
+__global__ void pivot_check(int n, const int *__restrict__ diag, const double *__restrict__ f, int *bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u = f[diag[i]];
+    if (u == 0.0 || !isfinite(u)) atomicMin(bad, i);
+}
+
+__global__ void set_int(int *p, int v) { *p = v; }
+
+void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
+    hala_ctx c = M->ctx;
+    HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
+    tri_exec<double, double, double>(c, M->L, TRI_SUB, 1.0, x, M->tmp64.as<double>(), M->U.out64.as<double>(),
+                                     M->U.slot_of_row.as<int>(), gate);
+    tri_exec<double, double, double>(c, M->U, TRI_SUB, 1.0, M->tmp64.as<double>(), r, M->L.out64.as<double>(),
+                                     M->L.slot_of_row.as<int>(), gate);
+}
+
+void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate) {
+    hala_ctx c = M->ctx;
+    HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs hala_ilu_factor_f64(..., want_f32=1)");
+    tri_exec<float, double, double>(c, M->L, TRI_SUB, 1.0f, x, M->tmp64.as<double>(), M->U.out32.as<float>(),
+                                    M->U.slot_of_row.as<int>(), gate);
+    tri_exec<float, double, double>(c, M->U, TRI_SUB, 1.0f, M->tmp64.as<double>(), r, M->L.out32.as<float>(),
+                                    M->L.slot_of_row.as<int>(), gate);
+}
+
+void ilu_prepare_adjoint(hala_ilu M) {
+    hala_ctx c = M->ctx;
+    if (!M->adjoint) {
+        if (!M->tp.built) build_transpose(c, M->n, M->n, M->nnz, M->pntr, M->indx, M->tp);
+        // U^T w = x: row k of U^T lists U_ik, i < k, ascending i; diagonal U_kk
+        build_deps(c, M->Ut, M->n, DEP_T_LOWER, M->pntr, M->indx, M->diag.as<int>(), &M->tp, false, false);
+        build_schedule(c, M->Ut, false);
+        // L^T y = w: row k of L^T lists L_ik, i > k, descending i; unit diagonal
+        build_deps(c, M->Lt, M->n, DEP_T_UPPER, M->pntr, M->indx, M->diag.as<int>(), &M->tp, true, true);
+        build_schedule(c, M->Lt, false);
+        M->adjoint = true;
+    }
+    if (!M->adjoint_loaded) {
+        load_values64(c, M->Ut, M->factor.as<double>());
+        load_values64(c, M->Lt, M->factor.as<double>());
+        M->adjoint_loaded = true;
+    }
+}
+
+void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate) {
+    hala_ctx c = M->ctx;
+    HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
+    ilu_prepare_adjoint(M);
+    tri_exec<double, double, double>(c, M->Ut, TRI_SUB, 1.0, x, M->tmp64.as<double>(), M->Lt.out64.as<double>(),
+                                     M->Lt.slot_of_row.as<int>(), gate);
+    tri_exec<double, double, double>(c, M->Lt, TRI_SUB, 1.0, M->tmp64.as<double>(), r, M->Ut.out64.as<double>(),
+                                     M->Ut.slot_of_row.as<int>(), gate);
+}
+
+}  // namespace hala_amd
+
+using namespace hala_amd;
+
+extern "C" {
+
+int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *indx, hala_ilu *out) {
+    return guard([&] {
+        HALA_REQUIRE(c && out, HALA_ERR_ARG, "hala_ilu_create: null argument");
+        *out = nullptr;
+        HALA_REQUIRE(n > 0 && nnz > 0 && nnz < INT_MAX && pntr && indx, HALA_ERR_ARG, "hala_ilu_create: bad sizes");
+        set_device(c);
+        int bad = validate_pattern(c, n, n, nnz, pntr, indx, 1);
+        HALA_REQUIRE(bad < 0, HALA_ERR_PATTERN,
+                     "hala_ilu_create: row " + std::to_string(bad) +
+                         " has unsorted/out-of-range columns or no diagonal entry");
+        auto *M = new hala_ilu_s();
+        try {
+            M->ctx = c;
+            M->n = n;
+            M->nnz = nnz;
+            M->pntr = pntr;
+            M->indx = indx;
+            M->diag.reserve((size_t)n * sizeof(int));
+            diag_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, pntr, indx, M->diag.as<int>());
+            HALA_CHECK_LAUNCH();
+            build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
+            build_schedule(c, M->L, false);
+            build_deps(c, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
+            build_schedule(c, M->U, false);
+            M->factor.reserve((size_t)nnz * sizeof(double));
+            M->tmp64.reserve((size_t)n * sizeof(double));
+        } catch (...) {
+            delete M;
+            throw;
+        }
+        *out = M;
+    });
+}
+
+int hala_ilu_destroy(hala_ilu M) {
+    return guard([&] {
+        if (!M) return;
+        (void)hipSetDevice(M->ctx->device);
+        (void)hipStreamSynchronize(M->ctx->stream);
+        delete M;
+    });
+}
+
+int hala_ilu_info(hala_ilu M, hala_ilu_info_t *info) {
+    return guard([&] {
+        HALA_REQUIRE(M && info, HALA_ERR_ARG, "hala_ilu_info: null argument");
+        info->n = M->n;
+        info->nnz = M->nnz;
+        info->nlev_lower = M->L.nlev;
+        info->nlev_upper = M->U.nlev;
+        info->nchunk_lower = M->L.nchunk;
+        info->nchunk_upper = M->U.nchunk;
+        info->slots_lower = M->L.nent;
+        info->slots_upper = M->U.nent;
+    });
+}
+
+int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
+    return guard([&] {
+        HALA_REQUIRE(M && vals, HALA_ERR_ARG, "hala_ilu_factor_f64: null argument");
+        hala_ctx c = M->ctx;
+        set_device(c);
+        double *f = M->factor.as<double>();
+        HALA_HIP(hipMemcpyAsync(f, vals, (size_t)M->nnz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        const std::vector<int> &lp = M->L.h_level_ptr;
+        const int *perm = M->L.perm.as<int>();
+        for (int l = 1; l < M->L.nlev; l++) {  // level 0 rows have no strictly lower entries
+            int cnt = lp[l + 1] - lp[l];
+            if (cnt <= 0) continue;
+            ilu_level_kernel<<<(cnt + 255) / 256, 256, 0, c->stream>>>(perm + lp[l], cnt, M->pntr, M->indx,
+                                                                      M->diag.as<int>(), f);
+            HALA_CHECK_LAUNCH();
+        }
+        int *bad = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 6);
+        set_int<<<1, 1, 0, c->stream>>>(bad, INT_MAX);
+        pivot_check<<<(M->n + 255) / 256, 256, 0, c->stream>>>(M->n, M->diag.as<int>(), f, bad);
+        HALA_CHECK_LAUNCH();
+        int b = fetch_i32(c, bad);
+        M->factored = false;
+        M->adjoint_loaded = false;
+        HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
+                     "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
+        load_values64(c, M->L, f);
+        load_values64(c, M->U, f);
+        if (want_f32) {
+            load_values32(c, M->L, f);
+            load_values32(c, M->U, f);
+            M->has32 = true;
+        }
+        M->factored = true;
+        HALA_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hala_ilu_apply_f64(hala_ilu M, const double *x, double *r) {
+    return guard([&] {
+        HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f64: null argument");
+        set_device(M->ctx);
+        ilu_apply64(M, x, r);
+    });
+}
+
+int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
+    return guard([&] {
+        HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f32: null argument");
+        HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs want_f32 at factorisation");
+        hala_ctx c = M->ctx;
+        set_device(c);
+        M->tmp32.reserve((size_t)M->n * sizeof(float));
+        tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>(), M->U.out32.as<float>(),
+                                      M->U.slot_of_row.as<int>());
+        tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r, M->L.out32.as<float>(),
+                                      M->L.slot_of_row.as<int>());
+    });
+}
+
+int hala_ilu_apply_f32x64(hala_ilu M, const double *x, double *r) {
+    return guard([&] {
+        HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f32x64: null argument");
+        set_device(M->ctx);
+        ilu_apply32x64(M, x, r);
+    });
+}
+
+int hala_ilu_apply_t_f64(hala_ilu M, const double *x, double *r) {
+    return guard([&] {
+        HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_t_f64: null argument");
+        set_device(M->ctx);
+        ilu_apply_t64(M, x, r);
+    });
+}
+
+int hala_ilu_get_symbolic(hala_ilu M, int upper, int *diag, int *lev, int *level_ptr, int *perm) {
+    return guard([&] {
+        HALA_REQUIRE(M, HALA_ERR_ARG, "hala_ilu_get_symbolic: null handle");
+        hala_ctx c = M->ctx;
+        set_device(c);
+        tri_plan &p = upper ? M->U : M->L;
+        auto cp = [&](int *dst, const dbuf &src, size_t count) {
+            if (dst) HALA_HIP(hipMemcpyAsync(dst, src.p, count * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+        };
+        cp(diag, M->diag, M->n);
+        cp(lev, p.lev, M->n);
+        cp(level_ptr, p.level_ptr, (size_t)p.nlev + 1);
+        cp(perm, p.perm, M->n);
+        HALA_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int hala_ilu_get_factor_f64(hala_ilu M, double *factor) {
+    return guard([&] {
+        HALA_REQUIRE(M && factor, HALA_ERR_ARG, "hala_ilu_get_factor_f64: null argument");
+        hala_ctx c = M->ctx;
+        set_device(c);
+        HALA_HIP(hipMemcpyAsync(factor, M->factor.p, (size_t)M->nnz * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+        HALA_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,42 @@
+// ilu.hpp -- ILU(0) handle (the gpu_ilu of gpu/hala_gpu_ilu.hpp:45-195).
+#pragma once
+#include "spmv.hpp"
+#include "trsolve.hpp"
+
+struct hala_ilu_s {
+    hala_ctx ctx = nullptr;
+    int n = 0;
+    int64_t nnz = 0;
+    const int *pntr = nullptr;  // aliased, as the reference's gpu_ilu
+    const int *indx = nullptr;
+    hala_amd::dbuf diag;       // diag[i]: position of a_ii
+    hala_amd::dbuf factor;     // combined L\U values (CSR order), FP64
+    hala_amd::tri_plan L, U;   // forward (unit L) and backward (U) solves
+    hala_amd::tri_plan Ut, Lt; // adjoint: U^T forward, L^T backward (built on demand)
+    hala_amd::csr_transpose tp;
+    hala_amd::dbuf tmp64, tmp32;  // natural-order hand-off between the two solves
+    bool factored = false, has32 = false, adjoint = false, adjoint_loaded = false;
+};
+
+struct hala_tri_s {
+    hala_ctx ctx = nullptr;
+    int n = 0;
+    int64_t nnz = 0;
+    const int *pntr = nullptr;
+    const int *indx = nullptr;
+    char uplo = 'L', diag = 'N';
+    hala_amd::tri_plan N, T;  // plain and transposed solves (T built on demand)
+    hala_amd::csr_transpose tp;
+    bool t_built = false;
+    const double *last_vals_n = nullptr, *last_vals_t = nullptr;
+    hala_amd::dbuf vals_copy_n, vals_copy_t;  // unused placeholders for future caching
+};
+
+namespace hala_amd {
+// r = (LU)^-1 x in the requested precision (see hala_ilu_apply_*)
+// gate (nullable, device flag): skip the apply when *gate != 0 (solver stop flag)
+void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
+void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
+void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
+void ilu_prepare_adjoint(hala_ilu M);
+}  // namespace hala_amd

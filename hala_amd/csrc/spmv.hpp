@@ -1,0 +1,84 @@
+// spmv.hpp -- CSR matrix handle and the sparse matrix-vector product cores.
+#pragma once
+#include "util.hpp"
+
+struct hala_csr_s {
+    hala_ctx ctx = nullptr;
+    int nrows = 0, ncols = 0;
+    int64_t nnz = 0;
+    const int *pntr = nullptr;  // device, not owned
+    const int *indx = nullptr;  // device, not owned
+    int max_row = 0, min_row = 0;
+    double mean_row = 0.0;
+    int kernel = 0;  // 0: LDS-staged block of rows, 1: wavefront per row
+    hala_amd::csr_transpose tp;  // built on first transposed product
+};
+
+namespace hala_amd {
+
+constexpr int SPMV_ROWS = 256;  // rows per block in the LDS-staged kernel (= canonical block)
+
+template <class TA>
+struct spmv_tile {
+    static constexpr int value = sizeof(TA) == 8 ? 2048 : 4096;  // products staged in LDS per pass
+};
+
+// LDS-staged product of one block of SPMV_ROWS consecutive rows.  The block's
+// nonzeros are streamed in coalesced tiles: every thread forms products
+// vals[j] * x[indx[j]] into LDS, then each thread adds the products of its own
+// row in CSR order.  Returns the row sum (TA(0) for rows past the end).
+// Must be called by all 256 threads of the block.
+template <class TV, class TX, class TA>
+__device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
+                                               const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod) {
+    constexpr int TILE = spmv_tile<TA>::value;
+    const int r0 = blockIdx.x * SPMV_ROWS;
+    const int row = r0 + threadIdx.x;
+    const int rlast = min(r0 + SPMV_ROWS, nrows);
+    const int bstart = pntr[r0];
+    const int bend = pntr[rlast];
+    int rs = 0, re = 0;
+    if (row < nrows) {
+        rs = pntr[row];
+        re = pntr[row + 1];
+    }
+    TA acc = TA(0);
+    for (int base = bstart; base < bend; base += TILE) {
+        const int cnt = min(TILE, bend - base);
+        int e = threadIdx.x;
+        for (; e + 3 * SPMV_ROWS < cnt; e += 4 * SPMV_ROWS) {
+            const int j = base + e;
+            int c0 = indx[j], c1 = indx[j + SPMV_ROWS], c2 = indx[j + 2 * SPMV_ROWS], c3 = indx[j + 3 * SPMV_ROWS];
+            TV v0 = vals[j], v1 = vals[j + SPMV_ROWS], v2 = vals[j + 2 * SPMV_ROWS], v3 = vals[j + 3 * SPMV_ROWS];
+            TA x0 = (TA)x[c0], x1 = (TA)x[c1], x2 = (TA)x[c2], x3 = (TA)x[c3];
+            prod[e] = (TA)v0 * x0;
+            prod[e + SPMV_ROWS] = (TA)v1 * x1;
+            prod[e + 2 * SPMV_ROWS] = (TA)v2 * x2;
+            prod[e + 3 * SPMV_ROWS] = (TA)v3 * x3;
+        }
+        for (; e < cnt; e += SPMV_ROWS) {
+            const int j = base + e;
+            prod[e] = (TA)vals[j] * (TA)x[indx[j]];
+        }
+        __syncthreads();
+        const int a = max(rs, base), b = min(re, base + cnt);
+        for (int j = a; j < b; j++) acc = acc + prod[j - base];
+        __syncthreads();
+    }
+    return acc;
+}
+
+// First row of an invalid pattern (-1 when valid): pointers must start at 0,
+// end at nnz and not decrease; columns sorted, unique, inside [0, ncols);
+// require_diag: every row holds its diagonal.
+int validate_pattern(hala_ctx c, int nrows, int ncols, int64_t nnz, const int *pntr, const int *indx,
+                     int require_diag);
+
+// Launch helpers (defined in spmv.hip)
+template <class TV, class TX, class TY>
+void spmv_axpby(hala_csr A, const TV *vals, const TX *x, TY alpha, TY beta, TY *y, bool f32_inner);
+template <class T>
+void spmv_trans(hala_csr A, T alpha, const T *vals, const T *x, T beta, T *y);
+void ensure_transpose(hala_csr A);
+
+}  // namespace hala_amd

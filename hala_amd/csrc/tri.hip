@@ -1,0 +1,113 @@
+// tri.hip -- standalone sparse triangular matrix (gpu_triangular_matrix of
+// gpu/hala_rocm_sparse_triangular.hpp:29-194) with the semantics of
+// sparse_trsv_array (sparse/hala_sparse_utils.hpp:284-335): the CSR holds one
+// triangle, the diagonal being the last ('L') or first ('U') entry of a row.
+#include <climits>
+
+#include "ilu.hpp"
+
+namespace hala_amd {
+
+// row's diagonal is the last (lower) / first (upper) entry and equals the row index
+__global__ void tri_check(int n, int lower, const int *__restrict__ pntr, const int *__restrict__ indx,
+                          int *__restrict__ diagpos, int *bad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = pntr[i], e = pntr[i + 1];
+    bool ok = e > s;
+    int d = -1;
+    if (ok) {
+        d = lower ? e - 1 : s;
+        ok = indx[d] == i;
+        if (lower) ok = ok && (s == d || indx[s] < i);
+        else ok = ok && (e - 1 == d || indx[e - 1] > i);
+    }
+    diagpos[i] = d < 0 ? s : d;
+    if (!ok) atomicMin(bad, i);
+}
+
+__global__ void set_i32_tri(int *p, int v) { *p = v; }
+
+}  // namespace hala_amd
+
+using namespace hala_amd;
+
+extern "C" {
+
+int hala_tri_create(hala_ctx c, char uplo, char diag, int n, int64_t nnz, const int *pntr, const int *indx,
+                    hala_tri *out) {
+    return guard([&] {
+        HALA_REQUIRE(c && out, HALA_ERR_ARG, "hala_tri_create: null argument");
+        *out = nullptr;
+        bool lower = (uplo == 'L' || uplo == 'l');
+        HALA_REQUIRE(lower || uplo == 'U' || uplo == 'u', HALA_ERR_ARG, "hala_tri_create: uplo must be L or U");
+        bool unit = (diag == 'U' || diag == 'u');
+        HALA_REQUIRE(unit || diag == 'N' || diag == 'n', HALA_ERR_ARG, "hala_tri_create: diag must be U or N");
+        HALA_REQUIRE(n > 0 && nnz > 0 && nnz < INT_MAX && pntr && indx, HALA_ERR_ARG, "hala_tri_create: bad sizes");
+        set_device(c);
+        int bad = validate_pattern(c, n, n, nnz, pntr, indx, 1);
+        HALA_REQUIRE(bad < 0, HALA_ERR_PATTERN, "hala_tri_create: invalid pattern at row " + std::to_string(bad));
+        auto *T = new hala_tri_s();
+        try {
+            T->ctx = c;
+            T->n = n;
+            T->nnz = nnz;
+            T->pntr = pntr;
+            T->indx = indx;
+            T->uplo = lower ? 'L' : 'U';
+            T->diag = unit ? 'U' : 'N';
+            dbuf dpos;
+            dpos.reserve((size_t)n * sizeof(int));
+            int *flag = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 7);
+            set_i32_tri<<<1, 1, 0, c->stream>>>(flag, INT_MAX);
+            tri_check<<<(n + 255) / 256, 256, 0, c->stream>>>(n, lower ? 1 : 0, pntr, indx, dpos.as<int>(), flag);
+            HALA_CHECK_LAUNCH();
+            int b = fetch_i32(c, flag);
+            HALA_REQUIRE(b == INT_MAX, HALA_ERR_PATTERN,
+                         "hala_tri_create: row " + std::to_string(b) + " is not " +
+                             (lower ? "lower triangular with the diagonal last" : "upper triangular with the diagonal first"));
+            build_deps(c, T->N, n, lower ? DEP_ROW_BUT_LAST : DEP_ROW_BUT_FIRST, pntr, indx, dpos.as<int>(), nullptr,
+                       unit, !lower);
+            build_schedule(c, T->N, false);
+            // transposed solve: L^T is upper (backward, source rows descending), U^T lower (forward, ascending)
+            build_transpose(c, n, n, nnz, pntr, indx, T->tp);
+            build_deps(c, T->T, n, lower ? DEP_T_UPPER : DEP_T_LOWER, pntr, indx, dpos.as<int>(), &T->tp, unit, lower);
+            build_schedule(c, T->T, false);
+            T->t_built = true;
+        } catch (...) {
+            delete T;
+            throw;
+        }
+        *out = T;
+    });
+}
+
+int hala_tri_destroy(hala_tri T) {
+    return guard([&] {
+        if (!T) return;
+        (void)hipSetDevice(T->ctx->device);
+        (void)hipStreamSynchronize(T->ctx->stream);
+        delete T;
+    });
+}
+
+// x = op(T)^-1 (alpha b): sum of the off-diagonal terms first, then
+// (alpha*b_i - s) [/ diag]; the transposed sums add the terms in the source-row
+// order of the reference's column sweep.
+int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, const double *b, double *x) {
+    return guard([&] {
+        HALA_REQUIRE(T && vals && b && x, HALA_ERR_ARG, "hala_trsv_f64: null argument");
+        hala_ctx c = T->ctx;
+        set_device(c);
+        bool tr = !(trans == 'N' || trans == 'n');
+        HALA_REQUIRE(!tr || trans == 'T' || trans == 't' || trans == 'C' || trans == 'c', HALA_ERR_ARG,
+                     "hala_trsv_f64: trans must be N, T or C");
+        tri_plan &p = tr ? T->T : T->N;
+        load_values64(c, p, vals);
+        tri_rearm(c, p);
+        tri_exec<double, double, double>(c, p, TRI_SUM, alpha, b, x, nullptr, nullptr);
+        check_wait_error(c);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,442 @@
+// trsolve.hip -- plan construction and sync-free level-ordered execution.
+#include <climits>
+
+#include "trsolve.hpp"
+
+namespace hala_amd {
+
+constexpr int WAVE = 64;
+constexpr unsigned SPIN_LIMIT = 1u << 22;  // polls before a wait gives up (a few seconds)
+
+int *wait_error_flag(hala_ctx c) { return reinterpret_cast<int *>(counters(c) + 120); }
+
+void check_wait_error(hala_ctx c) {
+    int v = fetch_i32(c, wait_error_flag(c));
+    if (v != 0) {
+        HALA_HIP(hipMemsetAsync(wait_error_flag(c), 0, sizeof(int), c->stream));
+        HALA_HIP(hipStreamSynchronize(c->stream));
+        throw status_error{HALA_ERR_NOCONV, "a triangular-solve dependency wait gave up (corrupt schedule?)"};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dependency lists
+// ---------------------------------------------------------------------------
+__global__ void dep_count(int n, int sel, const int *__restrict__ pntr, const int *__restrict__ diagpos,
+                          const int *__restrict__ tpntr, const int *__restrict__ trow, int *__restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    switch (sel) {
+    case DEP_LOWER: c = diagpos[i] - pntr[i]; break;
+    case DEP_UPPER: c = pntr[i + 1] - diagpos[i] - 1; break;
+    case DEP_ROW_BUT_LAST:
+    case DEP_ROW_BUT_FIRST: c = max(pntr[i + 1] - pntr[i] - 1, 0); break;
+    default:
+        for (int e = tpntr[i]; e < tpntr[i + 1]; e++) {
+            int r = trow[e];
+            if (sel == DEP_T_LOWER ? (r < i) : (r > i)) c++;
+        }
+    }
+    cnt[i] = c;
+}
+
+__global__ void dep_fill(int n, int sel, const int *__restrict__ pntr, const int *__restrict__ indx,
+                         const int *__restrict__ diagpos, const int *__restrict__ tpntr,
+                         const int *__restrict__ trow, const int *__restrict__ tmap, const int *__restrict__ dptr,
+                         int *__restrict__ ddep, int *__restrict__ dsrc, int *__restrict__ dgs, int unit) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int q = dptr[i];
+    switch (sel) {
+    case DEP_LOWER:
+        for (int j = pntr[i]; j < diagpos[i]; j++, q++) ddep[q] = indx[j], dsrc[q] = j;
+        break;
+    case DEP_UPPER:
+        for (int j = diagpos[i] + 1; j < pntr[i + 1]; j++, q++) ddep[q] = indx[j], dsrc[q] = j;
+        break;
+    case DEP_ROW_BUT_LAST:
+        for (int j = pntr[i]; j < pntr[i + 1] - 1; j++, q++) ddep[q] = indx[j], dsrc[q] = j;
+        break;
+    case DEP_ROW_BUT_FIRST:
+        for (int j = pntr[i] + 1; j < pntr[i + 1]; j++, q++) ddep[q] = indx[j], dsrc[q] = j;
+        break;
+    case DEP_T_LOWER:
+    case DEP_T_UPPER_ASC:
+        for (int e = tpntr[i]; e < tpntr[i + 1]; e++) {
+            int r = trow[e];
+            if (sel == DEP_T_LOWER ? (r < i) : (r > i)) ddep[q] = r, dsrc[q] = tmap[e], q++;
+        }
+        break;
+    case DEP_T_UPPER:
+        for (int e = tpntr[i + 1] - 1; e >= tpntr[i]; e--) {
+            int r = trow[e];
+            if (r > i) ddep[q] = r, dsrc[q] = tmap[e], q++;
+        }
+        break;
+    }
+    int d = -1;
+    if (!unit) {
+        if (sel == DEP_ROW_BUT_LAST) d = pntr[i + 1] - 1;
+        else if (sel == DEP_ROW_BUT_FIRST) d = pntr[i];
+        else d = diagpos[i];
+    }
+    dgs[i] = d;
+}
+
+void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, const int *indx, const int *diagpos,
+                const csr_transpose *tp, bool unit, bool backward) {
+    p.n = n;
+    p.unit = unit;
+    p.backward = backward;
+    const int *tpntr = tp ? tp->tpntr.as<int>() : nullptr;
+    const int *trow = tp ? tp->trow.as<int>() : nullptr;
+    const int *tmap = tp ? tp->tmap.as<int>() : nullptr;
+    dbuf cnt;
+    cnt.reserve((size_t)(n + 1) * sizeof(int));
+    p.dptr.reserve((size_t)(n + 1) * sizeof(int));
+    int g = (n + 255) / 256;
+    dep_count<<<g, 256, 0, c->stream>>>(n, sel, pntr, diagpos, tpntr, trow, cnt.as<int>());
+    HALA_CHECK_LAUNCH();
+    exclusive_scan_i32(c, cnt.as<int>(), p.dptr.as<int>(), n);
+    int total = fetch_i32(c, p.dptr.as<int>() + n);
+    p.ddep.reserve((size_t)(total > 0 ? total : 1) * sizeof(int));
+    p.dsrc.reserve((size_t)(total > 0 ? total : 1) * sizeof(int));
+    p.diag_src.reserve((size_t)n * sizeof(int));
+    dep_fill<<<g, 256, 0, c->stream>>>(n, sel, pntr, indx, diagpos, tpntr, trow, tmap, p.dptr.as<int>(),
+                                       p.ddep.as<int>(), p.dsrc.as<int>(), p.diag_src.as<int>(), unit ? 1 : 0);
+    HALA_CHECK_LAUNCH();
+    HALA_HIP(hipStreamSynchronize(c->stream));
+}
+
+// ---------------------------------------------------------------------------
+// level sets: sync-free, rows in order (forward) or reverse order (backward)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void level_kernel(int n, int backward, const int *__restrict__ dptr,
+                                                    const int *__restrict__ ddep, int *lev, unsigned *ticket,
+                                                    int *err, int nwaves) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int w = 0;
+    if (lane == 0) w = (int)atomicAdd(ticket, 1u);
+    w = __shfl(w, 0, WAVE);
+    if (w == nwaves - 1 && lane == 0) st_agent(ticket, 0u);  // last ticket re-arms the counter
+    int idx = w * WAVE + lane;
+    int row = backward ? n - 1 - idx : idx;
+    bool done = !(idx < n);
+    int k = 0, end = 0, m = 0;
+    if (!done) {
+        k = dptr[row];
+        end = dptr[row + 1];
+    }
+    unsigned spins = 0;
+    while (__any(!done)) {
+        if (!done) {
+            while (k < end) {
+                int l = ld_agent(lev + ddep[k]);
+                if (l < 0) break;
+                m = max(m, l + 1);
+                k++;
+            }
+            if (k == end) {
+                st_agent(lev + row, m);
+                done = true;
+            }
+        }
+        if (__any(!done)) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > SPIN_LIMIT) {
+                if (!done) {
+                    atomicOr(err, 1);
+                    st_agent(lev + row, 0);
+                    done = true;
+                }
+            }
+        }
+    }
+}
+
+__global__ void fill_i32_k(int *__restrict__ x, int64_t n, int v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+template <class T>
+__global__ void fill_sent_k(T *__restrict__ x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = sentinel<T>();
+}
+
+// chunk counts per level
+__global__ void level_chunks(int nlev, const int *__restrict__ level_ptr, int *__restrict__ nch) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nlev) nch[l] = (level_ptr[l + 1] - level_ptr[l] + WAVE - 1) / WAVE;
+}
+
+// position p in perm -> chunk slot
+__global__ void place_rows(int n, const int *__restrict__ perm, const int *__restrict__ lev,
+                           const int *__restrict__ level_ptr, const int *__restrict__ chunk_ptr,
+                           int *__restrict__ chunk_row, int *__restrict__ slot_of_row) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int row = perm[p];
+    int l = lev[row];
+    int q = p - level_ptr[l];
+    int slot = (chunk_ptr[l] + q / WAVE) * WAVE + (q % WAVE);
+    chunk_row[slot] = row;
+    slot_of_row[row] = slot;
+}
+
+__global__ void chunk_lengths(int nslots, const int *__restrict__ chunk_row, const int *__restrict__ dptr,
+                              int *__restrict__ chunk_len) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int row = chunk_row[s];
+    if (row < 0) return;
+    int len = dptr[row + 1] - dptr[row];
+    if (len > 0) atomicMax(chunk_len + s / WAVE, len);
+}
+
+__global__ void fill_entries(int nslots, const int *__restrict__ chunk_row, const int *__restrict__ chunk_off,
+                             const int *__restrict__ chunk_len, const int *__restrict__ dptr,
+                             const int *__restrict__ ddep, const int *__restrict__ dsrc,
+                             const int *__restrict__ slot_of_row, const int *__restrict__ diag_src,
+                             int *__restrict__ ent_dep, int *__restrict__ ent_src, int *__restrict__ dg_src) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int c = s / WAVE, lane = s % WAVE;
+    int row = chunk_row[s];
+    int len = chunk_len[c];
+    int64_t base = (int64_t)chunk_off[c] * WAVE + lane;
+    int k = 0;
+    if (row >= 0) {
+        int a = dptr[row], b = dptr[row + 1];
+        for (; k < b - a; k++) {
+            ent_dep[base + (int64_t)k * WAVE] = slot_of_row[ddep[a + k]];
+            ent_src[base + (int64_t)k * WAVE] = dsrc[a + k];
+        }
+        dg_src[s] = diag_src[row];
+    } else {
+        dg_src[s] = -1;
+    }
+    for (; k < len; k++) {
+        ent_dep[base + (int64_t)k * WAVE] = -1;
+        ent_src[base + (int64_t)k * WAVE] = -1;
+    }
+}
+
+void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
+    const int n = p.n;
+    HALA_REQUIRE(n > 0, HALA_ERR_ARG, "triangular plan: empty matrix");
+    int g = (n + 255) / 256;
+    p.lev.reserve((size_t)n * sizeof(int));
+    fill_i32_k<<<g, 256, 0, c->stream>>>(p.lev.as<int>(), n, -1);
+    HALA_CHECK_LAUNCH();
+    // levels
+    int nwaves = (n + WAVE - 1) / WAVE;
+    int lblocks = (nwaves + 3) / 4;
+    unsigned *ticket = counters(c) + 100;
+    level_kernel<<<lblocks, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, p.dptr.as<int>(), p.ddep.as<int>(),
+                                                 p.lev.as<int>(), ticket, wait_error_flag(c), lblocks * 4);
+    HALA_CHECK_LAUNCH();
+    check_wait_error(c);
+    p.nlev = reduce_max_i32(c, p.lev.as<int>(), n) + 1;
+    // permutation sorted by (level, row)
+    p.level_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));
+    p.perm.reserve((size_t)n * sizeof(int));
+    bucket_sort(c, n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>());
+    p.h_level_ptr.resize(p.nlev + 1);
+    HALA_HIP(hipMemcpyAsync(p.h_level_ptr.data(), p.level_ptr.p, (size_t)(p.nlev + 1) * sizeof(int),
+                            hipMemcpyDeviceToHost, c->stream));
+    // chunks: ceil(level size / 64) per level
+    dbuf nch, chunk_ptr;
+    nch.reserve((size_t)(p.nlev + 1) * sizeof(int));
+    chunk_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));
+    level_chunks<<<(p.nlev + 255) / 256, 256, 0, c->stream>>>(p.nlev, p.level_ptr.as<int>(), nch.as<int>());
+    HALA_CHECK_LAUNCH();
+    exclusive_scan_i32(c, nch.as<int>(), chunk_ptr.as<int>(), p.nlev);
+    p.nchunk = fetch_i32(c, chunk_ptr.as<int>() + p.nlev);
+    const int nslots = p.nchunk * WAVE;
+    p.chunk_row.reserve((size_t)nslots * sizeof(int));
+    p.slot_of_row.reserve((size_t)n * sizeof(int));
+    p.chunk_len.reserve((size_t)(p.nchunk + 1) * sizeof(int));
+    p.chunk_off.reserve((size_t)(p.nchunk + 1) * sizeof(int));
+    fill_i32_k<<<(nslots + 255) / 256, 256, 0, c->stream>>>(p.chunk_row.as<int>(), nslots, -1);
+    fill_i32_k<<<(p.nchunk + 256) / 256, 256, 0, c->stream>>>(p.chunk_len.as<int>(), p.nchunk + 1, 0);
+    HALA_CHECK_LAUNCH();
+    place_rows<<<g, 256, 0, c->stream>>>(n, p.perm.as<int>(), p.lev.as<int>(), p.level_ptr.as<int>(),
+                                         chunk_ptr.as<int>(), p.chunk_row.as<int>(), p.slot_of_row.as<int>());
+    HALA_CHECK_LAUNCH();
+    chunk_lengths<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), p.dptr.as<int>(),
+                                                              p.chunk_len.as<int>());
+    HALA_CHECK_LAUNCH();
+    exclusive_scan_i32(c, p.chunk_len.as<int>(), p.chunk_off.as<int>(), p.nchunk);
+    int slabs = fetch_i32(c, p.chunk_off.as<int>() + p.nchunk);
+    p.nent = (int64_t)slabs * WAVE;
+    size_t ne = (size_t)(p.nent > 0 ? p.nent : 1);
+    p.ent_dep.reserve(ne * sizeof(int));
+    p.ent_src.reserve(ne * sizeof(int));
+    p.dg_src.reserve((size_t)nslots * sizeof(int));
+    fill_entries<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
+        nslots, p.chunk_row.as<int>(), p.chunk_off.as<int>(), p.chunk_len.as<int>(), p.dptr.as<int>(),
+        p.ddep.as<int>(), p.dsrc.as<int>(), p.slot_of_row.as<int>(), p.diag_src.as<int>(), p.ent_dep.as<int>(),
+        p.ent_src.as<int>(), p.dg_src.as<int>());
+    HALA_CHECK_LAUNCH();
+    p.out64.reserve((size_t)nslots * sizeof(double));
+    if (want32) p.out32.reserve((size_t)nslots * sizeof(float));
+    tri_rearm(c, p);
+    HALA_HIP(hipStreamSynchronize(c->stream));
+}
+
+void tri_rearm(hala_ctx c, tri_plan &p) {
+    int64_t nslots = (int64_t)p.nchunk * WAVE;
+    int g = (int)((nslots + 255) / 256);
+    if (p.out64.p) {
+        fill_sent_k<double><<<g, 256, 0, c->stream>>>(p.out64.as<double>(), nslots);
+        HALA_CHECK_LAUNCH();
+    }
+    if (p.out32.p) {
+        fill_sent_k<float><<<g, 256, 0, c->stream>>>(p.out32.as<float>(), nslots);
+        HALA_CHECK_LAUNCH();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// values
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void gather_vals(int64_t nent, const int *__restrict__ src, const double *__restrict__ f,
+                            T *__restrict__ out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    int s = src[e];
+    out[e] = (s >= 0) ? (T)f[s] : T(0);
+}
+
+template <class T>
+static void load_values(hala_ctx c, tri_plan &p, const double *factor, dbuf &ent, dbuf &dg) {
+    int64_t nslots = (int64_t)p.nchunk * WAVE;
+    ent.reserve((size_t)(p.nent > 0 ? p.nent : 1) * sizeof(T));
+    dg.reserve((size_t)nslots * sizeof(T));
+    if (p.nent > 0) {
+        gather_vals<T><<<(int)((p.nent + 255) / 256), 256, 0, c->stream>>>(p.nent, p.ent_src.as<int>(), factor,
+                                                                          ent.as<T>());
+        HALA_CHECK_LAUNCH();
+    }
+    gather_vals<T><<<(int)((nslots + 255) / 256), 256, 0, c->stream>>>(nslots, p.dg_src.as<int>(), factor,
+                                                                      dg.as<T>());
+    HALA_CHECK_LAUNCH();
+}
+
+void load_values64(hala_ctx c, tri_plan &p, const double *factor) {
+    load_values<double>(c, p, factor, p.ent_val64, p.dg_val64);
+    p.has64 = true;
+}
+void load_values32(hala_ctx c, tri_plan &p, const double *factor) {
+    load_values<float>(c, p, factor, p.ent_val32, p.dg_val32);
+    if (!p.out32.p) {
+        p.out32.reserve((size_t)p.nchunk * WAVE * sizeof(float));
+        tri_rearm(c, p);
+    }
+    p.has32 = true;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wait_value(const T *p, int *err) {
+    T v = ld_agent(p);
+    if (!is_sent(v)) return v;
+    unsigned spins = 0;
+    for (;;) {
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_agent(p);
+        if (!is_sent(v)) return v;
+        if (++spins > SPIN_LIMIT) {
+            atomicOr(err, 1);
+            return T(0);
+        }
+    }
+}
+
+template <class T, class TIN, class TOUT, int MODE, bool UNIT>
+__global__ __launch_bounds__(256) void tri_exec_kernel(int nchunk, int nwaves, const int *__restrict__ chunk_row,
+                                                       const int *__restrict__ chunk_off,
+                                                       const int *__restrict__ chunk_len,
+                                                       const int *__restrict__ ent_dep,
+                                                       const T *__restrict__ ent_val, const T *__restrict__ dg_val,
+                                                       T alpha, const TIN *__restrict__ in, T *out,
+                                                       TOUT *__restrict__ out_nat, T *reset,
+                                                       const int *__restrict__ reset_slot, unsigned *ticket,
+                                                       int *err, const int *gate) {
+    if (gate && *gate) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    int w = 0;
+    if (lane == 0) w = (int)atomicAdd(ticket, 1u);
+    w = __shfl(w, 0, WAVE);
+    if (w == nwaves - 1 && lane == 0) st_agent(ticket, 0u);
+    if (w >= nchunk) return;
+    const int slot = w * WAVE + lane;
+    const int row = chunk_row[slot];
+    if (row < 0) return;
+    const int len = chunk_len[w];
+    const int64_t base = (int64_t)chunk_off[w] * WAVE + lane;
+    T acc;
+    if (MODE == TRI_SUB) {
+        acc = (T)in[row];
+        for (int k = 0; k < len; k++) {
+            int d = ent_dep[base + (int64_t)k * WAVE];
+            if (d < 0) break;
+            T v = ent_val[base + (int64_t)k * WAVE];
+            T xv = wait_value(out + d, err);
+            acc = acc - v * xv;
+        }
+    } else {
+        T s = T(0);
+        for (int k = 0; k < len; k++) {
+            int d = ent_dep[base + (int64_t)k * WAVE];
+            if (d < 0) break;
+            T v = ent_val[base + (int64_t)k * WAVE];
+            T xv = wait_value(out + d, err);
+            s = s + v * xv;
+        }
+        acc = alpha * (T)in[row] - s;
+    }
+    if (!UNIT) acc = acc / dg_val[slot];
+    st_agent(out + slot, acc);
+    if (out_nat) out_nat[row] = (TOUT)acc;
+    if (reset) reset[reset_slot[row]] = sentinel<T>();
+}
+
+template <class T, class TIN, class TOUT>
+void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
+              const int *reset_slot, const int *gate) {
+    constexpr bool is64 = sizeof(T) == 8;
+    HALA_REQUIRE(is64 ? p.has64 : p.has32, HALA_ERR_ARG, "triangular solve: values not loaded for this precision");
+    T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
+    const T *ev = is64 ? reinterpret_cast<const T *>(p.ent_val64.p) : reinterpret_cast<const T *>(p.ent_val32.p);
+    const T *dv = is64 ? reinterpret_cast<const T *>(p.dg_val64.p) : reinterpret_cast<const T *>(p.dg_val32.p);
+    int blocks = (p.nchunk + 3) / 4;
+    int nwaves = blocks * 4;
+    unsigned *ticket = counters(c) + 101;
+    int *err = wait_error_flag(c);
+#define HALA_TRI_LAUNCH(M, U)                                                                                   \
+    tri_exec_kernel<T, TIN, TOUT, M, U><<<blocks, 256, 0, c->stream>>>(                                         \
+        p.nchunk, nwaves, p.chunk_row.as<int>(), p.chunk_off.as<int>(), p.chunk_len.as<int>(),                 \
+        p.ent_dep.as<int>(), ev, dv, alpha, in, out, out_nat, reset, reset_slot, ticket, err, gate)
+    if (mode == TRI_SUB) {
+        if (p.unit) HALA_TRI_LAUNCH(TRI_SUB, true);
+        else HALA_TRI_LAUNCH(TRI_SUB, false);
+    } else {
+        if (p.unit) HALA_TRI_LAUNCH(TRI_SUM, true);
+        else HALA_TRI_LAUNCH(TRI_SUM, false);
+    }
+#undef HALA_TRI_LAUNCH
+    HALA_CHECK_LAUNCH();
+}
+
+template void tri_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, double *, const int *, const int *);
+template void tri_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, float *, const int *, const int *);
+template void tri_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, float *, const int *, const int *);
+
+}  // namespace hala_amd

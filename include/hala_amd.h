@@ -1,0 +1,216 @@
+/*
+ * hala_amd.h -- C ABI of libhala_amd.so, the gfx950 (MI355X) sparse iterative
+ * solve engine behind the hala:: templates in include/hala/.
+ *
+ * Plain C: device pointers, sizes and status codes only.  Every entry point
+ * returns HALA_OK (0) or a negative status; hala_last_error() gives the
+ * message of the last failure on the calling thread.  The C++ templates turn
+ * a non-zero status into std::runtime_error, as HALA does for library errors
+ * (gpu/hala_rocm_common.hpp:88-109 in the reference).
+ *
+ * Data layout: CSR with zero-based int32 pntr[nrows+1], indx[nnz], columns
+ * sorted within each row and the diagonal present in square matrices
+ * (sparse/hala_sparse_core.hpp:42).  Matrix and vector arguments are device
+ * pointers on the context's device unless a name says "host".
+ *
+ * Each declaration cites the reference interface it replaces (file:line in
+ * LIBHALA/hala).
+ */
+#ifndef HALA_AMD_H
+#define HALA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status -------------------------------------------------------------- */
+#define HALA_OK 0
+#define HALA_ERR_ARG (-1)       /* invalid argument / size */
+#define HALA_ERR_HIP (-2)       /* HIP runtime error */
+#define HALA_ERR_ALLOC (-3)     /* device or host allocation failed */
+#define HALA_ERR_PATTERN (-4)   /* unsorted columns, column out of range, missing diagonal */
+#define HALA_ERR_PIVOT (-5)     /* zero or non-finite pivot in ILU(0) */
+#define HALA_ERR_NOCONV (-6)    /* a device dependency wait gave up (never expected) */
+#define HALA_ERR_INTERNAL (-7)
+
+const char *hala_last_error(void);
+int hala_version(int *major, int *minor);
+
+typedef struct hala_ctx_s *hala_ctx;
+typedef struct hala_csr_s *hala_csr;
+typedef struct hala_ilu_s *hala_ilu;
+typedef struct hala_tri_s *hala_tri;
+typedef struct hala_batch_s *hala_batch;
+
+/* ---- engine: device, stream --------------------------------------------- */
+/* gpu_engine(int deviceid), device(), set_active_device():
+ * gpu/hala_gpu_engine.hpp:167-192, :242-245; gpu_device_count():
+ * gpu/hala_rocm_common.hpp:120-125.  stream == NULL uses the null stream. */
+int hala_device_count(int *count);
+int hala_ctx_create(int device, void *stream, hala_ctx *ctx);
+int hala_ctx_destroy(hala_ctx ctx);
+int hala_ctx_set_stream(hala_ctx ctx, void *stream);
+int hala_ctx_sync(hala_ctx ctx);
+int hala_ctx_device(hala_ctx ctx, int *device);
+
+/* ---- memory: gpu_vector, gpu_copy_n, fill -------------------------------- */
+/* gpu_allocate/gpu_free/gpu_copy_n: gpu/hala_rocm_common.hpp:233-297;
+ * gpu_vector::fill: gpu/hala_gpu_vector.hpp:147-155 (one kernel here instead
+ * of log2(n) synchronous copies).  kind: 0 host->device, 1 device->host,
+ * 2 device->device. */
+int hala_malloc(hala_ctx ctx, size_t bytes, void **ptr);
+int hala_free(hala_ctx ctx, void *ptr);
+int hala_memcpy(hala_ctx ctx, void *dst, const void *src, size_t bytes, int kind);
+int hala_fill_f64(hala_ctx ctx, double *x, int64_t n, double value);
+int hala_fill_f32(hala_ctx ctx, float *x, int64_t n, float value);
+int hala_fill_i32(hala_ctx ctx, int *x, int64_t n, int value);
+int hala_convert_f64_f32(hala_ctx ctx, int64_t n, const double *x, float *y);
+int hala_convert_f32_f64(hala_ctx ctx, int64_t n, const float *x, double *y);
+
+/* ---- BLAS-1 on the engine ------------------------------------------------ */
+/* dot: gpu/hala_gpu_blas1.hpp:179-198; norm2: :102-121; axpy: :205-222;
+ * scal: :229-245; vcopy: :49-77.  Reductions follow the canonical order
+ * (256-element blocks, halving tree, sequential pass over block partials, a
+ * final 256-wide tree); result_host is a host pointer. */
+int hala_dot_f64(hala_ctx ctx, int64_t n, const double *x, const double *y, double *result_host);
+int hala_nrm2_f64(hala_ctx ctx, int64_t n, const double *x, double *result_host);
+int hala_axpy_f64(hala_ctx ctx, int64_t n, double alpha, const double *x, double *y);
+int hala_scal_f64(hala_ctx ctx, int64_t n, double alpha, double *x);
+int hala_copy_f64(hala_ctx ctx, int64_t n, const double *x, double *y);
+int hala_dot_f32(hala_ctx ctx, int64_t n, const float *x, const float *y, float *result_host);
+int hala_nrm2_f32(hala_ctx ctx, int64_t n, const float *x, float *result_host);
+int hala_axpy_f32(hala_ctx ctx, int64_t n, float alpha, const float *x, float *y);
+int hala_scal_f32(hala_ctx ctx, int64_t n, float alpha, float *x);
+int hala_copy_f32(hala_ctx ctx, int64_t n, const float *x, float *y);
+/* vdivide z = x ./ y (the Jacobi apply): wax/hala_blas_extensions.hpp:139-151, :267-275 */
+int hala_vdivide_f64(hala_ctx ctx, int64_t n, const double *x, const double *y, double *z);
+/* dense gemv on a column-major m x n matrix: gpu/hala_gpu_blas2.hpp:40-60 */
+int hala_gemv_f64(hala_ctx ctx, char trans, int m, int n, double alpha, const double *A, int lda,
+                  const double *x, double beta, double *y);
+
+/* ---- CSR matrix and products --------------------------------------------- */
+/* make_sparse_matrix(gpu_engine, ...) / gpu_sparse_matrix:
+ * gpu/hala_rocm_sparse_general.hpp:74-185.  The handle aliases pntr/indx
+ * (non-owning, as the reference) and records row-length statistics that pick
+ * the product kernel.  hala_csr_validate checks sorted in-range columns
+ * (and, if require_diag, a diagonal entry in every row). */
+typedef struct {
+    int nrows, ncols;
+    int64_t nnz;
+    int max_row, min_row;
+    double mean_row;
+    int kernel; /* 0: LDS-staged block-of-rows, 1: wavefront per row */
+} hala_csr_info_t;
+int hala_csr_create(hala_ctx ctx, int nrows, int ncols, int64_t nnz, const int *pntr, const int *indx,
+                    hala_csr *csr);
+int hala_csr_destroy(hala_csr csr);
+int hala_csr_info(hala_csr csr, hala_csr_info_t *info);
+int hala_csr_set_kernel(hala_csr csr, int kernel);
+int hala_csr_validate(hala_csr csr, int require_diag, int *bad_row);
+
+/* gpu_sparse_matrix::gemv -> rocsparse_?csrmv: gpu/hala_rocm_sparse_general.hpp:115-128;
+ * free sparse_gemv(gpu_engine, ...): :188-199.  y = alpha op(A) x + beta y,
+ * trans 'N' or 'T' ('C' == 'T' for real types).  'N' sums each row in CSR
+ * order (sparse/hala_sparse_utils.hpp:104-111); 'T' adds the terms of
+ * y_k in increasing source-row order after scaling by beta (:112-117).
+ * beta == 0 does not read y. */
+int hala_spmv_f64(hala_csr A, char trans, double alpha, const double *vals, const double *x, double beta,
+                  double *y);
+int hala_spmv_f32(hala_csr A, char trans, float alpha, const float *vals, const float *x, float beta, float *y);
+/* FP32 values, FP32 products and accumulation, FP64 vectors:
+ * y = alpha * double(sum_f32(vals32 * float(x))) + beta y. */
+int hala_spmv_f32x64(hala_csr A, double alpha, const float *vals, const double *x, double beta, double *y);
+
+/* ---- ILU(0) ------------------------------------------------------------- */
+/* gpu_ilu<T> and make_ilu(gpu_engine, ...): gpu/hala_gpu_ilu.hpp:45-195.
+ * create: symbolic analysis on the device (diagonal positions, lower and
+ * upper level sets, level-ordered permutations, the chunked solve layout);
+ * factor: numeric ILU(0) of vals, in place on an owned copy (:85-104), level
+ * by level in the reference's operation order; apply: r = U^-1 L^-1 x
+ * (:150-172) with the reference CPU operation order
+ * (sparse/hala_sparse_utils.hpp:263-275); apply_t: r = (LU)^-T x (adjoint).
+ * want_f32 keeps an FP32 copy of the factor for the mixed-precision apply. */
+typedef struct {
+    int n;
+    int64_t nnz;
+    int nlev_lower, nlev_upper;
+    int nchunk_lower, nchunk_upper;
+    int64_t slots_lower, slots_upper; /* padded entries of the chunked layout */
+} hala_ilu_info_t;
+int hala_ilu_create(hala_ctx ctx, int n, int64_t nnz, const int *pntr, const int *indx, hala_ilu *ilu);
+int hala_ilu_destroy(hala_ilu ilu);
+int hala_ilu_info(hala_ilu ilu, hala_ilu_info_t *info);
+int hala_ilu_factor_f64(hala_ilu ilu, const double *vals, int want_f32);
+int hala_ilu_apply_f64(hala_ilu ilu, const double *x, double *r);
+int hala_ilu_apply_f32(hala_ilu ilu, const float *x, float *r);
+int hala_ilu_apply_f32x64(hala_ilu ilu, const double *x, double *r);
+int hala_ilu_apply_t_f64(hala_ilu ilu, const double *x, double *r);
+/* Copies of the symbolic and numeric results (device destinations; sizes:
+ * diag n, lev n, level_ptr nlev+1, perm n, factor nnz). */
+int hala_ilu_get_symbolic(hala_ilu ilu, int upper, int *diag, int *lev, int *level_ptr, int *perm);
+int hala_ilu_get_factor_f64(hala_ilu ilu, double *factor);
+
+/* ---- standalone triangular matrix ---------------------------------------- */
+/* make_triangular_matrix(gpu_engine, uplo, diag, ...) / trsv:
+ * gpu/hala_rocm_sparse_triangular.hpp:29-115, :185-194, semantics of
+ * sparse_trsv_array (sparse/hala_sparse_utils.hpp:284-335): the matrix holds
+ * one triangle with the diagonal last ('L') or first ('U') in each row. */
+int hala_tri_create(hala_ctx ctx, char uplo, char diag, int n, int64_t nnz, const int *pntr, const int *indx,
+                    hala_tri *tri);
+int hala_tri_destroy(hala_tri tri);
+int hala_trsv_f64(hala_tri tri, char trans, double alpha, const double *vals, const double *b, double *x);
+
+/* ---- solvers --------------------------------------------------------------- */
+/* solve_cg / solve_cg_ilu on gpu_engine: hex/solvers/hala_solvers_cg.hpp:181-361
+ * (loop solve_cg_core :92-156).  prec: 0 identity, 1 ILU(0) (ilu required),
+ * 2 Jacobi (pdiag = diagonal values).  Stops when iterations == max_iter or
+ * norm2(r) < tol (absolute); *iterations counts operator applications from 1.
+ * hist (device, nullable) receives norm2(r) per pass, at most max_hist. */
+int hala_cg_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag, const double *b,
+                double *x, double tol, int max_iter, int *iterations, double *hist, int max_hist);
+/* Adjoint CG: A^T y = b preconditioned with (LU)^-T. */
+int hala_cg_t_f64(hala_csr A, const double *vals, hala_ilu ilu, const double *b, double *x, double tol,
+                  int max_iter, int *iterations, double *hist, int max_hist);
+
+/* solve_gmres / solve_gmres_ilu on gpu_engine: hex/solvers/hala_solvers_gmres.hpp:125-267.
+ * prec 0 identity, 1 ILU(0).  max_outer counts restart cycles; *total counts
+ * preconditioner applications; hist (device) gets the Givens estimates. */
+int hala_gmres_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *b, double *x,
+                   double tol, int restart, int max_outer, int *total, double *hist, int max_hist, int *nhist);
+/* Mixed precision (new entry point): FP32 ILU(0) and FP32 product inside
+ * GMRES(restart), FP64 Arnoldi/Givens/dots and FP64 iterative refinement to
+ * norm2(b - A x) < rtol * norm2(b).  ilu must be factored with want_f32. */
+int hala_gmres_ir_f64(hala_csr A, const double *vals, const float *vals32, hala_ilu ilu, const double *b,
+                      double *x, double rtol, double inner_rtol, int restart, int max_refine, int *total,
+                      double *hist, int max_hist, int *nhist);
+
+/* ---- adjoint pieces ------------------------------------------------------- */
+/* gvals[j] = -lambda[row(j)] * x[indx[j]]; gcoef[c] = sum_e ccoef[e] * gvals[cnnz[e]]. */
+int hala_grad_vals_f64(hala_csr A, const double *lam, const double *x, double *gvals);
+int hala_grad_coef_f64(hala_ctx ctx, int ncoef, const int *cptr, const int *cnnz, const double *ccoef,
+                       const double *gvals, double *gcoef);
+
+/* ---- synthetic problems (host generators, deterministic, splitmix64) ----- */
+/* Each fills caller-provided host arrays; call with NULL arrays first to get
+ * sizes in *n / *nnz.  Stencils are sorted CSR with Dirichlet rows trimmed. */
+int hala_gen_laplace2d(int nx, int ny, double diag, double off, int *n, int64_t *nnz, int *pntr, int *indx,
+                       double *vals);
+int hala_gen_laplace3d(int nx, int ny, int nz, int *n, int64_t *nnz, int *pntr, int *indx, double *vals);
+int hala_gen_discretize_pde(int num_side, int *n, int64_t *nnz, int *pntr, int *indx, double *vals, double *f);
+int hala_gen_convdiff27(int m, uint64_t seed, double peclet, int *n, int64_t *nnz, int *pntr, int *indx,
+                        double *vals);
+int hala_gen_uniform(uint64_t seed, int64_t n, double lo, double hi, double *out);
+int hala_gen_batch_laplace(int nsys, uint64_t seed, int side_lo, int side_hi, int *n_total, int64_t *nnz_total,
+                           int *sys_rows, int *row_off, int64_t *nnz_off, int *pntr, int *indx, double *vals,
+                           double *dvals);
+int hala_gen_diffusion3d(int m, uint64_t seed, int *n, int64_t *nnz, int *ncoef, int64_t *nmap, double *kappa,
+                         int *pntr, int *indx, double *vals, int *cptr, int *cnnz, double *ccoef);
+int hala_gen_gaussian_source(int m, double sigma, double *f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

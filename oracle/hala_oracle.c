@@ -1,0 +1,682 @@
+/*
+ * hala_oracle.c -- CPU restatement of HALA's sparse iterative-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for libhala_amd.so.  Loaded by
+ * tests, the smoke check and the CPU-baseline leg of the benchmark, never by
+ * the product.  Built with -O2 -ffp-contract=off (see oracle/Makefile) so no
+ * product/sum is fused; the HIP kernels it checks are built the same way.
+ *
+ * Citations are file:line in the reference tree (LIBHALA/hala).
+ */
+#include "hala_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define CB 256 /* canonical block: elements per block and width of the final pass */
+
+/* Pairwise halving tree over a[0..w): a[t] += a[t+off] for off = w/2 .. 1. */
+static double tree_reduce(double *a, int w) {
+    for (int off = w / 2; off >= 1; off >>= 1)
+        for (int t = 0; t < off; t++) a[t] = a[t] + a[t + off];
+    return a[0];
+}
+
+/*
+ * Canonical dot product.  HALA's dot/norm2 call ddot_/dnrm2_
+ * (blas/hala_blas_1.hpp:105-123, :278-291) and leave the summation order to
+ * the linked BLAS.  This build fixes one order, shared with every GPU
+ * reduction:
+ *   1. partial[b] = tree_reduce(x[256b+t]*y[256b+t], t = 0..255), zero padded;
+ *   2. s[t] = 0 + partial[t] + partial[t+256] + ... (sequential), t = 0..255;
+ *   3. result = tree_reduce(s, 256).
+ */
+double hala_or_cdot(int64_t n, const double *x, const double *y) {
+    if (n <= 0) return 0.0;
+    int64_t nb = (n + CB - 1) / CB;
+    double *part = (double *)malloc((size_t)nb * sizeof(double));
+    double a[CB];
+    for (int64_t b = 0; b < nb; b++) {
+        for (int t = 0; t < CB; t++) {
+            int64_t i = b * CB + t;
+            a[t] = (i < n) ? x[i] * y[i] : 0.0;
+        }
+        part[b] = tree_reduce(a, CB);
+    }
+    for (int t = 0; t < CB; t++) {
+        double s = 0.0;
+        for (int64_t k = t; k < nb; k += CB) s = s + part[k];
+        a[t] = s;
+    }
+    free(part);
+    return tree_reduce(a, CB);
+}
+
+double hala_or_cnrm2(int64_t n, const double *x) { return sqrt(hala_or_cdot(n, x, x)); }
+
+/*
+ * Team dot used inside one workgroup of the batched solver: thread t of a team
+ * of `team` threads sums elements t, t+team, ... sequentially from 0, then the
+ * team is reduced with the halving tree.
+ */
+double hala_or_team_dot(int64_t n, const double *x, const double *y, int team) {
+    double *a = (double *)malloc((size_t)team * sizeof(double));
+    for (int t = 0; t < team; t++) {
+        double s = 0.0;
+        for (int64_t i = t; i < n; i += team) s = s + x[i] * y[i];
+        a[t] = s;
+    }
+    double r = tree_reduce(a, team);
+    free(a);
+    return r;
+}
+
+/* BLAS-1 in reference-BLAS element order (blas/hala_blas_1.hpp:210-217, :317-335). */
+static void axpy(int n, double a, const double *x, double *y) {
+    for (int i = 0; i < n; i++) y[i] = y[i] + a * x[i];
+}
+static void scal(int n, double a, double *x) {
+    for (int i = 0; i < n; i++) x[i] = a * x[i];
+}
+
+/*
+ * CSR y = alpha op(A) x + beta y.  sparse/hala_sparse_utils.hpp:104-118.
+ *  'N': sum = sum + vals[j]*x[indx[j]] in row order, then y = alpha*sum + beta*y.
+ *  'T'/'C' (real): y = beta*y, then y[indx[j]] += (alpha*x[i])*vals[j], rows ascending.
+ * beta == 0 does not read y (the rocSPARSE/BLAS convention the GPU engine follows);
+ * with a finite y this equals HALA's CPU result except for the sign of a zero.
+ */
+void hala_or_spmv(char trans, int nrows, int ncols, double alpha, const int *pntr, const int *indx,
+                  const double *vals, const double *x, double beta, double *y) {
+    if (trans == 'N' || trans == 'n') {
+        for (int i = 0; i < nrows; i++) {
+            double sum = 0.0;
+            for (int j = pntr[i]; j < pntr[i + 1]; j++) sum = sum + vals[j] * x[indx[j]];
+            y[i] = (beta == 0.0) ? alpha * sum + 0.0 : alpha * sum + beta * y[i];
+        }
+    } else {
+        if (beta == 0.0) {
+            for (int k = 0; k < ncols; k++) y[k] = 0.0;
+        } else {
+            scal(ncols, beta, y);
+        }
+        for (int i = 0; i < nrows; i++)
+            for (int j = pntr[i]; j < pntr[i + 1]; j++) y[indx[j]] = y[indx[j]] + (alpha * x[i]) * vals[j];
+    }
+}
+
+/* FP32 'N' product with FP32 accumulation, same order as hala_or_spmv. */
+void hala_or_spmv_f32(int nrows, float alpha, const int *pntr, const int *indx, const float *vals,
+                      const float *x, float beta, float *y) {
+    for (int i = 0; i < nrows; i++) {
+        float sum = 0.0f;
+        for (int j = pntr[i]; j < pntr[i + 1]; j++) sum = sum + vals[j] * x[indx[j]];
+        y[i] = (beta == 0.0f) ? alpha * sum + 0.0f : alpha * sum + beta * y[i];
+    }
+}
+
+/*
+ * diag[i] = first position in row i whose column is >= i
+ * (sparse/hala_sparse_core.hpp:45-60).  Returns the first row whose diagonal
+ * entry is missing, or -1 when every row has it (HALA assumes it, :42).
+ */
+int hala_or_diag(int n, const int *pntr, const int *indx, int *diag) {
+    int bad = -1;
+    for (int i = 0; i < n; i++) {
+        int d = pntr[i];
+        while (d < pntr[i + 1] && indx[d] < i) d++;
+        diag[i] = d;
+        if ((d == pntr[i + 1] || indx[d] != i) && bad < 0) bad = i;
+    }
+    return bad;
+}
+
+/*
+ * Level sets of the triangular solves over the combined LU pattern (no HALA
+ * function: rocSPARSE's csrsv analysis builds them internally,
+ * gpu/hala_rocm_sparse_triangular.hpp:71-76).
+ *   lower: lev[i] = 0 without strictly-lower entries, else 1 + max lev[j], j < i in row i;
+ *   upper: mirrored over the strictly-upper entries, rows taken from n-1 down.
+ * level_ptr[l] = number of rows with level < l (nlev+1 entries); perm lists
+ * the rows sorted by (level, row).  Returns nlev.
+ */
+int hala_or_levels(int n, const int *pntr, const int *indx, const int *diag, int upper, int *lev,
+                   int *level_ptr, int *perm) {
+    int maxl = -1;
+    if (!upper) {
+        for (int i = 0; i < n; i++) {
+            int m = 0;
+            for (int j = pntr[i]; j < diag[i]; j++)
+                if (lev[indx[j]] + 1 > m) m = lev[indx[j]] + 1;
+            lev[i] = m;
+            if (m > maxl) maxl = m;
+        }
+    } else {
+        for (int i = n - 1; i >= 0; i--) {
+            int m = 0;
+            for (int j = diag[i] + 1; j < pntr[i + 1]; j++)
+                if (lev[indx[j]] + 1 > m) m = lev[indx[j]] + 1;
+            lev[i] = m;
+            if (m > maxl) maxl = m;
+        }
+    }
+    int nlev = maxl + 1;
+    for (int l = 0; l <= nlev; l++) level_ptr[l] = 0;
+    for (int i = 0; i < n; i++) level_ptr[lev[i] + 1]++;
+    for (int l = 0; l < nlev; l++) level_ptr[l + 1] += level_ptr[l];
+    int *cur = (int *)malloc((size_t)(nlev + 1) * sizeof(int));
+    memcpy(cur, level_ptr, (size_t)(nlev + 1) * sizeof(int));
+    for (int i = 0; i < n; i++) perm[cur[lev[i]]++] = i;
+    free(cur);
+    return nlev;
+}
+
+/*
+ * ILU(0) in IKJ (row) order.  HALA's factorize_ilu_array
+ * (sparse/hala_sparse_utils.hpp:229-254) loops KIJ: for each pivot i it
+ * updates every later row j holding a_ji.  Entry (j,k) then receives the same
+ * updates in the same increasing-i order, each reading the same final row i,
+ * so the row-order form below performs the identical operation sequence on
+ * every entry at O(nnz * row length) cost instead of O(n^2).
+ */
+void hala_or_ilu0(int n, const int *pntr, const int *indx, const int *diag, double *ilu) {
+    for (int j = 0; j < n; j++) {
+        for (int jc = pntr[j]; jc < diag[j]; jc++) {
+            int i = indx[jc];
+            ilu[jc] = ilu[jc] / ilu[diag[i]];
+            double l = ilu[jc];
+            int ik = diag[i] + 1, jk = jc + 1;
+            while (ik < pntr[i + 1] && jk < pntr[j + 1]) {
+                if (indx[ik] == indx[jk]) {
+                    ilu[jk] = ilu[jk] - l * ilu[ik];
+                    ik++;
+                    jk++;
+                } else if (indx[ik] < indx[jk]) {
+                    ik++;
+                } else {
+                    jk++;
+                }
+            }
+        }
+    }
+}
+
+/* HALA's own KIJ loop order (sparse/hala_sparse_utils.hpp:229-254), O(n^2): small inputs only. */
+void hala_or_ilu0_kij(int n, const int *pntr, const int *indx, const int *diag, double *ilu) {
+    for (int i = 0; i < n; i++) {
+        double u = ilu[diag[i]];
+        for (int j = i + 1; j < n; j++) {
+            int jc = pntr[j];
+            while (jc < pntr[j + 1] && indx[jc] < i) jc++;
+            if (jc < pntr[j + 1] && indx[jc] == i) {
+                ilu[jc] = ilu[jc] / u;
+                double l = ilu[jc];
+                int ik = diag[i] + 1, jk = jc + 1;
+                while (ik < pntr[i + 1] && jk < pntr[j + 1]) {
+                    if (indx[ik] == indx[jk]) {
+                        ilu[jk] = ilu[jk] - l * ilu[ik];
+                        ik++;
+                        jk++;
+                    } else if (indx[ik] < indx[jk]) {
+                        ik++;
+                    } else {
+                        jk++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/*
+ * x <- U^-1 L^-1 x over the combined factor (sparse/hala_sparse_utils.hpp:263-275):
+ * forward x[i] -= L_ij x[j] term by term in row order (unit L); backward
+ * x[i] -= U_ij x[j] term by term, then x[i] /= U_ii.
+ */
+void hala_or_ilu_apply(int n, const int *pntr, const int *indx, const int *diag, const double *ilu, double *x) {
+    for (int i = 1; i < n; i++)
+        for (int j = pntr[i]; j < diag[i]; j++) x[i] = x[i] - ilu[j] * x[indx[j]];
+    for (int i = n - 1; i >= 0; i--) {
+        for (int j = diag[i] + 1; j < pntr[i + 1]; j++) x[i] = x[i] - ilu[j] * x[indx[j]];
+        x[i] = x[i] / ilu[diag[i]];
+    }
+}
+
+void hala_or_ilu_apply_f32(int n, const int *pntr, const int *indx, const int *diag, const float *ilu, float *x) {
+    for (int i = 1; i < n; i++)
+        for (int j = pntr[i]; j < diag[i]; j++) x[i] = x[i] - ilu[j] * x[indx[j]];
+    for (int i = n - 1; i >= 0; i--) {
+        for (int j = diag[i] + 1; j < pntr[i + 1]; j++) x[i] = x[i] - ilu[j] * x[indx[j]];
+        x[i] = x[i] / ilu[diag[i]];
+    }
+}
+
+/*
+ * x <- (LU)^-T x = L^-T U^-T x (the adjoint preconditioner; HALA has no
+ * transposed ILU apply).  U^T w = x column by column: w_i = x_i / U_ii, then
+ * x_k -= U_ik w_i for k > i; L^T y = w from the last row up: x_k -= L_ik x_i
+ * for k < i.  Every x_k therefore receives its updates in increasing i
+ * (U^T) and decreasing i (L^T) order, one term at a time.
+ */
+void hala_or_ilu_apply_t(int n, const int *pntr, const int *indx, const int *diag, const double *ilu, double *x) {
+    for (int i = 0; i < n; i++) {
+        x[i] = x[i] / ilu[diag[i]];
+        for (int j = diag[i] + 1; j < pntr[i + 1]; j++) x[indx[j]] = x[indx[j]] - ilu[j] * x[i];
+    }
+    for (int i = n - 1; i >= 0; i--)
+        for (int j = pntr[i]; j < diag[i]; j++) x[indx[j]] = x[indx[j]] - ilu[j] * x[i];
+}
+
+/*
+ * Standalone triangular solve op(T) x = alpha b for a CSR matrix holding one
+ * triangle, the diagonal being the last (L) or first (U) entry of each row
+ * (sparse/hala_sparse_utils.hpp:284-335).  'N' sums the off-diagonal terms
+ * first, then subtracts; 'T'/'C' is column oriented.
+ */
+void hala_or_trsv(char uplo, char trans, char diag, int n, double alpha, const int *pntr, const int *indx,
+                  const double *vals, const double *b, double *x) {
+    int unit = (diag == 'U' || diag == 'u');
+    int lower = (uplo == 'L' || uplo == 'l');
+    if (trans == 'N' || trans == 'n') {
+        if (lower) {
+            x[0] = unit ? alpha * b[0] : alpha * b[0] / vals[0];
+            for (int i = 1; i < n; i++) {
+                double s = 0.0;
+                for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) s = s + vals[j] * x[indx[j]];
+                x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i + 1] - 1];
+            }
+        } else {
+            x[n - 1] = unit ? alpha * b[n - 1] : alpha * b[n - 1] / vals[pntr[n] - 1];
+            for (int i = n - 2; i >= 0; i--) {
+                double s = 0.0;
+                for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) s = s + vals[j] * x[indx[j]];
+                x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i]];
+            }
+        }
+    } else {
+        for (int i = 0; i < n; i++) x[i] = 0.0 * x[i];
+        if (lower) {
+            for (int i = n - 1; i >= 0; i--) {
+                x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i + 1] - 1];
+                for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j];
+            }
+        } else {
+            for (int i = 0; i < n; i++) {
+                x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i]];
+                for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Preconditioners as the solvers apply them                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int kind;
+    int n;
+    const int *pntr, *indx, *diag;
+    const double *data;
+} prec_t;
+
+static void prec_apply(const prec_t *m, const double *r, double *z) {
+    int n = m->n;
+    switch (m->kind) {
+    case HALA_OR_PREC_IDENTITY: /* vcopy */
+        memcpy(z, r, (size_t)n * sizeof(double));
+        break;
+    case HALA_OR_PREC_ILU: /* cpu_ilu::apply: copy then sweep, sparse/hala_sparse_ilu.hpp:105-115 */
+        memcpy(z, r, (size_t)n * sizeof(double));
+        hala_or_ilu_apply(n, m->pntr, m->indx, m->diag, m->data, z);
+        break;
+    case HALA_OR_PREC_JACOBI: /* vdivide, wax/hala_blas_extensions.hpp:139-151 */
+        for (int i = 0; i < n; i++) z[i] = r[i] / m->data[i];
+        break;
+    case HALA_OR_PREC_ILU_T:
+        memcpy(z, r, (size_t)n * sizeof(double));
+        hala_or_ilu_apply_t(n, m->pntr, m->indx, m->diag, m->data, z);
+        break;
+    }
+}
+
+typedef double (*dot_fn)(int64_t, const double *, const double *, int);
+static double dot_canon(int64_t n, const double *x, const double *y, int team) {
+    (void)team;
+    return hala_or_cdot(n, x, y);
+}
+
+/*
+ * Preconditioned CG with HALA's solve_cg_core step order
+ * (hex/solvers/hala_solvers_cg.hpp:92-156 with the lambdas of :203-226):
+ * iterations start at 1 and count operator applications; the loop stops when
+ * iterations == max_iter or norm2(r) < tol (absolute, recursively updated r).
+ * hist[k] receives norm2(r) after the k-th loop pass.
+ */
+static int cg_core(int n, const int *pntr, const int *indx, const double *vals, const prec_t *m, const double *b,
+                   double *x, double tol, int max_iter, dot_fn dot, int team, double *hist, int max_hist) {
+    double *r = (double *)malloc((size_t)n * sizeof(double));
+    double *p = (double *)malloc((size_t)n * sizeof(double));
+    double *ap = (double *)malloc((size_t)n * sizeof(double));
+    double *z = (double *)malloc((size_t)n * sizeof(double));
+    memcpy(r, b, (size_t)n * sizeof(double));
+    hala_or_spmv('N', n, n, 1.0, pntr, indx, vals, x, 0.0, p);
+    axpy(n, -1.0, p, r);
+    prec_apply(m, r, z);
+    memcpy(p, z, (size_t)n * sizeof(double));
+    double zr = dot(n, r, z, team);
+    int it = 1;
+    for (;;) {
+        hala_or_spmv('N', n, n, 1.0, pntr, indx, vals, p, 0.0, ap);
+        it++;
+        double nzr = dot(n, p, ap, team);
+        double a = zr / nzr;
+        axpy(n, a, p, x);
+        axpy(n, -a, ap, r);
+        double rn = sqrt(dot(n, r, r, team));
+        if (hist && it - 2 < max_hist) hist[it - 2] = rn;
+        if (it == max_iter || rn < tol) break;
+        prec_apply(m, r, z);
+        nzr = dot(n, r, z, team);
+        a = nzr / zr;
+        scal(n, a, p);
+        axpy(n, 1.0, z, p);
+        zr = nzr;
+    }
+    free(r);
+    free(p);
+    free(ap);
+    free(z);
+    return it;
+}
+
+int hala_or_cg(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *mpntr,
+               const int *mindx, const int *mdiag, const double *mdata, const double *b, double *x, double tol,
+               int max_iter, double *hist, int max_hist) {
+    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata};
+    return cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, dot_canon, 0, hist, max_hist);
+}
+
+/* Jacobi-PCG of one batched system, reductions in team order. */
+int hala_or_cg_team(int n, const int *pntr, const int *indx, const double *vals, const double *dvals,
+                    const double *b, double *x, double tol, int max_iter, int team, double *hist, int max_hist) {
+    prec_t m = {HALA_OR_PREC_JACOBI, n, 0, 0, 0, dvals};
+    return cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, hala_or_team_dot, team, hist, max_hist);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Givens rotations and packed triangular solve (reference BLAS)             */
+/* ------------------------------------------------------------------------- */
+
+/* drotg, reference BLAS formulation (blas/hala_blas_1.hpp:361-386 binds drotg_). */
+void hala_or_rotg(double *a, double *b, double *c, double *s) {
+    double sa = *a, sb = *b;
+    double roe = (fabs(sa) > fabs(sb)) ? sa : sb;
+    double scale = fabs(sa) + fabs(sb);
+    double r, z;
+    if (scale == 0.0) {
+        *c = 1.0;
+        *s = 0.0;
+        r = 0.0;
+        z = 0.0;
+    } else {
+        double as = sa / scale, bs = sb / scale;
+        r = scale * sqrt(as * as + bs * bs);
+        r = (roe >= 0.0) ? r : -r;
+        *c = sa / r;
+        *s = sb / r;
+        z = 1.0;
+        if (fabs(sa) > fabs(sb)) z = *s;
+        if (fabs(sb) >= fabs(sa) && *c != 0.0) z = 1.0 / *c;
+    }
+    *a = r;
+    *b = z;
+}
+
+/* drot on one pair (blas/hala_blas_1.hpp:393-410 binds drot_). */
+void hala_or_rot(double *x, double *y, double c, double s) {
+    double t = c * *x + s * *y;
+    *y = c * *y - s * *x;
+    *x = t;
+}
+
+/* dtpsv('U','N','N'): packed column-major upper triangle, column sweep from the last column. */
+void hala_or_tpsv_un(int n, const double *ap, double *x) {
+    int kk = n * (n + 1) / 2 - 1;
+    for (int j = n - 1; j >= 0; j--) {
+        if (x[j] != 0.0) {
+            x[j] = x[j] / ap[kk];
+            double t = x[j];
+            int k = kk - 1;
+            for (int i = j - 1; i >= 0; i--) {
+                x[i] = x[i] - t * ap[k];
+                k--;
+            }
+        }
+        kk -= j + 1;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* GMRES                                                                      */
+/* ------------------------------------------------------------------------- */
+
+/* Operator/preconditioner of one GMRES instance. */
+typedef struct {
+    int n;
+    const int *pntr, *indx, *diag;
+    const double *vals;
+    const float *vals32; /* non-null: FP32 operator and FP32 ILU */
+    const double *ilu;
+    const float *ilu32;
+    int prec; /* IDENTITY or ILU */
+    float *w32a, *w32b;
+} gm_op_t;
+
+/* t = alpha*A*x + beta*t in FP64, or the FP32 product converted back. */
+static void gm_spmv(gm_op_t *o, double alpha, const double *x, double beta, double *t) {
+    if (!o->vals32) {
+        hala_or_spmv('N', o->n, o->n, alpha, o->pntr, o->indx, o->vals, x, beta, t);
+        return;
+    }
+    for (int i = 0; i < o->n; i++) o->w32a[i] = (float)x[i];
+    hala_or_spmv_f32(o->n, 1.0f, o->pntr, o->indx, o->vals32, o->w32a, 0.0f, o->w32b);
+    for (int i = 0; i < o->n; i++)
+        t[i] = (beta == 0.0) ? alpha * (double)o->w32b[i] + 0.0 : alpha * (double)o->w32b[i] + beta * t[i];
+}
+
+static void gm_prec(gm_op_t *o, const double *t, double *r) {
+    int n = o->n;
+    if (o->prec == HALA_OR_PREC_IDENTITY) {
+        memcpy(r, t, (size_t)n * sizeof(double));
+    } else if (!o->ilu32) {
+        memcpy(r, t, (size_t)n * sizeof(double));
+        hala_or_ilu_apply(n, o->pntr, o->indx, o->diag, o->ilu, r);
+    } else {
+        for (int i = 0; i < n; i++) o->w32a[i] = (float)t[i];
+        hala_or_ilu_apply_f32(n, o->pntr, o->indx, o->diag, o->ilu32, o->w32a);
+        for (int i = 0; i < n; i++) r[i] = (double)o->w32a[i];
+    }
+}
+
+/*
+ * Restarted left-preconditioned GMRES with HALA's step order
+ * (hex/solvers/hala_solvers_gmres.hpp:125-230): one classical Gram-Schmidt
+ * pass through gemv 'T' / 'N' (krylov_project, :67-72), Givens rotations on the
+ * host, absolute tolerance on the Givens estimate |s_k Z_k|, and the last
+ * rotation of a cycle never applied to Z (only the "still going" branch
+ * rotates Z, :210-216).  max_outer counts restart cycles; the return value
+ * counts preconditioner applications.
+ *   tol_rel > 0: the tolerance is tol_rel times the first preconditioned
+ *                residual norm of the call (used by the refinement inner solve);
+ *   x_zero: x is known to be zero, so b - A x is b itself (no product).
+ */
+static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double tol_rel, int x_zero, int restart,
+                      int max_outer, double *hist, int max_hist, int *nhist) {
+    int n = o->n;
+    double *W = (double *)malloc((size_t)n * (size_t)restart * sizeof(double));
+    double *t = (double *)malloc((size_t)n * sizeof(double));
+    double *r = (double *)malloc((size_t)n * sizeof(double));
+    double *H = (double *)malloc((size_t)restart * (restart + 1) / 2 * sizeof(double) + sizeof(double));
+    double *S = (double *)malloc((size_t)(restart + 1) * sizeof(double));
+    double *C = (double *)malloc((size_t)(restart + 1) * sizeof(double));
+    double *Z = (double *)malloc((size_t)(restart + 1) * sizeof(double));
+    double *cf = (double *)malloc((size_t)(restart + 1) * sizeof(double));
+    double inner_res, outer_res = tol + 1.0;
+    int total = 0, outer = 0, nh = nhist ? *nhist : 0;
+    int first = 1;
+    while (outer_res > tol && outer < max_outer) {
+        int nH = 0, nZ = 0;
+        if (x_zero && outer == 0) {
+            memcpy(t, b, (size_t)n * sizeof(double));
+        } else {
+            memcpy(t, b, (size_t)n * sizeof(double));
+            gm_spmv(o, -1.0, x, 1.0, t);
+        }
+        gm_prec(o, t, r);
+        total++;
+        inner_res = hala_or_cnrm2(n, r);
+        if (first && tol_rel > 0.0) {
+            tol = tol_rel * inner_res;
+            first = 0;
+        }
+        scal(n, 1.0 / inner_res, r);
+        Z[nZ++] = inner_res;
+        memcpy(W, r, (size_t)n * sizeof(double));
+        int k = 0;
+        while (inner_res > tol && k < restart) {
+            gm_spmv(o, 1.0, r, 0.0, t);
+            gm_prec(o, t, r);
+            total++;
+            for (int j = 0; j <= k; j++) cf[j] = hala_or_cdot(n, W + (size_t)j * n, r);
+            for (int i = 0; i < n; i++) {
+                double yi = r[i];
+                for (int j = 0; j <= k; j++) yi = yi + (-cf[j]) * W[(size_t)j * n + i];
+                r[i] = yi;
+            }
+            double nrm = hala_or_cnrm2(n, r);
+            scal(n, 1.0 / nrm, r);
+            for (int i = 0; i < k; i++) hala_or_rot(&cf[i], &cf[i + 1], C[i], S[i]);
+            double c, s, beta = nrm;
+            hala_or_rotg(&cf[k], &beta, &c, &s);
+            for (int j = 0; j <= k; j++) H[nH++] = cf[j];
+            S[k] = s;
+            C[k] = c;
+            inner_res = fabs(s * Z[nZ - 1]);
+            if (hist && nh < max_hist) hist[nh] = inner_res;
+            nh++;
+            k++;
+            if (inner_res > tol && k < restart) {
+                memcpy(W + (size_t)k * n, r, (size_t)n * sizeof(double));
+                Z[nZ++] = 0.0;
+                hala_or_rot(&Z[k - 1], &Z[k], c, s);
+            }
+        }
+        if (nH > 0) {
+            hala_or_tpsv_un(nZ, H, Z);
+            for (int i = 0; i < n; i++) {
+                double yi = x[i];
+                for (int j = 0; j < nZ; j++) yi = yi + (1.0 * Z[j]) * W[(size_t)j * n + i];
+                x[i] = yi;
+            }
+        }
+        outer++;
+        outer_res = inner_res;
+    }
+    if (nhist) *nhist = nh;
+    free(W);
+    free(t);
+    free(r);
+    free(H);
+    free(S);
+    free(C);
+    free(Z);
+    free(cf);
+    return total;
+}
+
+int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,
+                  const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
+                  double *hist, int max_hist, int *nhist) {
+    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0};
+    if (nhist) *nhist = 0;
+    return gmres_core(&o, b, x, tol, 0.0, 0, restart, max_outer, hist, max_hist, nhist);
+}
+
+/*
+ * Mixed-precision iterative refinement (no HALA counterpart): the FP64 outer
+ * loop forms r = b - A x with the FP64 product, stops when
+ * norm2(r) < rtol*norm2(b), and otherwise solves A d = r with GMRES(restart)
+ * whose operator and ILU(0) preconditioner are FP32 (FP32 accumulation) while
+ * its vectors, Gram-Schmidt, Givens and dot products stay FP64; the inner
+ * solve stops at inner_rtol times its first preconditioned residual norm
+ * (at most inner_cycles restarts), then x += d.
+ * hist receives the outer relative residuals; returns the total number of
+ * inner preconditioner applications, *nhist the number of outer residuals.
+ */
+int hala_or_gmres_ir32(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
+                       const int *diag, const float *ilu32, const double *b, double *x, double rtol,
+                       double inner_rtol, int restart, int max_refine, double *hist, int max_hist, int *nhist) {
+    const int inner_cycles = 20;
+    float *w32a = (float *)malloc((size_t)n * sizeof(float));
+    float *w32b = (float *)malloc((size_t)n * sizeof(float));
+    double *r = (double *)malloc((size_t)n * sizeof(double));
+    double *d = (double *)malloc((size_t)n * sizeof(double));
+    gm_op_t o = {n, pntr, indx, diag, vals, vals32, 0, ilu32, HALA_OR_PREC_ILU, w32a, w32b};
+    double bn = hala_or_cnrm2(n, b);
+    int total = 0, nh = 0;
+    for (int it = 0; it <= max_refine; it++) {
+        memcpy(r, b, (size_t)n * sizeof(double));
+        hala_or_spmv('N', n, n, -1.0, pntr, indx, vals, x, 1.0, r);
+        double rel = hala_or_cnrm2(n, r) / bn;
+        if (hist && nh < max_hist) hist[nh] = rel;
+        nh++;
+        if (rel < rtol || it == max_refine) break;
+        memset(d, 0, (size_t)n * sizeof(double));
+        total += gmres_core(&o, r, d, 0.0, inner_rtol, 1, restart, inner_cycles, 0, 0, 0);
+        axpy(n, 1.0, d, x);
+    }
+    if (nhist) *nhist = nh;
+    free(w32a);
+    free(w32b);
+    free(r);
+    free(d);
+    return total;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adjoint pieces                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* dL/dA_ij = -lambda_i x_j on every stored entry. */
+void hala_or_grad_vals(int n, const int *pntr, const int *indx, const double *lam, const double *x, double *gvals) {
+    for (int i = 0; i < n; i++)
+        for (int j = pntr[i]; j < pntr[i + 1]; j++) gvals[j] = -(lam[i] * x[indx[j]]);
+}
+
+/* Chain rule onto stencil coefficients: g_c = sum_e ccoef[e] * gvals[cnnz[e]], e in cptr[c]..cptr[c+1]. */
+void hala_or_grad_coef(int ncoef, const int *cptr, const int *cnnz, const double *ccoef, const double *gvals,
+                       double *gcoef) {
+    for (int c = 0; c < ncoef; c++) {
+        double s = 0.0;
+        for (int e = cptr[c]; e < cptr[c + 1]; e++) s = s + ccoef[e] * gvals[cnnz[e]];
+        gcoef[c] = s;
+    }
+}
+
+/* CSR transpose; each transposed row lists its entries by increasing original row. */
+void hala_or_transpose(int nrows, int ncols, const int *pntr, const int *indx, const double *vals, int *tpntr,
+                       int *tindx, double *tvals) {
+    for (int k = 0; k <= ncols; k++) tpntr[k] = 0;
+    for (int j = 0; j < pntr[nrows]; j++) tpntr[indx[j] + 1]++;
+    for (int k = 0; k < ncols; k++) tpntr[k + 1] += tpntr[k];
+    int *cur = (int *)malloc((size_t)(ncols + 1) * sizeof(int));
+    memcpy(cur, tpntr, (size_t)(ncols + 1) * sizeof(int));
+    for (int i = 0; i < nrows; i++)
+        for (int j = pntr[i]; j < pntr[i + 1]; j++) {
+            int q = cur[indx[j]]++;
+            tindx[q] = i;
+            tvals[q] = vals[j];
+        }
+    free(cur);
+}

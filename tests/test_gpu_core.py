@@ -1,0 +1,383 @@
+"""GPU parity of the core kernels against the CPU oracle (bit for bit).
+
+Every FP64 kernel here follows the oracle's operation order exactly (no fused
+multiply-add, sequential row sums, canonical reductions), so the comparisons
+are on bit patterns unless a test says otherwise.
+"""
+import numpy as np
+import pytest
+
+import hala_amd as H
+import oracle
+from hala_amd import stencils as st
+from tests.conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(engine, a, dtype=None):
+    return engine.load(a, dtype)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def upload(engine, p, i, v):
+    return dev(engine, p), dev(engine, i), dev(engine, v)
+
+
+# ---------------------------------------------------------------- BLAS-1
+@pytest.mark.parametrize("n", [1, 7, 255, 256, 257, 1000, 65536 + 7, 300001])
+def test_dot_nrm2_canonical(engine, n):
+    x = st.uniform(11, n)
+    y = st.uniform(12, n)
+    gx, gy = dev(engine, x), dev(engine, y)
+    assert bits(np.array([H.dot(engine, gx, gy)])) == bits(np.array([oracle.cdot(x, y)]))
+    assert bits(np.array([H.norm2(engine, gx)])) == bits(np.array([oracle.cnrm2(x)]))
+
+
+def test_sqrt_correctly_rounded(engine):
+    # nrm2 = sqrt(cdot): single-element vectors make cdot exact (x*x), isolating the device sqrt
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        v = np.array([rng.uniform(1e-3, 1e3) * 10.0 ** rng.integers(-30, 30)])
+        g = dev(engine, v)
+        assert H.norm2(engine, g) == np.sqrt(v[0] * v[0])
+
+
+def test_axpy_scal_copy(engine):
+    n = 100003
+    x, y = st.uniform(1, n), st.uniform(2, n)
+    gx, gy = dev(engine, x), dev(engine, y)
+    H.axpy(engine, -0.37, gx, gy)
+    assert np.array_equal(bits(host(gy)), bits(y + (-0.37) * x))
+    H.scal(engine, 1.9, gy)
+    assert np.array_equal(bits(host(gy)), bits(1.9 * (y + (-0.37) * x)))
+    gz = engine.vector(n)
+    H.vcopy(engine, gx, gz)
+    assert np.array_equal(host(gz), x)
+
+
+# ---------------------------------------------------------------- SpMV
+MATS = {
+    "lap2d_256": lambda: st.laplace2d(256, 256),
+    "lap3d_40": lambda: st.laplace3d(40, 40, 40),
+    "cd27_24": lambda: st.convdiff27(24, seed=3),
+    "pde_33": lambda: st.discretize_pde(33)[:3],
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_spmv_f64_bitexact(engine, name):
+    p, i, v = MATS[name]()
+    n = len(p) - 1
+    gp, gi, gv = upload(engine, p, i, v)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+    assert A.info()["kernel"] == 0
+    x, y0 = st.uniform(3, n), st.uniform(4, n)
+    for alpha, beta in [(1.0, 0.0), (2.5, -0.5), (-1.0, 1.0)]:
+        gx, gy = dev(engine, x), dev(engine, y0)
+        A.gemv("N", alpha, gx, beta, gy)
+        ref = oracle.spmv("N", n, n, alpha, p, i, v, x, beta, y0)
+        assert np.array_equal(bits(host(gy)), bits(ref)), (alpha, beta)
+    # transposed product: source-row order of the reference scatter
+    gx, gy = dev(engine, x), dev(engine, y0)
+    A.gemv("T", 1.5, gx, 0.25, gy)
+    ref = oracle.spmv("T", n, n, 1.5, p, i, v, x, 0.25, y0)
+    assert np.array_equal(bits(host(gy)), bits(ref))
+
+
+def test_spmv_wave_kernel(engine):
+    # long rows take the wavefront-per-row kernel; its butterfly sum differs in order only
+    n = 2000
+    rng = np.random.default_rng(5)
+    rows = [np.unique(np.concatenate([[r], rng.choice(n, 150, replace=False)])) for r in range(n)]
+    p = np.zeros(n + 1, np.int32)
+    p[1:] = np.cumsum([len(r) for r in rows])
+    i = np.concatenate(rows).astype(np.int32)
+    v = rng.uniform(-1, 1, len(i))
+    gp, gi, gv = upload(engine, p, i, v)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+    assert A.info()["kernel"] == 1
+    x = st.uniform(8, n)
+    gx, gy = dev(engine, x), engine.vector(n)
+    A.gemv("N", 1.0, gx, 0.0, gy)
+    ref = oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))
+    np.testing.assert_allclose(host(gy), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_spmv_f32_variants(engine):
+    import torch
+
+    p, i, v = st.convdiff27(16, seed=3)
+    n = len(p) - 1
+    v32 = v.astype(np.float32)
+    x = st.uniform(9, n)
+    gp, gi = dev(engine, p), dev(engine, i)
+    gv32 = dev(engine, v32)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv32)
+    # pure FP32
+    gx32 = dev(engine, x.astype(np.float32))
+    gy32 = engine.vector(n, dtype=torch.float32)
+    A.gemv("N", 1.0, gx32, 0.0, gy32)
+    ref = oracle.spmv_f32(p, i, v32, x.astype(np.float32))
+    assert np.array_equal(bits(host(gy32)), bits(ref))
+    # FP32 values and accumulation, FP64 vectors
+    gx, gy = dev(engine, x), engine.vector(n)
+    A.gemv("N", 1.0, gx, 0.0, gy)
+    ref64 = 1.0 * oracle.spmv_f32(p, i, v32, x.astype(np.float32)).astype(np.float64) + 0.0
+    assert np.array_equal(bits(host(gy)), bits(ref64))
+
+
+def test_reference_known_answers(engine):
+    # tests/sparse_tests.hpp:165-191 / tests/cuda_sparse_tests.hpp:10-30: 5x5 tridiagonal, alpha = 2
+    p = np.array([0, 2, 5, 8, 11, 13], np.int32)
+    i = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32)
+    v = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], np.float64)
+    x = np.array([1, 2, 3, 4, 5], np.float64)
+    A = np.zeros((5, 5))
+    for r in range(5):
+        A[r, i[p[r]:p[r + 1]]] = v[p[r]:p[r + 1]]
+    gp, gi, gv = upload(engine, p, i, v)
+    gy = engine.vector(5)
+    H.sparse_gemv(engine, "N", 5, 5, 2.0, gp, gi, gv, dev(engine, x), 0.0, gy)
+    assert np.array_equal(host(gy), 2.0 * A @ x)
+    H.sparse_gemv(engine, "T", 5, 5, 2.0, gp, gi, gv, dev(engine, x), 0.0, gy)
+    assert np.array_equal(host(gy), 2.0 * A.T @ x)
+    # cmake/post_install_test.sh: diagonal 3x3 -> {4, 10, 18}
+    gp = dev(engine, np.array([0, 1, 2, 3], np.int32))
+    gi = dev(engine, np.array([0, 1, 2], np.int32))
+    gv = dev(engine, np.array([4.0, 5.0, 6.0]))
+    gy = engine.vector(3)
+    H.sparse_gemv(engine, "N", 3, 3, 1.0, gp, gi, gv, dev(engine, np.array([1.0, 2.0, 3.0])), 0.0, gy)
+    assert list(host(gy)) == [4.0, 10.0, 18.0]
+
+
+def test_pattern_validation(engine):
+    p = np.array([0, 2, 3], np.int32)
+    i = np.array([1, 0, 5], np.int32)  # unsorted row 0, out-of-range column in row 1
+    with pytest.raises(H.HalaError):
+        H.make_sparse_matrix(engine, 2, dev(engine, p), dev(engine, i), dev(engine, np.ones(3)))
+
+
+# ---------------------------------------------------------------- ILU(0)
+ILU_MATS = {
+    "lap2d_64": lambda: st.laplace2d(64, 64),
+    "lap3d_24": lambda: st.laplace3d(24, 24, 24),
+    "cd27_16": lambda: st.convdiff27(16, seed=3),
+    "pde_20": lambda: st.discretize_pde(20)[:3],
+}
+
+
+@pytest.mark.parametrize("name", sorted(ILU_MATS))
+def test_ilu_symbolic_numeric_apply(engine, name):
+    import torch
+
+    p, i, v = ILU_MATS[name]()
+    n = len(p) - 1
+    gp, gi, gv = upload(engine, p, i, v)
+    M = H.make_ilu(engine, gp, gi, gv, want_f32=True)
+    dg, bad = oracle.diag(p, i)
+    assert bad == -1
+    for upper in (False, True):
+        d_, lev, lptr, perm = M.symbolic(upper)
+        olev, olptr, operm = oracle.levels(p, i, dg, upper)
+        assert np.array_equal(host(d_), dg)
+        assert np.array_equal(host(lev), olev)
+        assert np.array_equal(host(lptr), olptr)
+        assert np.array_equal(host(perm), operm)
+    f = oracle.ilu0(p, i, v, dg)
+    assert np.array_equal(bits(host(M.get_factor())), bits(f))
+    x = st.uniform(21, n)
+    gr = engine.vector(n)
+    for _ in range(3):  # repeated applies re-arm the ready flags
+        M.apply(dev(engine, x), gr)
+        assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply(p, i, dg, f, x)))
+    # FP32 factor, FP32 sweep
+    x32 = x.astype(np.float32)
+    g32 = engine.vector(n, dtype=torch.float32)
+    M.apply(dev(engine, x32), g32)
+    assert np.array_equal(bits(host(g32)), bits(oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x32)))
+    # FP32 sweep between FP64 vectors
+    M.apply_f32x64(dev(engine, x), gr)
+    ref = oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x32).astype(np.float64)
+    assert np.array_equal(bits(host(gr)), bits(ref))
+    # adjoint apply (LU)^-T
+    for _ in range(2):
+        M.apply_t(dev(engine, x), gr)
+        assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply_t(p, i, dg, f, x)))
+
+
+def test_ilu_reference_known_answer(engine):
+    # tests/sparse_tests.hpp:423-440
+    p = np.array([0, 3, 6, 9], np.int32)
+    i = np.array([0, 1, 2, 0, 1, 2, 0, 1, 2], np.int32)
+    v = np.array([3.0, 2.0, 1.0, 2.0, 4.0, 3.0, 2.0, 1.0, 6.0])
+    M = H.make_ilu(engine, *upload(engine, p, i, v))
+    d_, _, _, _ = M.symbolic()
+    assert list(host(d_)) == [0, 4, 8]
+    ref = np.array([3.0, 2.0, 1.0, 2.0 / 3.0, 8.0 / 3.0, 7.0 / 3.0, 2.0 / 3.0, -0.125, 5.625])
+    np.testing.assert_allclose(host(M.get_factor()), ref, rtol=1e-15, atol=0)
+    gr = engine.vector(3)
+    M.apply(dev(engine, np.array([1.0, 2.0, 3.0])), gr)
+    np.testing.assert_allclose(host(gr), [1 / 9, 1 / 9, 4 / 9], rtol=1e-15)
+
+
+def test_ilu_zero_pivot_reported(engine):
+    p = np.array([0, 2, 4], np.int32)
+    i = np.array([0, 1, 0, 1], np.int32)
+    v = np.array([1.0, 1.0, 1.0, 1.0])  # u_11 = 1 - 1*1 = 0
+    with pytest.raises(H.HalaError, match="pivot in row 1"):
+        H.make_ilu(engine, *upload(engine, p, i, v))
+
+
+# ---------------------------------------------------------------- triangular
+def test_trsv_reference_cases(engine):
+    # tests/sparse_tests.hpp:259-311
+    M = 3
+    A = np.array([1.0, 4.0, 5.0, 0.0, 2.0, 0.0, 0.0, 0.0, 3.0]).reshape(3, 3, order="F")
+    xref = np.array([-3.0, -2.0, -1.0])
+    cases = [
+        ("L", "N", [0, 1, 3, 5], [0, 0, 1, 0, 2], [1.0, 4.0, 2.0, 5.0, 3.0], "N", A),
+        ("L", "T", [0, 1, 3, 5], [0, 0, 1, 0, 2], [1.0, 4.0, 2.0, 5.0, 3.0], "N", A.T),
+        ("U", "N", [0, 3, 4, 5], [0, 1, 2, 1, 2], [1.0, 4.0, 5.0, 2.0, 3.0], "N", A.T),
+        ("U", "T", [0, 3, 4, 5], [0, 1, 2, 1, 2], [1.0, 4.0, 5.0, 2.0, 3.0], "N", A),
+    ]
+    for uplo, trans, p, i, v, dg, Aop in cases:
+        b = 2.0 * (Aop @ xref)
+        gp, gi, gv = upload(engine, np.array(p, np.int32), np.array(i, np.int32), np.array(v))
+        T = H.make_triangular_matrix(engine, uplo, dg, gp, gi, gv)
+        gx = engine.vector(M)
+        T.trsv(trans, 0.5, dev(engine, b), gx)
+        np.testing.assert_allclose(host(gx), xref, rtol=1e-14), (uplo, trans)
+        ref = oracle.trsv(uplo, trans, dg, 0.5, p, i, v, b)
+        assert np.array_equal(bits(host(gx)), bits(ref))
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("dg", ["N", "U"])
+def test_trsv_random_triangle(engine, uplo, trans, dg):
+    p, i, v = st.laplace3d(12, 12, 12)
+    n = len(p) - 1
+    rows = []
+    for r in range(n):
+        c = i[p[r]:p[r + 1]]
+        w = v[p[r]:p[r + 1]]
+        keep = c <= r if uplo == "L" else c >= r
+        rows.append((c[keep], w[keep] * (1.0 + 0.01 * r % 7)))
+    tp = np.zeros(n + 1, np.int32)
+    tp[1:] = np.cumsum([len(c) for c, _ in rows])
+    ti = np.concatenate([c for c, _ in rows]).astype(np.int32)
+    tv = np.concatenate([w for _, w in rows])
+    b = st.uniform(31, n)
+    T = H.make_triangular_matrix(engine, uplo, dg, *upload(engine, tp, ti, tv))
+    gx = engine.vector(n)
+    for _ in range(2):
+        T.trsv(trans, 1.5, dev(engine, b), gx)
+        ref = oracle.trsv(uplo, trans, dg, 1.5, tp, ti, tv, b)
+        assert np.array_equal(bits(host(gx)), bits(ref))
+
+
+# ---------------------------------------------------------------- CG
+def test_cg_identity_bitexact_history(engine):
+    import torch
+
+    p, i, v = st.laplace2d(96, 96)
+    n = len(p) - 1
+    b = st.uniform(101, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    it, x, hist = oracle.cg(p, i, v, b, tol=tol)
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = engine.vector(n)
+    gh = torch.zeros(5000, dtype=torch.float64, device=gx.device)
+    git = H.solve_cg(engine, tol, gp, gi, gv, None, dev(engine, b), gx, hist=gh)
+    assert git == it
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+
+
+def test_cg_ilu_published_table(engine):
+    # examples/example_gpu.cpp:90-98: CPU ILU-PCG iterations and L2 errors; the GPU path must give
+    # the same table and agree with the CPU solution to 1e-11 (:363-365)
+    table = {4: (9, 1.000499e-03), 8: (14, 3.092774e-04), 16: (23, 8.669565e-05), 32: (41, 2.300756e-05),
+             64: (78, 5.930229e-06), 128: (152, 1.505631e-06)}
+    for ns, (iters, err) in table.items():
+        p, i, v, f = st.discretize_pde(ns)
+        gp, gi, gv = upload(engine, p, i, v)
+        ilu = H.make_ilu(engine, gp, gi, gv)
+        gu = engine.vector(len(f))
+        git = H.solve_cg_ilu(engine, 1e-8, gp, gi, gv, ilu, dev(engine, f), gu)
+        u = host(gu)
+        assert git == iters
+        assert float("%.6e" % st.l2_error(u)) == err
+        dg, _ = oracle.diag(p, i)
+        it, xo, _ = oracle.cg(p, i, v, f, tol=1e-8, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg,
+                              mdata=oracle.ilu0(p, i, v, dg))
+        assert np.linalg.norm(u - xo) < 1e-11
+        assert np.array_equal(bits(u), bits(xo))
+
+
+def test_cg_ilu_3d_bitexact(engine):
+    import torch
+
+    p, i, v = st.laplace3d(32, 32, 32)
+    n = len(p) - 1
+    b = st.uniform(202, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    dg, _ = oracle.diag(p, i)
+    it, x, hist = oracle.cg(p, i, v, b, tol=tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg,
+                            mdata=oracle.ilu0(p, i, v, dg))
+    gp, gi, gv = upload(engine, p, i, v)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    gx = engine.vector(n)
+    gh = torch.zeros(2000, dtype=torch.float64, device=gx.device)
+    git = H.solve_cg_ilu(engine, tol, gp, gi, gv, ilu, dev(engine, b), gx, hist=gh)
+    assert git == it
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    r = b - oracle.spmv("N", n, n, 1.0, p, i, v, host(gx), 0.0, np.zeros(n))
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+
+
+def test_cg_jacobi_bitexact(engine):
+    p, i, v = st.laplace2d(50, 40, diag=4.3)
+    n = len(p) - 1
+    d = np.full(n, 4.3)
+    b = st.uniform(303, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    it, x, _ = oracle.cg(p, i, v, b, tol=tol, prec=oracle.PREC_JACOBI, mdata=d)
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = engine.vector(n)
+    git = H.solve_cg(engine, tol, gp, gi, gv, ("jacobi", dev(engine, d)), dev(engine, b), gx)
+    assert git == it
+    assert np.array_equal(bits(host(gx)), bits(x))
+
+
+def test_cg_reference_small(engine):
+    # tests/solvers_tests.hpp:12-55: tridiagonal -> ILU exact -> two iterations; cyclic 5x5 to 1e-9
+    p = np.array([0, 2, 5, 8, 11, 13], np.int32)
+    i = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32)
+    v = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], np.float64)
+    xref = np.arange(1.0, 6.0)
+    A = np.zeros((5, 5))
+    for r in range(5):
+        A[r, i[p[r]:p[r + 1]]] = v[p[r]:p[r + 1]]
+    b = A @ xref
+    gp, gi, gv = upload(engine, p, i, v)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    gx = engine.vector(5)
+    H.solve_cg(engine, H.stop_criteria(0.0, 2), gp, gi, gv, ilu, dev(engine, b), gx)
+    assert np.abs(host(gx) - xref).sum() < 1e-14
+    p = np.array([0, 3, 6, 9, 12, 15], np.int32)
+    i = np.array([0, 1, 4, 0, 1, 2, 1, 2, 3, 2, 3, 4, 0, 3, 4], np.int32)
+    v = np.array([2, 1, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 1, 2], np.float64)
+    A = np.zeros((5, 5))
+    for r in range(5):
+        A[r, i[p[r]:p[r + 1]]] = v[p[r]:p[r + 1]]
+    b = A @ xref
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = dev(engine, np.array([1.0, 3.0, -4.0, -7.0, 11.0]))
+    H.solve_cg_ilu(engine, 1e-9, gp, gi, gv, None, dev(engine, b), gx)
+    assert np.abs(host(gx) - xref).sum() / np.linalg.norm(xref) < 1e-9 * 10
