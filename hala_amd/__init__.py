@@ -52,12 +52,20 @@ class gpu_engine:
         self.handle = h
 
     def __del__(self):
-        if getattr(self, "handle", None) is not None and lib() is not None:
+        if getattr(self, "handle", None) is not None and callable(lib):
             lib().hala_ctx_destroy(self.handle)
             self.handle = None
 
     def device(self):
         return self._device
+
+    def set_option(self, name, value):
+        call("hala_ctx_set_option", self.handle, name.encode(), int(value))
+
+    def get_option(self, name):
+        v = C.c_int()
+        call("hala_ctx_get_option", self.handle, name.encode(), C.byref(v))
+        return v.value
 
     def synchronize(self):
         call("hala_ctx_sync", self.handle)
@@ -103,7 +111,7 @@ class gpu_sparse_matrix:
         self.handle = h
 
     def __del__(self):
-        if getattr(self, "handle", None) is not None:
+        if getattr(self, "handle", None) is not None and callable(lib):
             lib().hala_csr_destroy(self.handle)
             self.handle = None
 
@@ -154,7 +162,7 @@ class gpu_ilu:
             self.factor(vals, want_f32)
 
     def __del__(self):
-        if getattr(self, "handle", None) is not None:
+        if getattr(self, "handle", None) is not None and callable(lib):
             lib().hala_ilu_destroy(self.handle)
             self.handle = None
 
@@ -219,7 +227,7 @@ class gpu_triangular_matrix:
         self.handle = h
 
     def __del__(self):
-        if getattr(self, "handle", None) is not None:
+        if getattr(self, "handle", None) is not None and callable(lib):
             lib().hala_tri_destroy(self.handle)
             self.handle = None
 
@@ -348,3 +356,61 @@ def solve_gmres_ir(engine, matrix, vals, vals32, ilu, b, x, rtol=1e-10, inner_rt
     call("hala_gmres_ir_f64", matrix.handle, _ptr(vals), _ptr(vals32), ilu.handle, _ptr(b), _ptr(x), float(rtol),
          float(inner_rtol), int(restart), int(max_refine), C.byref(tot), _ptr(hist), mh, C.byref(nh))
     return tot.value, nh.value
+
+
+# ---- batched independent systems -------------------------------------------------
+class batch_solver:
+    """Jacobi-PCG over many independent packed CSR systems in one persistent launch."""
+
+    def __init__(self, engine, row_off, pntr, indx):
+        self.engine = engine
+        self.row_off, self.pntr, self.indx = row_off, pntr, indx
+        self.nsys = int(row_off.numel()) - 1
+        h = C.c_void_p()
+        call("hala_batch_create", engine.handle, self.nsys, _ptr(row_off), _ptr(pntr), _ptr(indx), C.byref(h))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and callable(lib):
+            lib().hala_batch_destroy(self.handle)
+            self.handle = None
+
+    def set_shape(self, team, wg_per_cu):
+        call("hala_batch_set_shape", self.handle, int(team), int(wg_per_cu))
+
+    def shape(self):
+        t, w = C.c_int(), C.c_int()
+        call("hala_batch_get_shape", self.handle, C.byref(t), C.byref(w))
+        return t.value, w.value
+
+    def solve(self, vals, dvals, b, x, rtol=1e-10, max_iter=100000, iters=None):
+        call("hala_batch_pcg_f64", self.handle, _ptr(vals), _ptr(dvals), _ptr(b), _ptr(x), float(rtol), int(max_iter),
+             _ptr(iters))
+        return x
+
+    def tune(self, vals, dvals, b, x, rtol=1e-10, max_iter=100000, reps=2):
+        t, w, ms = C.c_int(), C.c_int(), C.c_double()
+        call("hala_batch_tune", self.handle, _ptr(vals), _ptr(dvals), _ptr(b), _ptr(x), float(rtol), int(max_iter),
+             int(reps), C.byref(t), C.byref(w), C.byref(ms))
+        return t.value, w.value, ms.value
+
+
+# ---- adjoint pieces ------------------------------------------------------------
+def grad_vals(matrix, lam, x, gvals):
+    """gvals[j] = -lam[row(j)] * x[col(j)]."""
+    call("hala_grad_vals_f64", matrix.handle, _ptr(lam), _ptr(x), _ptr(gvals))
+    return gvals
+
+
+def grad_coef(engine, cptr, cnnz, ccoef, gvals, gcoef):
+    """gcoef[c] = sum_e ccoef[e] * gvals[cnnz[e]] over e in cptr[c]..cptr[c+1]."""
+    call("hala_grad_coef_f64", engine.handle, int(cptr.numel()) - 1, _ptr(cptr), _ptr(cnnz), _ptr(ccoef), _ptr(gvals),
+         _ptr(gcoef))
+    return gcoef
+
+
+def assemble(engine, aptr, aidx, aw, coef, vals):
+    """vals[j] = sum_e aw[e] * coef[aidx[e]] over e in aptr[j]..aptr[j+1]."""
+    call("hala_assemble_f64", engine.handle, int(aptr.numel()) - 1, _ptr(aptr), _ptr(aidx), _ptr(aw), _ptr(coef),
+         _ptr(vals))
+    return vals

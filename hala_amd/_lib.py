@@ -44,6 +44,8 @@ SIGNATURES = {
     "hala_ctx_set_stream": [vp, vp],
     "hala_ctx_sync": [vp],
     "hala_ctx_device": [vp, pi32],
+    "hala_ctx_set_option": [vp, C.c_char_p, i32],
+    "hala_ctx_get_option": [vp, C.c_char_p, pi32],
     "hala_malloc": [vp, C.c_size_t, pvp],
     "hala_free": [vp, vp],
     "hala_memcpy": [vp, vp, vp, C.c_size_t, i32],
@@ -89,6 +91,13 @@ SIGNATURES = {
     "hala_cg_t_f64": [vp, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_gmres_f64": [vp, vp, i32, vp, vp, vp, f64, i32, i32, pi32, vp, i32, pi32],
     "hala_gmres_ir_f64": [vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, pi32, vp, i32, pi32],
+    "hala_assemble_f64": [vp, i64, vp, vp, vp, vp, vp],
+    "hala_batch_create": [vp, i32, vp, vp, vp, pvp],
+    "hala_batch_destroy": [vp],
+    "hala_batch_set_shape": [vp, i32, i32],
+    "hala_batch_get_shape": [vp, pi32, pi32],
+    "hala_batch_pcg_f64": [vp, vp, vp, vp, vp, f64, i32, vp],
+    "hala_batch_tune": [vp, vp, vp, vp, vp, f64, i32, i32, pi32, pi32, pf64],
     "hala_grad_vals_f64": [vp, vp, vp, vp],
     "hala_grad_coef_f64": [vp, i32, vp, vp, vp, vp, vp],
     "hala_gen_laplace2d": [i32, i32, f64, f64, pi32, pi64, vp, vp, vp],

@@ -36,11 +36,12 @@ void cdot_launch(hala_ctx c, int64_t n, const T *x, const T *y, T *out_dev, int 
     constexpr int K = 4;
     int64_t nb = (n + CB - 1) / CB;
     if (nb < 1) nb = 1;
-    c->red.reserve(nb * sizeof(T));
+    T *part;
+    if constexpr (sizeof(T) == 8) part = reinterpret_cast<T *>(partials64(c, nb));
+    else part = reinterpret_cast<T *>(partials32(c, nb));
     int64_t g = (nb + K - 1) / K;
     HALA_REQUIRE(g < (1ll << 31), HALA_ERR_ARG, "vector too long");
-    cdot_kernel<T, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, c->red.as<T>(), counters(c) + 0, out_dev,
-                                                          mode);
+    cdot_kernel<T, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, part, lb_counter(c, LB_DOT), out_dev, mode);
     HALA_CHECK_LAUNCH();
 }
 template void cdot_launch<double>(hala_ctx, int64_t, const double *, const double *, double *, int);
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, const double *__rest
             if (b < nb) st_agent(pc + b, v[k]);
         }
     }
-    if (last_block(counters_ + col, gridDim.x)) {
+    if (last_block(counters_ + col * (LB_GROUPS + 1), gridDim.x)) {
         double r = final_pass<double>(pc, nb, sh);
         if (threadIdx.x == 0) y[col] = (beta == 0.0) ? alpha * r + 0.0 : alpha * r + beta * y[col];
     }
@@ -231,10 +232,10 @@ int hala_gemv_f64(hala_ctx c, char trans, int m, int n, double alpha, const doub
             constexpr int K = 4;
             int64_t nb = (m + CB - 1) / CB;
             if (nb < 1) nb = 1;
-            c->red.reserve((size_t)nb * n * sizeof(double));
+            double *part = partials64(c, (size_t)nb * n);
             dim3 g((unsigned)((nb + K - 1) / K), (unsigned)n);
-            gemv_t_kernel<K><<<g, CB, 0, c->stream>>>(m, A, lda, x, alpha, beta, y, nb, c->red.as<double>(),
-                                                     counters(c) + 8);
+            gemv_t_kernel<K><<<g, CB, 0, c->stream>>>(m, A, lda, x, alpha, beta, y, nb, part,
+                                                     lb_counter(c, LB_GEMV));
             HALA_CHECK_LAUNCH();
         }
     });

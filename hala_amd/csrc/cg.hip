@@ -9,6 +9,7 @@
 // Every scalar lives on the device; each kernel returns at once when the
 // stop flag is set, so the host launches passes in batches and reads the
 // flag once per batch instead of blocking on every dot product.
+#include <algorithm>
 #include <cmath>
 
 #include "blas1.hpp"
@@ -85,19 +86,23 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
                                                       double *part, unsigned *cnt, cg_state *st) {
     if (st->done) return;
     __shared__ double prod[spmv_tile<double>::value];
-    double acc = csr_block_rowsum<double, double, double>(n, pntr, indx, vals, p, prod);
-    int i = blockIdx.x * SPMV_ROWS + threadIdx.x;
-    double v = 0.0;
-    if (i < n) {
-        double y = 1.0 * acc + 0.0;
-        ap[i] = y;
-        v = p[i] * y;
+    double *sh = prod;  // LDS reused for the tree
+    const int nb = (n + SPMV_ROWS - 1) / SPMV_ROWS;
+    // grid-stride over 256-row blocks: one arrival (atomic) per workgroup, not per row block
+    for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        double acc = csr_block_rowsum<double, double, double>(n, pntr, indx, vals, p, prod, blk);
+        int i = blk * SPMV_ROWS + threadIdx.x;
+        double v = 0.0;
+        if (i < n) {
+            double y = 1.0 * acc + 0.0;
+            ap[i] = y;
+            v = p[i] * y;
+        }
+        v = tree256_1<double>(v, sh);
+        if (threadIdx.x == 0) st_agent(part + blk, v);
     }
-    double *sh = prod;  // reuse LDS for the tree
-    v = tree256_1<double>(v, sh);
-    if (threadIdx.x == 0) st_agent(part + blockIdx.x, v);
     if (last_block(cnt, gridDim.x)) {
-        double t = final_pass<double>(part, gridDim.x, sh);
+        double t = final_pass<double>(part, nb, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->zr / t;
             st->it = st->it + 1;
@@ -256,15 +261,14 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     c->vec.reserve((size_t)4 * n * sizeof(double));
     double *r = c->vec.as<double>();
     double *p = r + n, *ap = p + n, *z = ap + n;
-    c->red.reserve((size_t)2 * nb * sizeof(double));
-    double *part = c->red.as<double>(), *part2 = part + nb;
-    unsigned *cnt = counters(c);
+    double *part = partials64(c, (size_t)2 * nb), *part2 = part + nb;
     cg_state *st = cg_state_ptr(c);
     const int *pntr = P.transposed ? P.A->tp.tpntr.as<int>() : P.A->pntr;
     const int *indx = P.transposed ? P.A->tp.trow.as<int>() : P.A->indx;
     const double *vals = P.transposed ? P.tvals : P.vals;
 
     const int gs = (int)((n + SPMV_ROWS - 1) / SPMV_ROWS);
+    const int gsd = std::min(gs, c->num_cus * c->opt.spmv_blocks_per_cu);  // grid-stride SpMV+dot
     const int ge = (int)((nb + EK - 1) / EK);
     const int gp = grid_for(n, 256 * 4, 16384);
     cg_init_kernel<<<1, 1, 0, c->stream>>>(st, tol, max_iter, hist, max_hist);
@@ -276,31 +280,31 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
         else ilu_apply64(P.M, in, out, g);
     };
     switch (P.prec) {
-    case PREC_ID: cg_start_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, r, nullptr, nullptr, p, part, cnt + 1, st); break;
+    case PREC_ID: cg_start_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, r, nullptr, nullptr, p, part, lb_counter(c, LB_CG_START), st); break;
     case PREC_JAC:
-        cg_start_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, r, nullptr, P.pdiag, p, part, cnt + 1, st);
+        cg_start_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, r, nullptr, P.pdiag, p, part, lb_counter(c, LB_CG_START), st);
         break;
     default:
         precond(r, z, nullptr);
-        cg_start_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, r, z, nullptr, p, part, cnt + 1, st);
+        cg_start_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, r, z, nullptr, p, part, lb_counter(c, LB_CG_START), st);
     }
     HALA_CHECK_LAUNCH();
 
     auto one_pass = [&]() {
-        cg_spmv_kernel<true><<<gs, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, cnt + 2, st);
+        cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
         switch (P.prec) {
         case PREC_ID:
-            cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, cnt + 3, st);
+            cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
             cg_pupdate_kernel<PREC_ID><<<gp, 256, 0, c->stream>>>(n, nullptr, r, nullptr, p, st);
             break;
         case PREC_JAC:
-            cg_update_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, p, ap, P.pdiag, x, r, part, part2, cnt + 3, st);
+            cg_update_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, p, ap, P.pdiag, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
             cg_pupdate_kernel<PREC_JAC><<<gp, 256, 0, c->stream>>>(n, nullptr, r, P.pdiag, p, st);
             break;
         default:
-            cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, cnt + 3, st);
+            cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
             precond(r, z, gate);
-            cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, cnt + 4, st);
+            cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, lb_counter(c, LB_CG_RZ), st);
             cg_pupdate_kernel<PREC_ILU><<<gp, 256, 0, c->stream>>>(n, z, nullptr, nullptr, p, st);
         }
         HALA_CHECK_LAUNCH();
@@ -311,7 +315,7 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     while (!done) {
         for (int k = 0; k < batch; k++) one_pass();
         cg_read(c, &it, &done, &status, &rn);
-        if (batch < 32) batch *= 2;
+        if (batch < c->opt.cg_batch_max) batch = std::min(2 * batch, std::max(c->opt.cg_batch_max, 1));
     }
     check_wait_error(c);
     if (iterations) *iterations = it;

@@ -96,15 +96,27 @@ struct dbuf {
 // ---------------------------------------------------------------------------
 // context (the engine): device, stream and reusable scratch
 // ---------------------------------------------------------------------------
+// Launch-shape options (set with hala_ctx_set_option, picked by the autotuner).
+struct hala_options {
+    int tri_waves_per_cu = 8;  // persistent triangular-solve pool: waves per CU
+    int tri_pool_waves = 0;    // absolute pool size (overrides waves per CU when > 0)
+    int tri_sleep = 1;         // s_sleep slots between polls of an unready dependency
+    int cg_batch_max = 32;     // solver passes launched between stop-flag reads
+    int spmv_blocks_per_cu = 8;  // workgroups per CU of the grid-stride fused SpMV + dot
+};
+
 struct hala_ctx_s {
+    hala_options opt;
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int num_cus = 256;
     // small device scratch: reduction partials, scalars, counters
-    hala_amd::dbuf red;      // reduction partials
+    hala_amd::dbuf red;      // reduction partials, FP64 (sentinel between reductions)
+    hala_amd::dbuf red32;    // reduction partials, FP32 (sentinel between reductions)
     hala_amd::dbuf scal;     // device scalars (8 KiB), zero-initialised at creation
     hala_amd::dbuf vec;      // solver work vectors
+    hala_amd::dbuf lbc;      // last-block arrival counters (zeroed at creation, self re-arming)
     double *host_pinned = nullptr;  // pinned host landing zone for scalars
 };
 
@@ -192,31 +204,97 @@ __device__ __forceinline__ T tree256_1(T v, T *sh) {
     return a[0];
 }
 
+// Partials of a reduction are value-as-flag slots: the buffer holds sentinels
+// between reductions; each block stores its partials with agent-scope stores,
+// and the block that finishes last polls every slot until it is no longer the
+// sentinel, then re-arms it.  No release/acquire fence is needed (on gfx950 an
+// agent-scope fence writes back and invalidates the whole L2, which costs far
+// more than the reduction itself).
+template <class T>
+__device__ __forceinline__ T poll_slot(T *p) {
+    T v = ld_agent(p);
+    unsigned spins = 0;
+    while (is_sent(v)) {
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_agent(p);
+        if (++spins > (1u << 24)) return T(0);  // never expected: bounded wait
+    }
+    return v;
+}
+
 // Final canonical pass over nb partials by one 256-thread block:
 // s[t] = 0 + p[t] + p[t+256] + ..., then tree256.  Result valid in thread 0.
+// Re-arms every slot it reads.
 template <class T>
-__device__ __forceinline__ T final_pass(const T *part, int64_t nb, T *sh) {
+__device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh) {
+    constexpr int B = 8;  // slots loaded per thread at once
     T s = T(0);
-    for (int64_t k = threadIdx.x; k < nb; k += CB) s = s + ld_agent(part + k);
+    for (int64_t k0 = threadIdx.x; k0 < nb; k0 += (int64_t)CB * B) {
+        T v[B];
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            int64_t k = k0 + (int64_t)CB * j;
+            v[j] = (k < nb) ? ld_agent(part + k) : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            int64_t k = k0 + (int64_t)CB * j;
+            if (k < nb) {
+                T x = v[j];
+                if (is_sent(x)) x = poll_slot(part + k);
+                st_agent(part + k, sentinel<T>());
+                s = s + x;
+            }
+        }
+    }
     return tree256_1<T>(s, sh);
 }
 
-// "Last block" completion: every block calls this after writing its partials
-// (thread 0 only stores them).  Returns true in exactly one block, after all
-// partials are visible; the counter is re-armed to zero for the next launch.
+// "Last block" election: every block calls this after its thread 0 stored its
+// partials.  Returns true in exactly one block (the last to arrive), which
+// then collects the partials with final_pass(); the counter is re-armed to
+// zero.  The atomic is relaxed: final_pass() waits on the slots themselves.
+// Arrivals are counted in two stages (LB_GROUPS group counters, then one top
+// counter) because atomics on one address serialise across the whole chip.
+// `counter` points at LB_GROUPS + 1 zeroed words owned by one reduction site.
+constexpr unsigned LB_GROUPS = 64;
 __device__ __forceinline__ bool last_block(unsigned *counter, unsigned nblocks) {
     __shared__ int am_last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == nblocks - 1);
-        if (am_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned b = blockIdx.x;  // nblocks counts blockIdx.x (a 2-D grid gives each y its own site)
+        const unsigned g = b % LB_GROUPS;
+        const unsigned ngroups = nblocks < LB_GROUPS ? nblocks : LB_GROUPS;
+        const unsigned gsize = (nblocks - g + LB_GROUPS - 1) / LB_GROUPS;  // blocks with index % G == g
+        int last = 0;
+        unsigned prev = __hip_atomic_fetch_add(counter + 1 + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gsize - 1) {
+            __hip_atomic_store(counter + 1 + g, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned top = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == ngroups - 1) {
+                __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = 1;
+            }
+        }
+        am_last = last;
     }
     __syncthreads();
     return am_last != 0;
 }
 
+// Reduction partial buffers (sentinel-filled when grown), defined in ctx.hip.
+double *partials64(hala_ctx c, size_t count);
+float *partials32(hala_ctx c, size_t count);
+
 inline unsigned *counters(hala_ctx c) { return reinterpret_cast<unsigned *>(c->scal.as<double>() + S_CNT); }
+
+// Arrival counters of reduction site `site` (LB_GROUPS + 1 zeroed words each).
+constexpr int LB_SITES = 160;
+inline unsigned *lb_counter(hala_ctx c, int site) { return c->lbc.as<unsigned>() + site * (LB_GROUPS + 1); }
+enum lb_site : int { LB_DOT = 0, LB_CG_START = 1, LB_CG_SPMV = 2, LB_CG_UPDATE = 3, LB_CG_RZ = 4, LB_GM_NRM = 5,
+                     LB_GM_ORTH = 6, LB_IR_NRM = 7, LB_GEMV = 16 /* .. 16 + 32 columns */,
+                     LB_GM_COL = 64 /* .. 64 + 64 Krylov columns */ };
+constexpr int GM_MAX_RESTART = 64;
 
 inline int grid_for(int64_t work, int per_block, int cap) {
     int64_t g = (work + per_block - 1) / per_block;

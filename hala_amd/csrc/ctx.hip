@@ -40,6 +40,30 @@ static void convert(hala_ctx c, int64_t n, const A *x, B *y) {
     HALA_CHECK_LAUNCH();
 }
 
+template <class T>
+__global__ void fill_sentinel_kernel(T *__restrict__ x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) x[i] = sentinel<T>();
+}
+
+template <class T>
+static T *partials(hala_ctx c, dbuf &b, size_t count) {
+    if (count < 1) count = 1;
+    size_t need = count * sizeof(T);
+    if (need > b.bytes) {
+        HALA_HIP(hipStreamSynchronize(c->stream));  // a reduction in flight may still read the old buffer
+        b.reserve(need < (size_t)(64 * 1024) * sizeof(T) ? (size_t)(64 * 1024) * sizeof(T) : need);
+        int64_t n = (int64_t)(b.bytes / sizeof(T));
+        fill_sentinel_kernel<T><<<grid_for(n, 256 * 4, 4096), 256, 0, c->stream>>>(b.as<T>(), n);
+        HALA_CHECK_LAUNCH();
+    }
+    return b.as<T>();
+}
+
+double *partials64(hala_ctx c, size_t count) { return partials<double>(c, c->red, count); }
+float *partials32(hala_ctx c, size_t count) { return partials<float>(c, c->red32, count); }
+
 }  // namespace hala_amd
 
 using namespace hala_amd;
@@ -80,7 +104,10 @@ int hala_ctx_create(int device, void *stream, hala_ctx *out) {
         try {
             c->scal.reserve(S_TOTAL * sizeof(double));
             HALA_HIP(hipMemsetAsync(c->scal.p, 0, S_TOTAL * sizeof(double), c->stream));
-            c->red.reserve(64 * 1024 * sizeof(double));
+            c->lbc.reserve((size_t)LB_SITES * (LB_GROUPS + 1) * sizeof(unsigned));
+            HALA_HIP(hipMemsetAsync(c->lbc.p, 0, c->lbc.bytes, c->stream));
+            partials64(c, 64 * 1024);
+            partials32(c, 64 * 1024);
             HALA_HIP(hipHostMalloc((void **)&c->host_pinned, 1024 * sizeof(double), hipHostMallocDefault));
             HALA_HIP(hipStreamSynchronize(c->stream));
         } catch (...) {
@@ -123,6 +150,35 @@ int hala_ctx_device(hala_ctx c, int *device) {
     return guard([&] {
         HALA_REQUIRE(c && device, HALA_ERR_ARG, "null argument");
         *device = c->device;
+    });
+}
+
+static int *option_slot(hala_ctx c, const char *name) {
+    std::string k = name ? name : "";
+    if (k == "tri_waves_per_cu") return &c->opt.tri_waves_per_cu;
+    if (k == "tri_pool_waves") return &c->opt.tri_pool_waves;
+    if (k == "tri_sleep") return &c->opt.tri_sleep;
+    if (k == "cg_batch_max") return &c->opt.cg_batch_max;
+    if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
+    return nullptr;
+}
+
+int hala_ctx_set_option(hala_ctx c, const char *name, int value) {
+    return guard([&] {
+        HALA_REQUIRE(c, HALA_ERR_ARG, "null context");
+        int *slot = option_slot(c, name);
+        HALA_REQUIRE(slot, HALA_ERR_ARG, std::string("unknown option ") + (name ? name : "(null)"));
+        HALA_REQUIRE(value >= 0 && value <= (1 << 20), HALA_ERR_ARG, "option value out of range");
+        *slot = value;
+    });
+}
+
+int hala_ctx_get_option(hala_ctx c, const char *name, int *value) {
+    return guard([&] {
+        HALA_REQUIRE(c && value, HALA_ERR_ARG, "null argument");
+        int *slot = option_slot(c, name);
+        HALA_REQUIRE(slot, HALA_ERR_ARG, std::string("unknown option ") + (name ? name : "(null)"));
+        *value = *slot;
     });
 }
 

@@ -30,9 +30,10 @@ struct spmv_tile {
 // Must be called by all 256 threads of the block.
 template <class TV, class TX, class TA>
 __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
-                                               const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod) {
+                                               const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod,
+                                               int blk) {
     constexpr int TILE = spmv_tile<TA>::value;
-    const int r0 = blockIdx.x * SPMV_ROWS;
+    const int r0 = blk * SPMV_ROWS;
     const int row = r0 + threadIdx.x;
     const int rlast = min(r0 + SPMV_ROWS, nrows);
     const int bstart = pntr[r0];
@@ -66,6 +67,12 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
         __syncthreads();
     }
     return acc;
+}
+
+template <class TV, class TX, class TA>
+__device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
+                                               const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod) {
+    return csr_block_rowsum<TV, TX, TA>(nrows, pntr, indx, vals, x, prod, blockIdx.x);
 }
 
 // First row of an invalid pattern (-1 when valid): pointers must start at 0,

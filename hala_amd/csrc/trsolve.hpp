@@ -40,6 +40,7 @@ struct tri_plan {
     dbuf lev, level_ptr, perm;
     std::vector<int> h_level_ptr;
     dbuf chunk_row, chunk_len, chunk_off, slot_of_row;
+
     dbuf ent_dep, ent_src, dg_src;
     // values
     dbuf ent_val64, ent_val32, dg_val64, dg_val32;

@@ -55,6 +55,11 @@ int hala_ctx_destroy(hala_ctx ctx);
 int hala_ctx_set_stream(hala_ctx ctx, void *stream);
 int hala_ctx_sync(hala_ctx ctx);
 int hala_ctx_device(hala_ctx ctx, int *device);
+/* Launch-shape options (no reference counterpart; the reference's rocSPARSE
+ * calls pick their own): "tri_waves_per_cu", "tri_pool_waves", "tri_sleep",
+ * "cg_batch_max".  Unknown names return HALA_ERR_ARG. */
+int hala_ctx_set_option(hala_ctx ctx, const char *name, int value);
+int hala_ctx_get_option(hala_ctx ctx, const char *name, int *value);
 
 /* ---- memory: gpu_vector, gpu_copy_n, fill -------------------------------- */
 /* gpu_allocate/gpu_free/gpu_copy_n: gpu/hala_rocm_common.hpp:233-297;
@@ -187,11 +192,34 @@ int hala_gmres_ir_f64(hala_csr A, const double *vals, const float *vals32, hala_
                       double *x, double rtol, double inner_rtol, int restart, int max_refine, int *total,
                       double *hist, int max_hist, int *nhist);
 
+/* ---- batched independent systems ----------------------------------------- */
+/* New entry points (HALA's solve_batch_cg, hex/solvers/hala_solvers_cg_batch.hpp:68-249,
+ * shares one matrix across right-hand sides; these solve nsys different
+ * matrices).  Packed CSR: row_off[nsys+1] system row offsets, pntr[n_total+1]
+ * absolute positions, indx local to each system (at most 8128 rows each).
+ * Every system runs Jacobi-PCG (z = r ./ dvals) with HALA's CG step order to
+ * norm2(r_s) < rtol * norm2(b_s), in one persistent launch replayed from a
+ * captured HIP graph; iters (device, nullable) receives each count.
+ * hala_batch_tune times the launch shapes and keeps the fastest. */
+int hala_batch_create(hala_ctx ctx, int nsys, const int *row_off, const int *pntr, const int *indx,
+                      hala_batch *batch);
+int hala_batch_destroy(hala_batch batch);
+int hala_batch_set_shape(hala_batch batch, int team, int wg_per_cu);
+int hala_batch_get_shape(hala_batch batch, int *team, int *wg_per_cu);
+int hala_batch_pcg_f64(hala_batch batch, const double *vals, const double *dvals, const double *b, double *x,
+                       double rtol, int max_iter, int *iters);
+int hala_batch_tune(hala_batch batch, const double *vals, const double *dvals, const double *b, double *x,
+                    double rtol, int max_iter, int reps, int *team, int *wg_per_cu, double *ms);
+
 /* ---- adjoint pieces ------------------------------------------------------- */
 /* gvals[j] = -lambda[row(j)] * x[indx[j]]; gcoef[c] = sum_e ccoef[e] * gvals[cnnz[e]]. */
 int hala_grad_vals_f64(hala_csr A, const double *lam, const double *x, double *gvals);
 int hala_grad_coef_f64(hala_ctx ctx, int ncoef, const int *cptr, const int *cnnz, const double *ccoef,
                        const double *gvals, double *gcoef);
+/* Values from coefficients: vals[j] = sum_e aw[e] * coef[acoef_idx[e]], e in aptr[j]..aptr[j+1]
+ * (sequential from 0), e.g. the stencil entries from per-face conductivities. */
+int hala_assemble_f64(hala_ctx ctx, int64_t nnz, const int *aptr, const int *acoef_idx, const double *aw,
+                      const double *coef, double *vals);
 
 /* ---- synthetic problems (host generators, deterministic, splitmix64) ----- */
 /* Each fills caller-provided host arrays; call with NULL arrays first to get

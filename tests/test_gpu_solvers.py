@@ -1,0 +1,243 @@
+"""GPU parity of GMRES, the mixed-precision refinement, the batched solver and
+the adjoint against the CPU oracle."""
+import numpy as np
+import pytest
+
+import hala_amd as H
+import oracle
+from hala_amd import stencils as st
+from tests.conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(engine, a):
+    return engine.load(a)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def upload(engine, p, i, v):
+    return dev(engine, p), dev(engine, i), dev(engine, v)
+
+
+def dense(p, i, v, n):
+    A = np.zeros((n, n))
+    for r in range(n):
+        A[r, i[p[r]:p[r + 1]]] = v[p[r]:p[r + 1]]
+    return A
+
+
+# ---------------------------------------------------------------- GMRES
+def test_gmres_reference_small(engine):
+    # tests/solvers_tests.hpp:105-137
+    p = np.array([0, 2, 5, 8, 11, 13], np.int32)
+    i = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4], np.int32)
+    v = np.array([2, 1, 1, 2, 1, 1, 2, 1, 1, 2, 1, 1, 2], np.float64)
+    xref = np.arange(1.0, 6.0)
+    b = dense(p, i, v, 5) @ xref
+    gp, gi, gv = upload(engine, p, i, v)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    gx = engine.vector(5)
+    H.solve_gmres_ilu(engine, H.stop_criteria(0.0, 2), 2, gp, gi, gv, ilu, dev(engine, b), gx)
+    assert np.abs(host(gx) - xref).sum() < 1e-14
+    p = np.array([0, 2, 5, 8, 11, 12], np.int32)
+    i = np.array([0, 1, 0, 1, 3, 1, 2, 3, 0, 2, 3, 4], np.int32)
+    v = np.array([2.0, 1.0, 1.0, 3.0, 1.0, 1.0, 4.0, 1.0, 1.0, 1.0, 5.0, 2.0])
+    b = dense(p, i, v, 5) @ xref
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = engine.vector(5)
+    H.solve_gmres_ilu(engine, 1e-9, 6, gp, gi, gv, None, dev(engine, b), gx)
+    assert np.abs(host(gx) - xref).sum() < 1e-8
+
+
+@pytest.mark.parametrize("restart", [5, 30])
+def test_gmres_ilu_bitexact(engine, restart):
+    import torch
+
+    p, i, v = st.convdiff27(16, seed=3)
+    n = len(p) - 1
+    b = st.uniform(41, n)
+    dg, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, dg)
+    tol = 1e-10 * oracle.cnrm2(oracle.ilu_apply(p, i, dg, f, b))
+    tot, x, hist = oracle.gmres(p, i, v, b, tol=tol, restart=restart, prec=oracle.PREC_ILU, dg=dg, ilu=f)
+    gp, gi, gv = upload(engine, p, i, v)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    gx = engine.vector(n)
+    gh = torch.zeros(4000, dtype=torch.float64, device=gx.device)
+    gtot, gnh = H.solve_gmres_ilu(engine, tol, restart, gp, gi, gv, ilu, dev(engine, b), gx, hist=gh)
+    assert gtot == tot
+    assert gnh == len(hist)
+    assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+
+
+def test_gmres_identity_nonzero_x0(engine):
+    p, i, v = st.convdiff27(10, seed=9)
+    n = len(p) - 1
+    b = st.uniform(42, n)
+    x0 = st.uniform(43, n)
+    tol = 1e-9 * oracle.cnrm2(b)
+    tot, x, hist = oracle.gmres(p, i, v, b, x0=x0, tol=tol, restart=20, prec=oracle.PREC_IDENTITY)
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = dev(engine, x0)
+    gtot, _ = H.solve_gmres(engine, tol, 20, gp, gi, gv, None, dev(engine, b), gx)
+    assert gtot == tot
+    assert np.array_equal(bits(host(gx)), bits(x))
+
+
+def test_gmres_ir_mixed_bitexact(engine):
+    import torch
+
+    p, i, v = st.convdiff27(16, seed=3)
+    n = len(p) - 1
+    b = st.uniform(44, n)
+    dg, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, dg)
+    tot, x, hist = oracle.gmres_ir32(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40)
+    gp, gi, gv = upload(engine, p, i, v)
+    gv32 = gv.float()
+    ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+    gx = engine.vector(n)
+    gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
+    gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv32, ilu, dev(engine, b), gx, rtol=1e-10, inner_rtol=1e-4,
+                                 restart=30, max_refine=40, hist=gh)
+    assert gtot == tot and gnh == len(hist)
+    assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    r = b - oracle.spmv("N", n, n, 1.0, p, i, v, host(gx), 0.0, np.zeros(n))
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+
+
+# ---------------------------------------------------------------- batched
+def _batch(engine, nsys, seed, lo, hi):
+    d = st.batch_laplace(nsys, seed=seed, side_lo=lo, side_hi=hi)
+    b = st.uniform(seed + 100, len(d["dvals"]))
+    g = {k: dev(engine, d[k]) for k in ("row_off", "pntr", "indx", "vals", "dvals")}
+    return d, b, g
+
+
+def _oracle_batch(d, b, rtol, team):
+    ro = d["row_off"]
+    xs, its = [], []
+    for s in range(len(ro) - 1):
+        r0, r1 = ro[s], ro[s + 1]
+        z0 = d["pntr"][r0]
+        p = d["pntr"][r0:r1 + 1] - z0
+        z1 = d["pntr"][r1]
+        tol = rtol * np.sqrt(oracle.team_dot(b[r0:r1], b[r0:r1], team))
+        it, x, _ = oracle.cg_team(p, d["indx"][z0:z1], d["vals"][z0:z1], d["dvals"][r0:r1], b[r0:r1], tol, team)
+        xs.append(x)
+        its.append(it)
+    return np.concatenate(xs), np.array(its)
+
+
+@pytest.mark.parametrize("team,wg", [(64, 4), (256, 2), (1024, 1)])
+def test_batch_pcg_bitexact(engine, team, wg):
+    import torch
+
+    d, b, g = _batch(engine, 48, 7, 14, 40)
+    bs = H.batch_solver(engine, g["row_off"], g["pntr"], g["indx"])
+    bs.set_shape(team, wg)
+    xo, ito = _oracle_batch(d, b, 1e-10, team)
+    gb = dev(engine, b)
+    iters = torch.zeros(48, dtype=torch.int32, device=gb.device)
+    for rep in range(3):  # graph replays, with eager work in between
+        gx = engine.vector(len(b))
+        bs.solve(g["vals"], g["dvals"], gb, gx, rtol=1e-10, iters=iters)
+        assert np.array_equal(host(iters), ito), rep
+        assert np.array_equal(bits(host(gx)), bits(xo)), rep
+        torch.cuda.synchronize()
+
+
+def test_batch_tuner(engine):
+    d, b, g = _batch(engine, 96, 8, 14, 70)
+    bs = H.batch_solver(engine, g["row_off"], g["pntr"], g["indx"])
+    gb = dev(engine, b)
+    gx = engine.vector(len(b))
+    team, wg, ms = bs.tune(g["vals"], g["dvals"], gb, gx, rtol=1e-10, reps=1)
+    assert (team, wg) == bs.shape() and ms > 0
+    gx = engine.vector(len(b))
+    bs.solve(g["vals"], g["dvals"], gb, gx, rtol=1e-10)
+    xo, _ = _oracle_batch(d, b, 1e-10, team)
+    assert np.array_equal(bits(host(gx)), bits(xo))
+
+
+# ---------------------------------------------------------------- adjoint
+def test_assemble_matches_generator(engine):
+    from hala_amd.autograd import coef_to_entry_map
+
+    d = st.diffusion3d(8, seed=5)
+    aptr, aidx, aw = coef_to_entry_map(len(d["indx"]), d["cptr"], d["cnnz"], d["ccoef"])
+    out = engine.vector(len(d["indx"]))
+    H.assemble(engine, dev(engine, aptr), dev(engine, aidx), dev(engine, aw), dev(engine, d["kappa"]), out)
+    assert np.array_equal(bits(host(out)), bits(d["vals"]))
+
+
+def _adjoint_oracle(d, f, gx, tol_f, tol_a):
+    p, i, v = d["pntr"], d["indx"], d["vals"]
+    n = len(p) - 1
+    dg, _ = oracle.diag(p, i)
+    fac = oracle.ilu0(p, i, v, dg)
+    _, x, _ = oracle.cg(p, i, v, f, tol=tol_f, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg, mdata=fac)
+    tp, ti, tv = oracle.transpose(n, n, p, i, v)
+    _, lam, _ = oracle.cg(tp, ti, tv, gx, tol=tol_a, prec=oracle.PREC_ILU_T, mpntr=p, mindx=i, mdiag=dg, mdata=fac)
+    gvals = oracle.grad_vals(p, i, lam, x)
+    return x, lam, oracle.grad_coef(d["cptr"], d["cnnz"], d["ccoef"], gvals)
+
+
+def test_adjoint_gradient_bitexact(engine):
+    import torch
+    from hala_amd.autograd import SparseLinearProblem, sparse_solve
+
+    m = 16
+    d = st.diffusion3d(m, seed=5)
+    dt = st.diffusion3d(m, seed=6)
+    f = st.gaussian_source(m)
+    prob = SparseLinearProblem(engine, d["pntr"], d["indx"], d["cptr"], d["cnnz"], d["ccoef"], f, rtol=1e-12)
+    kappa = dev(engine, d["kappa"]).requires_grad_(True)
+    # target from the second conductivity field
+    xt, _ = prob.forward_solve(dev(engine, dt["kappa"]))
+    xt = xt.clone()
+    x = sparse_solve(kappa, prob)
+    loss = 0.5 * torch.sum((x - xt) ** 2)
+    loss.backward()
+    gk = host(kappa.grad)
+    gx = host(x.detach() - xt)
+    tol_a = 1e-12 * float(torch.linalg.vector_norm(x.detach() - xt))
+    xo, lam, go = _adjoint_oracle(d, f, gx, prob.tol, tol_a)
+    assert np.array_equal(bits(host(x.detach())), bits(xo))
+    assert np.array_equal(bits(gk), bits(go))
+
+
+def test_adjoint_finite_differences(engine):
+    import torch
+    from hala_amd.autograd import SparseLinearProblem
+
+    m = 12
+    d = st.diffusion3d(m, seed=5)
+    dt = st.diffusion3d(m, seed=6)
+    f = st.gaussian_source(m)
+    prob = SparseLinearProblem(engine, d["pntr"], d["indx"], d["cptr"], d["cnnz"], d["ccoef"], f, rtol=1e-14)
+    xt, _ = prob.forward_solve(dev(engine, dt["kappa"]))
+    xt = xt.clone()
+    k0 = dev(engine, d["kappa"])
+
+    def loss(k):
+        x, _ = prob.forward_solve(k)
+        return 0.5 * float(torch.sum((x - xt) ** 2))
+
+    x, _ = prob.forward_solve(k0)
+    g = host(prob.gradient(k0, x - xt))
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for c in rng.choice(len(d["kappa"]), 8, replace=False):
+        kp, km = k0.clone(), k0.clone()
+        kp[c] += h
+        km[c] -= h
+        fd = (loss(kp) - loss(km)) / (2 * h)
+        assert abs(fd - g[c]) <= 1e-5 * max(abs(g).max(), 1e-30), (c, fd, g[c])
