@@ -447,9 +447,9 @@ int hala_gmres_ir_f64(hala_csr A, const double *vals, const float *vals32, hala_
                       double rtol, double inner_rtol, int restart, int max_refine, int *total, double *hist,
                       int max_hist, int *nhist) {
     return guard([&] {
-        HALA_REQUIRE(A && vals && vals32 && ilu && b && x, HALA_ERR_ARG, "hala_gmres_ir_f64: null argument");
-        HALA_REQUIRE(ilu->factored && ilu->has32, HALA_ERR_ARG,
-                     "hala_gmres_ir_f64: factor the ILU with want_f32 = 1");
+        HALA_REQUIRE(A && vals && ilu && b && x, HALA_ERR_ARG, "hala_gmres_ir_f64: null argument");
+        HALA_REQUIRE(ilu->factored && (!vals32 || ilu->has32), HALA_ERR_ARG,
+                     "hala_gmres_ir_f64: factor the ILU with want_f32 = 1 for the FP32 inner solve");
         HALA_REQUIRE(max_refine >= 0 && inner_rtol > 0.0 && inner_rtol < 1.0, HALA_ERR_ARG,
                      "hala_gmres_ir_f64: bad refinement parameters");
         hala_ctx c = A->ctx;

@@ -1,0 +1,578 @@
+// hala.hpp -- drop-in hala:: templates for the sparse iterative-solve path on
+// MI355X (gfx950).  The engine, device vector, CSR matrix, triangular matrix,
+// ILU(0) and BLAS-1 interfaces mirror the reference's (gpu/hala_gpu_engine.hpp,
+// gpu/hala_gpu_vector.hpp, gpu/hala_rocm_sparse_general.hpp,
+// gpu/hala_rocm_sparse_triangular.hpp, gpu/hala_gpu_ilu.hpp,
+// gpu/hala_gpu_blas1.hpp, gpu/hala_gpu_overloads.hpp); the work happens in
+// libhala_amd.so through the C ABI of hala_amd.h, so this header needs only a
+// C++14 compiler and the library (no HIP headers).  Library errors become
+// std::runtime_error, as in the reference (gpu/hala_rocm_common.hpp:88-109).
+//
+// Scope: real scalars (float, double), unit strides, the gpu_engine.  Vectors
+// on the device are hala::gpu_vector<T> or hala::gpu_wrapped_array<T>.
+#pragma once
+
+#include <cassert>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <valarray>
+#include <vector>
+
+#include "../hala_amd.h"
+
+namespace hala {
+
+inline void check_hala(int status, const char *where) {
+    if (status != HALA_OK)
+        throw std::runtime_error(std::string("libhala_amd encountered an error with code: ") + std::to_string(status) +
+                                 " at " + where + ": " + hala_last_error());
+}
+
+// runtime_assert: common/hala_core.hpp:140
+inline void runtime_assert(bool condition, const char *message) {
+    if (!condition) throw std::runtime_error(message);
+}
+
+inline int gpu_device_count() {
+    int c = 0;
+    check_hala(hala_device_count(&c), "hala_device_count()");
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+// gpu/hala_gpu_engine.hpp:167-307: device id plus the library context (stream).
+// Copies alias the context (non-owning), as the reference's copies alias handles.
+class gpu_engine {
+  public:
+    explicit gpu_engine(int deviceid = 0, void *stream = nullptr) : cgpu(deviceid) {
+        hala_ctx c = nullptr;
+        check_hala(hala_ctx_create(deviceid, stream, &c), "gpu_engine()");
+        ctx = std::shared_ptr<hala_ctx_s>(c, [](hala_ctx p) { hala_ctx_destroy(p); });
+    }
+    gpu_engine(gpu_engine const &other) = default;
+    gpu_engine(gpu_engine &&other) = default;
+    gpu_engine &operator=(gpu_engine const &) = default;
+    gpu_engine &operator=(gpu_engine &&) = default;
+
+    int device() const { return cgpu; }
+    void set_active_device() const {}  // every library call selects the engine's device itself
+    void synchronize() const { check_hala(hala_ctx_sync(ctx.get()), "gpu_engine::synchronize()"); }
+    void set_stream(void *stream) const { check_hala(hala_ctx_set_stream(ctx.get(), stream), "set_stream()"); }
+    hala_ctx handle() const { return ctx.get(); }
+
+    template <class VectorLike>
+    auto load(VectorLike const &cpu_data) const;
+    template <class VectorLike>
+    auto load(VectorLike const &cpu_data, size_t size) const;
+    template <class VectorLike>
+    auto unload(VectorLike const &gpu_data) const;
+    template <typename T>
+    auto vector(size_t num_entries, T value) const;
+
+  private:
+    int cgpu;
+    std::shared_ptr<hala_ctx_s> ctx;
+};
+
+// A process-wide engine per device for the device-vector plumbing (allocation,
+// copies); deliberately never destroyed, so no library call runs during the
+// static destruction at exit.
+inline hala_ctx default_ctx(int device) {
+    static auto *engines = new std::vector<gpu_engine *>();
+    if ((int)engines->size() <= device) engines->resize(device + 1, nullptr);
+    if (!(*engines)[device]) (*engines)[device] = new gpu_engine(device);
+    return (*engines)[device]->handle();
+}
+
+// ---------------------------------------------------------------------------
+// host vector adapters (the subset of common/hala_vector_defines.hpp used here)
+// ---------------------------------------------------------------------------
+template <class V>
+struct host_traits;
+template <typename T, class A>
+struct host_traits<std::vector<T, A>> {
+    using value_type = T;
+    static T *data(std::vector<T, A> &v) { return v.data(); }
+    static T const *data(std::vector<T, A> const &v) { return v.data(); }
+    static size_t size(std::vector<T, A> const &v) { return v.size(); }
+    static void resize(std::vector<T, A> &v, size_t n) { v.resize(n); }
+};
+template <typename T>
+struct host_traits<std::valarray<T>> {
+    using value_type = T;
+    static T *data(std::valarray<T> &v) { return std::begin(v); }
+    static T const *data(std::valarray<T> const &v) { return std::begin(v); }
+    static size_t size(std::valarray<T> const &v) { return v.size(); }
+    static void resize(std::valarray<T> &v, size_t n) {
+        if (v.size() != n) v.resize(n);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// device vector
+// ---------------------------------------------------------------------------
+// gpu/hala_gpu_vector.hpp:49-171: owning device buffer.
+template <typename T>
+class gpu_vector {
+  public:
+    using value_type = T;
+    explicit gpu_vector(int deviceid = 0) : gpu(deviceid) {}
+    gpu_vector(size_t num_entries, int deviceid) : gpu(deviceid) { alloc(num_entries); }
+    gpu_vector(gpu_vector const &other) : gpu(other.gpu) {
+        alloc(other.num);
+        copy_d2d(other.ptr, ptr, num);
+    }
+    gpu_vector(gpu_vector &&other) noexcept
+        : gpu(other.gpu), num(std::exchange(other.num, 0)), ptr(std::exchange(other.ptr, nullptr)) {}
+    ~gpu_vector() { clear(); }
+    gpu_vector &operator=(gpu_vector const &other) {
+        if (this != &other) {
+            resize(other.num);
+            copy_d2d(other.ptr, ptr, num);
+        }
+        return *this;
+    }
+    gpu_vector &operator=(gpu_vector &&other) noexcept {
+        std::swap(gpu, other.gpu);
+        std::swap(num, other.num);
+        std::swap(ptr, other.ptr);
+        return *this;
+    }
+    void clear() {
+        if (ptr) hala_free(default_ctx(gpu), ptr);
+        ptr = nullptr;
+        num = 0;
+    }
+    void resize(size_t n) {
+        if (n == num) return;
+        clear();
+        alloc(n);
+    }
+    template <class VectorLike>
+    void load(VectorLike const &cpu) {
+        using H = host_traits<std::remove_cv_t<VectorLike>>;
+        static_assert(std::is_same<T, typename H::value_type>::value, "type mismatch in gpu_vector::load()");
+        resize(H::size(cpu));
+        if (num)
+            check_hala(hala_memcpy(default_ctx(gpu), ptr, H::data(cpu), num * sizeof(T), 0), "gpu_vector::load()");
+    }
+    template <class VectorLike>
+    void unload(VectorLike &cpu) const {
+        using H = host_traits<VectorLike>;
+        static_assert(std::is_same<T, typename H::value_type>::value, "type mismatch in gpu_vector::unload()");
+        H::resize(cpu, num);
+        if (num)
+            check_hala(hala_memcpy(default_ctx(gpu), H::data(cpu), ptr, num * sizeof(T), 1), "gpu_vector::unload()");
+    }
+    std::vector<T> unload() const {
+        std::vector<T> v(num);
+        unload(v);
+        return v;
+    }
+    void fill(T value) {
+        if (!num) return;
+        hala_ctx c = default_ctx(gpu);
+        if (std::is_same<T, double>::value)
+            check_hala(hala_fill_f64(c, reinterpret_cast<double *>(ptr), (int64_t)num, (double)value), "fill()");
+        else if (std::is_same<T, float>::value)
+            check_hala(hala_fill_f32(c, reinterpret_cast<float *>(ptr), (int64_t)num, (float)value), "fill()");
+        else
+            check_hala(hala_fill_i32(c, reinterpret_cast<int *>(ptr), (int64_t)num, (int)value), "fill()");
+    }
+    T *data() { return ptr; }
+    T const *data() const { return ptr; }
+    size_t size() const { return num; }
+    int device() const { return gpu; }
+    bool empty() const { return num == 0; }
+
+  private:
+    void alloc(size_t n) {
+        num = n;
+        ptr = nullptr;
+        if (n) {
+            void *p = nullptr;
+            check_hala(hala_malloc(default_ctx(gpu), n * sizeof(T), &p), "gpu_vector alloc");
+            ptr = static_cast<T *>(p);
+        }
+    }
+    void copy_d2d(T const *src, T *dst, size_t n) const {
+        if (n) check_hala(hala_memcpy(default_ctx(gpu), dst, src, n * sizeof(T), 2), "gpu_vector copy");
+    }
+    int gpu = 0;
+    size_t num = 0;
+    T *ptr = nullptr;
+};
+
+// gpu/hala_gpu_wrap_array.hpp:57-80: non-owning view of device memory.
+template <typename T>
+struct gpu_wrapped_array {
+    using value_type = std::remove_cv_t<T>;
+    T *ptr;
+    size_t num;
+    int gpu;
+    T *data() const { return ptr; }
+    size_t size() const { return num; }
+    int device() const { return gpu; }
+    void resize(size_t n) { runtime_assert(n <= num, "cannot grow a wrapped device array"); }
+};
+template <typename T>
+gpu_wrapped_array<T> wrap_gpu_array(T *array, size_t num_entries, int device = 0) {
+    return {array, num_entries, device};
+}
+
+template <class VectorLike>
+auto make_gpu_vector(VectorLike const &cpu_data, int gpuid = 0) {
+    gpu_vector<typename host_traits<VectorLike>::value_type> v(gpuid);
+    v.load(cpu_data);
+    return v;
+}
+template <typename T>
+auto make_gpu_vector(size_t num_entries, int gpuid = 0) {
+    return gpu_vector<T>(num_entries, gpuid);
+}
+
+template <class VectorLike>
+auto gpu_engine::load(VectorLike const &cpu_data) const {
+    return make_gpu_vector(cpu_data, cgpu);
+}
+template <class VectorLike>
+auto gpu_engine::load(VectorLike const &cpu_data, size_t size) const {
+    using T = typename host_traits<VectorLike>::value_type;
+    std::vector<T> tmp(host_traits<VectorLike>::data(cpu_data), host_traits<VectorLike>::data(cpu_data) + size);
+    return make_gpu_vector(tmp, cgpu);
+}
+template <class VectorLike>
+auto gpu_engine::unload(VectorLike const &gpu_data) const {
+    return gpu_data.unload();
+}
+template <typename T>
+auto gpu_engine::vector(size_t num_entries, T value) const {
+    gpu_vector<T> x(num_entries, cgpu);
+    x.fill(value);
+    return x;
+}
+
+// binded_gpu_vector (gpu/hala_gpu_vector.hpp:224-249): loads on construction,
+// copies back on destruction.
+template <typename T, class VectorLike>
+struct binded_gpu_vector {
+    using value_type = T;
+    binded_gpu_vector(int gpuid, VectorLike &v) : vec(v), cvec(gpuid) { cvec.load(v); }
+    ~binded_gpu_vector() { cvec.unload(vec); }
+    T *data() { return cvec.data(); }
+    T const *data() const { return cvec.data(); }
+    void resize(size_t n) { cvec.resize(n); }
+    size_t size() const { return cvec.size(); }
+    int device() const { return cvec.device(); }
+    VectorLike &vec;
+    gpu_vector<T> cvec;
+};
+template <class VectorLike>
+auto gpu_bind_vector(gpu_engine const &e, VectorLike &x) {
+    return binded_gpu_vector<typename host_traits<VectorLike>::value_type, VectorLike>(e.device(), x);
+}
+
+// new_vector / set_zero on the engine (gpu/hala_gpu_engine.hpp:327-352)
+template <class VectorLike>
+auto new_vector(gpu_engine const &engine, VectorLike const &) {
+    using T = std::remove_cv_t<typename std::remove_reference_t<VectorLike>::value_type>;
+    return gpu_vector<T>(engine.device());
+}
+template <class VectorLike>
+void set_zero(gpu_engine const &engine, size_t num_entries, VectorLike &&x) {
+    if (num_entries == 0) return;
+    if (x.size() < num_entries) x.resize(num_entries);
+    using T = std::remove_cv_t<typename std::remove_reference_t<VectorLike>::value_type>;
+    if (std::is_same<T, double>::value)
+        check_hala(hala_fill_f64(engine.handle(), reinterpret_cast<double *>(x.data()), (int64_t)num_entries, 0.0),
+                   "set_zero()");
+    else
+        check_hala(hala_fill_f32(engine.handle(), reinterpret_cast<float *>(x.data()), (int64_t)num_entries, 0.0f),
+                   "set_zero()");
+}
+
+template <class V>
+using scalar_of = std::remove_cv_t<typename std::remove_reference_t<V>::value_type>;
+
+// ---------------------------------------------------------------------------
+// BLAS-1 on the engine (gpu/hala_gpu_blas1.hpp, gpu/hala_gpu_overloads.hpp)
+// ---------------------------------------------------------------------------
+template <class VX, class VY>
+void vcopy(gpu_engine const &engine, int N, VX const &x, int incx, VY &&y, int incy) {
+    runtime_assert(incx == 1 && incy == 1, "hala_amd: unit strides only");
+    if ((int)y.size() < N) y.resize(N);
+    using T = scalar_of<VX>;
+    if (std::is_same<T, double>::value)
+        check_hala(hala_copy_f64(engine.handle(), N, (double const *)x.data(), (double *)y.data()), "vcopy()");
+    else
+        check_hala(hala_copy_f32(engine.handle(), N, (float const *)x.data(), (float *)y.data()), "vcopy()");
+}
+template <class VX, class VY>
+void vcopy(gpu_engine const &engine, VX const &x, VY &&y) {
+    vcopy(engine, (int)x.size(), x, 1, y, 1);
+}
+template <class VX>
+auto norm2(gpu_engine const &engine, VX const &x) {
+    using T = scalar_of<VX>;
+    if (std::is_same<T, double>::value) {
+        double r;
+        check_hala(hala_nrm2_f64(engine.handle(), (int64_t)x.size(), (double const *)x.data(), &r), "norm2()");
+        return (T)r;
+    }
+    float r;
+    check_hala(hala_nrm2_f32(engine.handle(), (int64_t)x.size(), (float const *)x.data(), &r), "norm2()");
+    return (T)r;
+}
+template <class VX, class VY>
+auto dot(gpu_engine const &engine, VX const &x, VY const &y) {
+    using T = scalar_of<VX>;
+    if (std::is_same<T, double>::value) {
+        double r;
+        check_hala(hala_dot_f64(engine.handle(), (int64_t)x.size(), (double const *)x.data(), (double const *)y.data(),
+                                &r),
+                   "dot()");
+        return (T)r;
+    }
+    float r;
+    check_hala(hala_dot_f32(engine.handle(), (int64_t)x.size(), (float const *)x.data(), (float const *)y.data(), &r),
+               "dot()");
+    return (T)r;
+}
+template <typename FP, class VX, class VY>
+void axpy(gpu_engine const &engine, FP alpha, VX const &x, VY &&y) {
+    using T = scalar_of<VX>;
+    if (std::is_same<T, double>::value)
+        check_hala(hala_axpy_f64(engine.handle(), (int64_t)x.size(), (double)alpha, (double const *)x.data(),
+                                 (double *)y.data()),
+                   "axpy()");
+    else
+        check_hala(hala_axpy_f32(engine.handle(), (int64_t)x.size(), (float)alpha, (float const *)x.data(),
+                                 (float *)y.data()),
+                   "axpy()");
+}
+template <typename FP, class VX>
+void scal(gpu_engine const &engine, FP alpha, VX &&x) {
+    using T = scalar_of<VX>;
+    if (std::is_same<T, double>::value)
+        check_hala(hala_scal_f64(engine.handle(), (int64_t)x.size(), (double)alpha, (double *)x.data()), "scal()");
+    else
+        check_hala(hala_scal_f32(engine.handle(), (int64_t)x.size(), (float)alpha, (float *)x.data()), "scal()");
+}
+// dense column-major gemv (gpu/hala_gpu_blas2.hpp:40-60), FP64
+template <class VA, class VX, class VY>
+void gemv(gpu_engine const &engine, char trans, int M, int N, double alpha, VA const &A, VX const &x, double beta,
+          VY &&y) {
+    int need = (trans == 'N' || trans == 'n') ? M : N;
+    if ((int)y.size() < need) y.resize(need);
+    check_hala(hala_gemv_f64(engine.handle(), trans, M, N, alpha, (double const *)A.data(), M > 1 ? M : 1,
+                             (double const *)x.data(), beta, (double *)y.data()),
+               "gemv()");
+}
+
+// ---------------------------------------------------------------------------
+// CSR matrix (gpu/hala_rocm_sparse_general.hpp:74-199)
+// ---------------------------------------------------------------------------
+template <typename T>
+class gpu_sparse_matrix {
+  public:
+    using value_type = T;
+    using engine_type = gpu_engine;
+    template <class VP, class VI, class VV>
+    gpu_sparse_matrix(gpu_engine const &engine, int num_rows, int num_cols, int num_nz, VP const &pntr, VI const &indx,
+                      VV const &vals)
+        : rengine(engine), rvals(vals.data()), rows(num_rows), cols(num_cols), nnz(num_nz) {
+        hala_csr h = nullptr;
+        check_hala(hala_csr_create(engine.handle(), num_rows, num_cols, num_nz, pntr.data(), indx.data(), &h),
+                   "make_sparse_matrix()");
+        csr = std::shared_ptr<hala_csr_s>(h, [](hala_csr p) { hala_csr_destroy(p); });
+    }
+    template <class VP, class VI, class VV>
+    gpu_sparse_matrix(gpu_engine const &engine, int num_cols, VP const &pntr, VI const &indx, VV const &vals)
+        : gpu_sparse_matrix(engine, (int)pntr.size() - 1, num_cols, (int)indx.size(), pntr, indx, vals) {}
+
+    gpu_engine const &engine() const { return rengine; }
+    hala_csr handle() const { return csr.get(); }
+    int num_rows() const { return rows; }
+
+    template <typename FPa, class VX, typename FPb, class VY>
+    size_t gemv_buffer_size(char, FPa, VX const &, FPb, VY &&) const {
+        return 0;
+    }
+    template <typename FPa, class VX, typename FPb, class VY, class VB>
+    void gemv(char trans, FPa alpha, VX const &x, FPb beta, VY &&y, VB &&) const {
+        gemv(trans, alpha, x, beta, y);
+    }
+    template <typename FPa, class VX, typename FPb, class VY>
+    void gemv(char trans, FPa alpha, VX const &x, FPb beta, VY &&y) const {
+        size_t out = (trans == 'N' || trans == 'n') ? rows : cols;
+        if (y.size() < out) {
+            runtime_assert(beta == 0, "gemv: y too small for beta != 0");
+            y.resize(out);
+        }
+        if (std::is_same<T, double>::value)
+            check_hala(hala_spmv_f64(csr.get(), trans, (double)alpha, (double const *)rvals,
+                                     (double const *)x.data(), (double)beta, (double *)y.data()),
+                       "gpu_sparse_matrix::gemv()");
+        else
+            check_hala(hala_spmv_f32(csr.get(), trans, (float)alpha, (float const *)rvals, (float const *)x.data(),
+                                     (float)beta, (float *)y.data()),
+                       "gpu_sparse_matrix::gemv()");
+    }
+    T const *values() const { return rvals; }
+
+  private:
+    gpu_engine rengine;
+    T const *rvals;
+    int rows, cols, nnz;
+    std::shared_ptr<hala_csr_s> csr;
+};
+
+template <class VP, class VI, class VV>
+auto make_sparse_matrix(gpu_engine const &engine, int num_rows, int num_cols, int num_nz, VP const &pntr,
+                        VI const &indx, VV const &vals) {
+    return gpu_sparse_matrix<scalar_of<VV>>(engine, num_rows, num_cols, num_nz, pntr, indx, vals);
+}
+template <class VP, class VI, class VV>
+auto make_sparse_matrix(gpu_engine const &engine, int num_cols, VP const &pntr, VI const &indx, VV const &vals) {
+    return gpu_sparse_matrix<scalar_of<VV>>(engine, num_cols, pntr, indx, vals);
+}
+template <typename FPa, class VP, class VI, class VV, class VX, typename FPb, class VY>
+void sparse_gemv(gpu_engine const &engine, char trans, int M, int N, FPa alpha, VP const &pntr, VI const &indx,
+                 VV const &vals, VX const &x, FPb beta, VY &&y) {
+    make_sparse_matrix(engine, M, N, (int)indx.size(), pntr, indx, vals).gemv(trans, alpha, x, beta, y);
+}
+template <class M, typename FPa, class VX, typename FPb, class VY, class VB>
+void sparse_gemv(M const &matrix, char trans, FPa alpha, VX const &x, FPb beta, VY &&y, VB &&buf) {
+    matrix.gemv(trans, alpha, x, beta, y, buf);
+}
+template <class M, typename FPa, class VX, typename FPb, class VY>
+size_t sparse_gemv_buffer_size(M const &matrix, char trans, FPa alpha, VX const &x, FPb beta, VY &&y) {
+    return matrix.gemv_buffer_size(trans, alpha, x, beta, y);
+}
+
+// ---------------------------------------------------------------------------
+// triangular matrix (gpu/hala_rocm_sparse_triangular.hpp:29-194)
+// ---------------------------------------------------------------------------
+template <typename T>
+class gpu_triangular_matrix {
+  public:
+    using value_type = T;
+    using engine_type = gpu_engine;
+    template <class VP, class VI, class VV>
+    gpu_triangular_matrix(gpu_engine const &engine, char uplo, char diag, VP const &pntr, VI const &indx,
+                          VV const &vals, char /*policy*/ = 'N')
+        : rengine(engine), rvals(vals.data()), n((int)pntr.size() - 1) {
+        static_assert(std::is_same<T, double>::value, "hala_amd: triangular solves are FP64");
+        hala_tri h = nullptr;
+        check_hala(hala_tri_create(engine.handle(), uplo, diag, n, (int64_t)indx.size(), pntr.data(), indx.data(), &h),
+                   "make_triangular_matrix()");
+        tri = std::shared_ptr<hala_tri_s>(h, [](hala_tri p) { hala_tri_destroy(p); });
+    }
+    template <typename FP, class VB, class VX>
+    size_t trsv_buffer_size(char, FP, VB const &, VX &&) const {
+        return 0;
+    }
+    template <typename FP, class VB, class VX, class VW>
+    void trsv(char trans, FP alpha, VB const &b, VX &&x, VW &&) const {
+        trsv(trans, alpha, b, x);
+    }
+    template <typename FP, class VB, class VX>
+    void trsv(char trans, FP alpha, VB const &b, VX &&x) const {
+        if ((int)x.size() < n) x.resize(n);
+        check_hala(hala_trsv_f64(tri.get(), trans, (double)alpha, (double const *)rvals, (double const *)b.data(),
+                                 (double *)x.data()),
+                   "gpu_triangular_matrix::trsv()");
+    }
+
+  private:
+    gpu_engine rengine;
+    T const *rvals;
+    int n;
+    std::shared_ptr<hala_tri_s> tri;
+};
+template <class VP, class VI, class VV>
+auto make_triangular_matrix(gpu_engine const &engine, char uplo, char diag, VP const &pntr, VI const &indx,
+                            VV const &vals, char policy = 'N') {
+    return gpu_triangular_matrix<scalar_of<VV>>(engine, uplo, diag, pntr, indx, vals, policy);
+}
+template <class TM, typename FP, class VB, class VX>
+void sparse_trsv(char trans, TM const &tri, FP alpha, VB const &b, VX &&x) {
+    tri.trsv(trans, alpha, b, x);
+}
+
+// ---------------------------------------------------------------------------
+// ILU(0) (gpu/hala_gpu_ilu.hpp:45-195, wax/hala_sparse_extensions.hpp:51-107)
+// ---------------------------------------------------------------------------
+template <typename T>
+class gpu_ilu {
+  public:
+    using value_type = T;
+    using engine_type = gpu_engine;
+    template <class VP, class VI, class VV>
+    gpu_ilu(gpu_engine const &engine, VP const &pntr, VI const &indx, VV const &vals, char policy = 'N',
+            bool want_f32 = false)
+        : rengine(engine), n((int)pntr.size() - 1), rpolicy(policy) {
+        static_assert(std::is_same<T, double>::value, "hala_amd: ILU factors are computed in FP64");
+        hala_ilu h = nullptr;
+        check_hala(hala_ilu_create(engine.handle(), n, (int64_t)indx.size(), pntr.data(), indx.data(), &h),
+                   "make_ilu()");
+        ilu = std::shared_ptr<hala_ilu_s>(h, [](hala_ilu p) { hala_ilu_destroy(p); });
+        check_hala(hala_ilu_factor_f64(h, (double const *)vals.data(), want_f32 ? 1 : 0), "make_ilu()");
+    }
+    template <class VX, class VR>
+    size_t buffer_size(VX const &, VR &&, int) const {
+        return 0;
+    }
+    template <class VX, class VR>
+    gpu_vector<T> get_temp_buffer(VX const &, VR &&, int) const {
+        return gpu_vector<T>(rengine.device());
+    }
+    template <class VX, class VR, class VT>
+    void apply(VX const &x, VR &&r, int num_rhs, VT &&) const {
+        apply(x, r, num_rhs);
+    }
+    template <class VX, class VR>
+    void apply(VX const &x, VR &&r, int num_rhs = 1) const {
+        runtime_assert(num_rhs == 1, "hala_amd: one right-hand side per ILU apply");
+        if ((int)r.size() < n) r.resize(n);
+        check_hala(hala_ilu_apply_f64(ilu.get(), (double const *)x.data(), (double *)r.data()), "gpu_ilu::apply()");
+    }
+    gpu_engine const &engine() const { return rengine; }
+    hala_ilu handle() const { return ilu.get(); }
+
+  private:
+    gpu_engine rengine;
+    int n;
+    char rpolicy;
+    std::shared_ptr<hala_ilu_s> ilu;
+};
+
+template <class VP, class VI, class VV>
+auto make_ilu(gpu_engine const &engine, VP const &pntr, VI const &indx, VV const &vals, char policy = 'N') {
+    return gpu_ilu<scalar_of<VV>>(engine, pntr, indx, vals, policy);
+}
+template <class ILU, class VX, class VR, class VT>
+void ilu_apply(ILU const &ilu, VX const &x, VR &&r, int num_rhs, VT &&tmp) {
+    ilu.apply(x, r, num_rhs, tmp);
+}
+template <class ILU, class VX, class VR>
+void ilu_apply(ILU const &ilu, VX const &x, VR &&r, int num_rhs = 1) {
+    ilu.apply(x, r, num_rhs);
+}
+template <class ILU, class VX, class VR>
+auto ilu_temp_buffer(ILU const &ilu, VX const &x, VR &&r, int num_rhs) {
+    return ilu.get_temp_buffer(x, r, num_rhs);
+}
+template <class ILU, class VX, class VR>
+size_t ilu_buffer_size(ILU const &ilu, VX const &x, VR &&r, int num_rhs) {
+    return ilu.buffer_size(x, r, num_rhs);
+}
+
+}  // namespace hala

@@ -1,0 +1,275 @@
+// hala_solvers.hpp -- drop-in hala:: Krylov solvers on the gpu_engine
+// (hex/solvers/hala_solvers_core.hpp, hala_solvers_cg.hpp, hala_solvers_gmres.hpp).
+//
+// solve_cg_ilu / solve_gmres_ilu run entirely on the device (one host
+// synchronisation per batch of passes / per restart cycle).  solve_cg and
+// solve_gmres with a user preconditioner follow the reference's host-driven
+// loop over engine BLAS calls with the same step order.
+#pragma once
+
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "hala.hpp"
+
+namespace hala {
+
+// hex/solvers/hala_solvers_core.hpp:52-71
+template <typename precision>
+struct stop_criteria {
+    int max_iter;
+    precision tol;
+    stop_criteria(int max_iterations) : max_iter(max_iterations), tol(0.0) {}
+    template <typename U, typename = std::enable_if_t<std::is_floating_point<U>::value>>
+    stop_criteria(U const &tolerance) : max_iter(std::numeric_limits<int>::max()), tol((precision)tolerance) {}
+    template <typename U>
+    stop_criteria(U const &tolerance, int max_iterations) : max_iter(max_iterations), tol((precision)tolerance) {}
+};
+
+// hex/solvers/hala_solvers_core.hpp:80-143 (gpu_engine form)
+template <typename T>
+using preconditioner = std::function<void(gpu_vector<T> const &x, gpu_vector<T> &r)>;
+
+template <class VX>
+void prepare_x(gpu_engine const &engine, int n, VX &x) {
+    if ((int)x.size() < n) {
+        x.resize(n);
+        set_zero(engine, (size_t)n, x);
+    }
+}
+
+// ---- CG ------------------------------------------------------------------
+// solve_cg(engine, stop, pntr, indx, vals, precon, b, x):
+// hex/solvers/hala_solvers_cg.hpp:181-244 with solve_cg_core :92-156.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+             VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && (int)b.size() == n, "solve_cg: size mismatch");
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    gpu_vector<double> r(engine.device()), p(engine.device()), ap(engine.device()), z(engine.device());
+    vcopy(engine, b, r);
+    matrix.gemv('N', 1.0, x, 0.0, p);
+    axpy(engine, -1.0, p, r);
+    precon(r, z);
+    vcopy(engine, z, p);
+    double zr = dot(engine, r, z);
+    int iterations = 1;
+    for (;;) {
+        matrix.gemv('N', 1.0, p, 0.0, ap);
+        iterations++;
+        double nzr = dot(engine, p, ap);
+        double a = zr / nzr;
+        axpy(engine, a, p, x);
+        axpy(engine, -a, ap, r);
+        if (iterations == stop.max_iter || norm2(engine, r) < stop.tol) break;
+        precon(r, z);
+        nzr = dot(engine, r, z);
+        a = nzr / zr;
+        scal(engine, a, p);
+        axpy(engine, 1.0, z, p);
+        zr = nzr;
+    }
+    return iterations;
+}
+
+// solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x): hex/solvers/hala_solvers_cg.hpp:312-326.
+// Runs device-resident (hala_cg_f64); identical arithmetic to solve_cg with
+// the ILU apply as preconditioner.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && (int)b.size() == n, "solve_cg_ilu: size mismatch");
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    int it = 0;
+    check_hala(hala_cg_f64(matrix.handle(), (double const *)vals.data(), 1, ilu.handle(), nullptr,
+                           (double const *)b.data(), (double *)x.data(), stop.tol, stop.max_iter, &it, nullptr, 0),
+               "solve_cg_ilu()");
+    return it;
+}
+
+// solve_cg_ilu(engine, stop, pntr, indx, vals, b, x, policy): factor on the fly (:354-361).
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, VB const &b, VX &x, char policy = 'N') {
+    auto ilu = make_ilu(engine, pntr, indx, vals, policy);
+    return solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x);
+}
+
+// Device-resident CG with the identity (z = r) or Jacobi (z = r ./ d) preconditioner.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_device(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                    VV const &vals, double const *jacobi_diag, VB const &b, VX &x) {
+    const int n = (int)pntr.size() - 1;
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    int it = 0;
+    check_hala(hala_cg_f64(matrix.handle(), (double const *)vals.data(), jacobi_diag ? 2 : 0, nullptr, jacobi_diag,
+                           (double const *)b.data(), (double *)x.data(), stop.tol, stop.max_iter, &it, nullptr, 0),
+               "solve_cg_device()");
+    return it;
+}
+
+// ---- GMRES ----------------------------------------------------------------
+namespace detail {
+// reference BLAS drotg / drot / dtpsv('U','N','N') on the host (the reference
+// calls them on the CPU even with a GPU engine, hex/solvers/hala_solvers_gmres.hpp:196-222)
+inline void rotg(double &a, double &b, double &c, double &s) {
+    double sa = a, sb = b;
+    double roe = (std::fabs(sa) > std::fabs(sb)) ? sa : sb;
+    double scale = std::fabs(sa) + std::fabs(sb), r, z;
+    if (scale == 0.0) {
+        c = 1.0;
+        s = 0.0;
+        r = 0.0;
+        z = 0.0;
+    } else {
+        double as = sa / scale, bs = sb / scale;
+        r = scale * std::sqrt(as * as + bs * bs);
+        r = (roe >= 0.0) ? r : -r;
+        c = sa / r;
+        s = sb / r;
+        z = 1.0;
+        if (std::fabs(sa) > std::fabs(sb)) z = s;
+        if (std::fabs(sb) >= std::fabs(sa) && c != 0.0) z = 1.0 / c;
+    }
+    a = r;
+    b = z;
+}
+inline void rot(double &x, double &y, double c, double s) {
+    double t = c * x + s * y;
+    y = c * y - s * x;
+    x = t;
+}
+inline void tpsv_un(int n, std::vector<double> const &ap, std::vector<double> &x) {
+    int kk = n * (n + 1) / 2 - 1;
+    for (int j = n - 1; j >= 0; j--) {
+        if (x[j] != 0.0) {
+            x[j] = x[j] / ap[kk];
+            double t = x[j];
+            int k = kk - 1;
+            for (int i = j - 1; i >= 0; i--) {
+                x[i] = x[i] - t * ap[k];
+                k--;
+            }
+        }
+        kk -= j + 1;
+    }
+}
+}  // namespace detail
+
+// solve_gmres(engine, stop, restart, pntr, indx, vals, precon, b, x):
+// hex/solvers/hala_solvers_gmres.hpp:125-230 (host Givens, device vectors).
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                VI const &indx, VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && (int)b.size() == n && restart > 0, "solve_gmres: bad arguments");
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    std::vector<double> H, S, C, Z;
+    double inner_res = 0.0, outer_res = stop.tol + 1.0;
+    int total = 0, outer = 0;
+    gpu_vector<double> t(n, engine.device()), r(n, engine.device()), W((size_t)n * restart, engine.device());
+    gpu_vector<double> cd(engine.device()), zd(engine.device());
+    while (outer_res > stop.tol && outer < stop.max_iter) {
+        H.clear();
+        S.clear();
+        C.clear();
+        Z.clear();
+        vcopy(engine, b, t);
+        matrix.gemv('N', -1.0, x, 1.0, t);
+        precon(t, r);
+        total++;
+        inner_res = norm2(engine, r);
+        scal(engine, 1.0 / inner_res, r);
+        Z.push_back(inner_res);
+        check_hala(hala_copy_f64(engine.handle(), n, r.data(), W.data()), "solve_gmres()");
+        int k = 0;
+        while (inner_res > stop.tol && k < restart) {
+            matrix.gemv('N', 1.0, r, 0.0, t);
+            precon(t, r);
+            total++;
+            auto Wk = wrap_gpu_array(W.data(), (size_t)n * (k + 1), engine.device());
+            cd.resize(k + 1);
+            gemv(engine, 'T', n, k + 1, 1.0, Wk, r, 0.0, cd);
+            gemv(engine, 'N', n, k + 1, -1.0, Wk, cd, 1.0, r);
+            std::vector<double> coeffs = cd.unload();
+            double nrm = norm2(engine, r);
+            scal(engine, 1.0 / nrm, r);
+            for (int i = 0; i < k; i++) detail::rot(coeffs[i], coeffs[i + 1], C[i], S[i]);
+            double c, s, beta = nrm;
+            detail::rotg(coeffs[k], beta, c, s);
+            H.insert(H.end(), coeffs.begin(), coeffs.end());
+            S.push_back(s);
+            C.push_back(c);
+            inner_res = std::fabs(S.back() * Z.back());
+            k++;
+            if (inner_res > stop.tol && k < restart) {
+                check_hala(hala_copy_f64(engine.handle(), n, r.data(), W.data() + (size_t)k * n), "solve_gmres()");
+                Z.push_back(0.0);
+                detail::rot(Z[k - 1], Z[k], C.back(), S.back());
+            }
+        }
+        if (!H.empty()) {
+            detail::tpsv_un((int)Z.size(), H, Z);
+            zd.load(Z);
+            auto Wk = wrap_gpu_array(W.data(), (size_t)n * Z.size(), engine.device());
+            gemv(engine, 'N', n, (int)Z.size(), 1.0, Wk, zd, 1.0, x);
+        }
+        outer++;
+        outer_res = inner_res;
+    }
+    return total;
+}
+
+// solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x):
+// hex/solvers/hala_solvers_gmres.hpp:250-267, device-resident (hala_gmres_f64).
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                    VI const &indx, VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && (int)b.size() == n, "solve_gmres_ilu: size mismatch");
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    int total = 0;
+    check_hala(hala_gmres_f64(matrix.handle(), (double const *)vals.data(), 1, ilu.handle(),
+                              (double const *)b.data(), (double *)x.data(), stop.tol, restart, stop.max_iter, &total,
+                              nullptr, 0, nullptr),
+               "solve_gmres_ilu()");
+    return total;
+}
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                    VI const &indx, VV const &vals, VB const &b, VX &x, char policy = 'N') {
+    auto ilu = make_ilu(engine, pntr, indx, vals, policy);
+    return solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x);
+}
+
+// ---- new entry points (no reference counterpart) ------------------------------
+// FP32 ILU(0) + FP32 product inside GMRES(restart), FP64 iterative refinement to
+// norm2(b - A x) < rtol norm2(b).  Returns the inner preconditioner applications.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ir(gpu_engine const &engine, double rtol, int restart, VP const &pntr, VI const &indx,
+                   VV const &vals, VB const &b, VX &x, double inner_rtol = 1e-4, int max_refine = 50) {
+    const int n = (int)pntr.size() - 1;
+    prepare_x(engine, n, x);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    gpu_ilu<double> ilu(engine, pntr, indx, vals, 'N', true);
+    gpu_vector<float> v32(vals.size(), engine.device());
+    check_hala(hala_convert_f64_f32(engine.handle(), (int64_t)vals.size(), (double const *)vals.data(), v32.data()),
+               "solve_gmres_ir()");
+    int total = 0;
+    check_hala(hala_gmres_ir_f64(matrix.handle(), (double const *)vals.data(), v32.data(), ilu.handle(),
+                                 (double const *)b.data(), (double *)x.data(), rtol, inner_rtol, restart, max_refine,
+                                 &total, nullptr, 0, nullptr),
+               "solve_gmres_ir()");
+    return total;
+}
+
+}  // namespace hala

@@ -187,7 +187,9 @@ int hala_gmres_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const
                    double tol, int restart, int max_outer, int *total, double *hist, int max_hist, int *nhist);
 /* Mixed precision (new entry point): FP32 ILU(0) and FP32 product inside
  * GMRES(restart), FP64 Arnoldi/Givens/dots and FP64 iterative refinement to
- * norm2(b - A x) < rtol * norm2(b).  ilu must be factored with want_f32. */
+ * norm2(b - A x) < rtol * norm2(b).  ilu must be factored with want_f32.
+ * vals32 == NULL runs the same refinement with the FP64 product and ILU inside
+ * (the all-FP64 path the mixed one is compared with). */
 int hala_gmres_ir_f64(hala_csr A, const double *vals, const float *vals32, hala_ilu ilu, const double *b,
                       double *x, double rtol, double inner_rtol, int restart, int max_refine, int *total,
                       double *hist, int max_hist, int *nhist);

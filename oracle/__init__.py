@@ -43,7 +43,7 @@ _SIG = {
     "hala_or_cg": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, f64, i32, vp, i32]),
     "hala_or_cg_team": (i32, [i32, vp, vp, vp, vp, vp, vp, f64, i32, i32, vp, i32]),
     "hala_or_gmres": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, f64, i32, i32, vp, i32, pi32]),
-    "hala_or_gmres_ir32": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, vp, i32, pi32]),
+    "hala_or_gmres_ir": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, vp, i32, pi32]),
     "hala_or_rotg": (None, [vp, vp, vp, vp]),
     "hala_or_rot": (None, [vp, vp, f64, f64]),
     "hala_or_tpsv_un": (None, [i32, vp, vp]),
@@ -225,17 +225,22 @@ def gmres(pntr, indx, vals, b, x0=None, tol=0.0, restart=30, max_outer=2**31 - 1
     return tot, x, hist[: nh.value].copy()
 
 
-def gmres_ir32(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=50):
+def gmres_ir(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=50, fp32=True):
+    """Iterative refinement around ILU(0)-GMRES; fp32 selects the FP32 product and ILU inside."""
     n = len(pntr) - 1
     x = np.zeros(n)
     hist = np.zeros(max_refine + 2)
     nh = C.c_int(0)
-    v32 = np.ascontiguousarray(vals, np.float32)
-    i32_ = np.ascontiguousarray(ilu64, np.float32)
-    tot = lib().hala_or_gmres_ir32(n, P(ci32(pntr)), P(ci32(indx)), P(c64(vals)), P(v32), P(ci32(dg)), P(i32_),
-                                   P(c64(b)), P(x), rtol, inner_rtol, restart, max_refine, P(hist), hist.size,
-                                   C.byref(nh))
+    v32 = np.ascontiguousarray(vals, np.float32) if fp32 else None
+    i32_ = np.ascontiguousarray(ilu64, np.float32) if fp32 else None
+    tot = lib().hala_or_gmres_ir(n, P(ci32(pntr)), P(ci32(indx)), P(c64(vals)), P(v32), P(ci32(dg)), P(c64(ilu64)),
+                                 P(i32_), P(c64(b)), P(x), rtol, inner_rtol, restart, max_refine, P(hist), hist.size,
+                                 C.byref(nh))
     return tot, x, hist[: nh.value].copy()
+
+
+def gmres_ir32(pntr, indx, vals, dg, ilu64, b, **kw):
+    return gmres_ir(pntr, indx, vals, dg, ilu64, b, fp32=True, **kw)
 
 
 def rotg(a, b):

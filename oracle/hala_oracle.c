@@ -614,15 +614,16 @@ int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, i
  * hist receives the outer relative residuals; returns the total number of
  * inner preconditioner applications, *nhist the number of outer residuals.
  */
-int hala_or_gmres_ir32(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
-                       const int *diag, const float *ilu32, const double *b, double *x, double rtol,
-                       double inner_rtol, int restart, int max_refine, double *hist, int max_hist, int *nhist) {
+int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
+                     const int *diag, const double *ilu, const float *ilu32, const double *b, double *x,
+                     double rtol, double inner_rtol, int restart, int max_refine, double *hist, int max_hist,
+                     int *nhist) {
     const int inner_cycles = 20;
     float *w32a = (float *)malloc((size_t)n * sizeof(float));
     float *w32b = (float *)malloc((size_t)n * sizeof(float));
     double *r = (double *)malloc((size_t)n * sizeof(double));
     double *d = (double *)malloc((size_t)n * sizeof(double));
-    gm_op_t o = {n, pntr, indx, diag, vals, vals32, 0, ilu32, HALA_OR_PREC_ILU, w32a, w32b};
+    gm_op_t o = {n, pntr, indx, diag, vals, vals32, ilu, vals32 ? ilu32 : 0, HALA_OR_PREC_ILU, w32a, w32b};
     double bn = hala_or_cnrm2(n, b);
     int total = 0, nh = 0;
     for (int it = 0; it <= max_refine; it++) {

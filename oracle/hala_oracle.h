@@ -64,9 +64,11 @@ int hala_or_cg_team(int n, const int *pntr, const int *indx, const double *vals,
 int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,
                   const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
                   double *hist, int max_hist, int *nhist);
-int hala_or_gmres_ir32(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
-                       const int *diag, const float *ilu32, const double *b, double *x, double rtol,
-                       double inner_rtol, int restart, int max_refine, double *hist, int max_hist, int *nhist);
+/* vals32 != NULL: FP32 product and FP32 ILU (ilu32) inside; NULL: FP64 product and ILU (ilu) inside. */
+int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
+                     const int *diag, const double *ilu, const float *ilu32, const double *b, double *x,
+                     double rtol, double inner_rtol, int restart, int max_refine, double *hist, int max_hist,
+                     int *nhist);
 
 /* ---- Givens / packed triangle (BLAS reference semantics) ----------------- */
 void hala_or_rotg(double *a, double *b, double *c, double *s);

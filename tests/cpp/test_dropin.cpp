@@ -1,0 +1,242 @@
+// test_dropin.cpp -- user code written against HALA's API, compiled with g++
+// against include/hala/ and libhala_amd.so (no HIP headers).  The cases are
+// the reference's own sparse/solver tests (tests/sparse_tests.hpp,
+// tests/solvers_tests.hpp, tests/cuda_sparse_tests.hpp) and the GPU example's
+// PCG table (examples/example_gpu.cpp:90-98, :363-365).
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <valarray>
+#include <vector>
+
+#include "hala/hala.hpp"
+#include "hala/hala_solvers.hpp"
+
+static int failures = 0;
+#define CHECK(cond)                                                              \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);          \
+            failures++;                                                          \
+        }                                                                        \
+    } while (0)
+
+static double diffsum(std::vector<double> const &a, std::vector<double> const &b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); i++) s += std::fabs(a[i] - b[i]);
+    return s;
+}
+
+static std::vector<double> densemul(std::vector<int> const &p, std::vector<int> const &ix, std::vector<double> const &v,
+                                    std::vector<double> const &x, bool trans = false) {
+    int n = (int)p.size() - 1;
+    std::vector<double> y(trans ? x.size() : n, 0.0);
+    for (int r = 0; r < n; r++)
+        for (int j = p[r]; j < p[r + 1]; j++) {
+            if (trans) y[ix[j]] += v[j] * x[r];
+            else y[r] += v[j] * x[ix[j]];
+        }
+    return y;
+}
+
+int main() {
+    hala::gpu_engine engine(0);
+
+    // tests/cuda_sparse_tests.hpp:10-30 and tests/sparse_tests.hpp:165-191
+    {
+        std::vector<int> pntr = {0, 2, 5, 8, 11, 13};
+        std::vector<int> indx = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4};
+        std::vector<double> vals = {2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0};
+        std::vector<double> x = {1.0, 2.0, 3.0, 4.0, 5.0}, y;
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gx = hala::make_gpu_vector(x);
+        hala::gpu_vector<double> gy;
+        hala::sparse_gemv(engine, 'N', 5, 5, 2.0, gp, gi, gv, gx, 0.0, gy);
+        gy.unload(y);
+        auto ref = densemul(pntr, indx, vals, x);
+        for (auto &r : ref) r *= 2.0;
+        CHECK(diffsum(y, ref) == 0.0);
+        hala::sparse_gemv(engine, 'T', 5, 5, 2.0, gp, gi, gv, gx, 0.0, gy);
+        gy.unload(y);
+        ref = densemul(pntr, indx, vals, x, true);
+        for (auto &r : ref) r *= 2.0;
+        CHECK(diffsum(y, ref) == 0.0);
+        auto mat = hala::make_sparse_matrix(engine, 5, gp, gi, gv);
+        mat.gemv('N', 1.0, gx, 0.0, gy);
+        CHECK(std::fabs(hala::dot(engine, gy, gx) - 190.0) < 1e-12);  // x^T A x = 4+16+36+64+70
+        std::printf("PASS sparse_gemv\n");
+    }
+    // tests/sparse_tests.hpp:259-311 (lower, non-unit; N and T)
+    {
+        std::vector<int> pntr = {0, 1, 3, 5}, indx = {0, 0, 1, 0, 2};
+        std::vector<double> vals = {1.0, 4.0, 2.0, 5.0, 3.0}, xref = {-3.0, -2.0, -1.0};
+        // A = [[1,0,0],[4,2,0],[5,0,3]];  b = 2 A xref (N), 2 A^T xref (T); alpha = 0.5
+        std::vector<double> bN = {-6.0, -32.0, -36.0}, bT = {-32.0, -8.0, -6.0}, x;
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto tri = hala::make_triangular_matrix(engine, 'L', 'N', gp, gi, gv);
+        hala::gpu_vector<double> gx;
+        hala::sparse_trsv('N', tri, 0.5, hala::make_gpu_vector(bN), gx);
+        gx.unload(x);
+        CHECK(diffsum(x, xref) < 1e-14);
+        hala::sparse_trsv('T', tri, 0.5, hala::make_gpu_vector(bT), gx);
+        gx.unload(x);
+        CHECK(diffsum(x, xref) < 1e-14);
+        std::printf("PASS sparse_trsv\n");
+    }
+    // tests/sparse_tests.hpp:423-440 (ILU known answer)
+    {
+        std::vector<int> pntr = {0, 3, 6, 9}, indx = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+        std::vector<double> vals = {3.0, 2.0, 1.0, 2.0, 4.0, 3.0, 2.0, 1.0, 6.0}, x = {1.0, 2.0, 3.0}, r;
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+        hala::gpu_vector<double> gr;
+        hala::ilu_apply(ilu, hala::make_gpu_vector(x), gr);
+        gr.unload(r);
+        CHECK(diffsum(r, {1.0 / 9.0, 1.0 / 9.0, 4.0 / 9.0}) < 1e-14);
+        std::printf("PASS ilu\n");
+    }
+    // tests/solvers_tests.hpp:12-55 (CG)
+    {
+        std::vector<int> pntr = {0, 2, 5, 8, 11, 13};
+        std::vector<int> indx = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4};
+        std::vector<double> vals = {2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0};
+        std::vector<double> xref = {1.0, 2.0, 3.0, 4.0, 5.0}, x;
+        auto b = densemul(pntr, indx, vals, xref);
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gb = hala::make_gpu_vector(b);
+        auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+        hala::gpu_vector<double> gx;
+        hala::solve_cg(engine, 2, gp, gi, gv,
+                       [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                           hala::ilu_apply(ilu, inx, outr);
+                       },
+                       gb, gx);
+        gx.unload(x);
+        CHECK(diffsum(x, xref) < 1e-14);
+        pntr = {0, 3, 6, 9, 12, 15};
+        indx = {0, 1, 4, 0, 1, 2, 1, 2, 3, 2, 3, 4, 0, 3, 4};
+        vals = {2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 2.0};
+        b = densemul(pntr, indx, vals, xref);
+        gp = hala::make_gpu_vector(pntr);
+        gi = hala::make_gpu_vector(indx);
+        gv = hala::make_gpu_vector(vals);
+        gb = hala::make_gpu_vector(b);
+        auto ilu2 = hala::make_ilu(engine, gp, gi, gv, 'N');
+        gx.load(std::vector<double>{1.0, 3.0, -4.0, -7.0, 11.0});
+        hala::solve_cg_ilu(engine, 1.E-9, gp, gi, gv, ilu2, gb, gx);
+        gx.unload(x);
+        CHECK(diffsum(x, xref) < 1e-9 * 7.5 * 10);
+        gx.load(std::vector<double>{1.0, 3.0, -4.0, -7.0, 11.0});
+        hala::solve_cg_ilu(engine, 1.E-9, gp, gi, gv, gb, gx);  // factorisation behind the scene
+        std::vector<double> x2;
+        gx.unload(x2);
+        CHECK(diffsum(x2, x) == 0.0);
+        std::printf("PASS solve_cg\n");
+    }
+    // tests/solvers_tests.hpp:105-137 (GMRES)
+    {
+        std::vector<int> pntr = {0, 2, 5, 8, 11, 12};
+        std::vector<int> indx = {0, 1, 0, 1, 3, 1, 2, 3, 0, 2, 3, 4};
+        std::vector<double> vals = {2.0, 1.0, 1.0, 3.0, 1.0, 1.0, 4.0, 1.0, 1.0, 1.0, 5.0, 2.0};
+        std::vector<double> xref = {1.0, 2.0, 3.0, 4.0, 5.0}, x, x2;
+        auto b = densemul(pntr, indx, vals, xref);
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gb = hala::make_gpu_vector(b);
+        hala::gpu_vector<double> gx, gx2;
+        hala::solve_gmres_ilu(engine, 1.E-9, 6, gp, gi, gv, gb, gx);
+        gx.unload(x);
+        CHECK(diffsum(x, xref) < 1e-8);
+        // the host-driven form with a user preconditioner computes the same iterate
+        auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+        hala::solve_gmres(engine, 1.E-9, 6, gp, gi, gv,
+                          [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                              hala::ilu_apply(ilu, inx, outr);
+                          },
+                          gb, gx2);
+        gx2.unload(x2);
+        CHECK(diffsum(x2, x) == 0.0);
+        std::printf("PASS solve_gmres\n");
+    }
+    // examples/example_gpu.cpp:90-98 on the GPU engine, and GPU == CPU-oracle agreement (:363-365)
+    {
+        int sides[6] = {4, 8, 16, 32, 64, 128};
+        int iters[6] = {9, 14, 23, 41, 78, 152};
+        for (int t = 0; t < 6; t++) {
+            int n = 0;
+            int64_t nnz = 0;
+            hala::check_hala(hala_gen_discretize_pde(sides[t], &n, &nnz, nullptr, nullptr, nullptr, nullptr), "gen");
+            std::valarray<int> pntr(n + 1), indx((size_t)nnz);
+            std::valarray<double> vals((size_t)nnz), f(n), u(0.0, n);
+            hala::check_hala(hala_gen_discretize_pde(sides[t], &n, &nnz, std::begin(pntr), std::begin(indx),
+                                                     std::begin(vals), std::begin(f)),
+                             "gen");
+            auto gp = engine.load(pntr);
+            auto gi = engine.load(indx);
+            auto gv = engine.load(vals);
+            auto gf = engine.load(f);
+            auto gu = engine.load(u);
+            auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+            int it = hala::solve_cg(engine, 1.E-8, gp, gi, gv,
+                                    [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                                        hala::ilu_apply(ilu, inx, outr);
+                                    },
+                                    gf, gu);
+            CHECK(it == iters[t]);
+            auto gu2 = engine.load(std::valarray<double>(0.0, n));
+            int it2 = hala::solve_cg_ilu(engine, 1.E-8, gp, gi, gv, ilu, gf, gu2);
+            CHECK(it2 == it);
+            hala::axpy(engine, -1.0, gu, gu2);
+            CHECK(hala::norm2(engine, gu2) == 0.0);  // fused device path == host-driven loop, bit for bit
+        }
+        std::printf("PASS example_gpu table\n");
+    }
+    // mixed precision refinement reaches the FP64 residual
+    {
+        int n = 0;
+        int64_t nnz = 0;
+        hala::check_hala(hala_gen_convdiff27(12, 3, 10.0, &n, &nnz, nullptr, nullptr, nullptr), "gen");
+        std::vector<int> pntr(n + 1), indx((size_t)nnz);
+        std::vector<double> vals((size_t)nnz), b(n);
+        hala::check_hala(hala_gen_convdiff27(12, 3, 10.0, &n, &nnz, pntr.data(), indx.data(), vals.data()), "gen");
+        hala::check_hala(hala_gen_uniform(5, n, -1.0, 1.0, b.data()), "gen");
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gb = hala::make_gpu_vector(b);
+        hala::gpu_vector<double> gx, r;
+        hala::solve_gmres_ir(engine, 1e-10, 30, gp, gi, gv, gb, gx);
+        hala::vcopy(engine, gb, r);
+        hala::sparse_gemv(engine, 'N', n, n, -1.0, gp, gi, gv, gx, 1.0, r);
+        CHECK(hala::norm2(engine, r) < 1e-10 * hala::norm2(engine, gb));
+        std::printf("PASS solve_gmres_ir\n");
+    }
+    // library errors surface as std::runtime_error
+    {
+        bool thrown = false;
+        try {
+            std::vector<int> pntr = {0, 2, 3}, indx = {1, 0, 5};
+            std::vector<double> vals = {1, 1, 1};
+            auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+            auto m = hala::make_sparse_matrix(engine, 2, gp, gi, gv);
+        } catch (std::runtime_error &) {
+            thrown = true;
+        }
+        CHECK(thrown);
+        std::printf("PASS errors\n");
+    }
+    std::printf(failures ? "FAILED %d\n" : "ALL PASSED\n", failures);
+    return failures ? 1 : 0;
+}

@@ -89,7 +89,8 @@ def test_gmres_identity_nonzero_x0(engine):
     assert np.array_equal(bits(host(gx)), bits(x))
 
 
-def test_gmres_ir_mixed_bitexact(engine):
+@pytest.mark.parametrize("fp32", [True, False], ids=["fp32_inner", "fp64_inner"])
+def test_gmres_ir_bitexact(engine, fp32):
     import torch
 
     p, i, v = st.convdiff27(16, seed=3)
@@ -97,10 +98,11 @@ def test_gmres_ir_mixed_bitexact(engine):
     b = st.uniform(44, n)
     dg, _ = oracle.diag(p, i)
     f = oracle.ilu0(p, i, v, dg)
-    tot, x, hist = oracle.gmres_ir32(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40)
+    tot, x, hist = oracle.gmres_ir(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40,
+                                   fp32=fp32)
     gp, gi, gv = upload(engine, p, i, v)
-    gv32 = gv.float()
-    ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
+    gv32 = gv.float() if fp32 else None
+    ilu = H.make_ilu(engine, gp, gi, gv, want_f32=fp32)
     A = H.make_sparse_matrix(engine, n, gp, gi, gv)
     gx = engine.vector(n)
     gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
