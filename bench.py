@@ -1,15 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: ILU(0)-preconditioned CG to a 1e-10 relative residual on the
-128^3 7-point Poisson operator (n = 2,097,152, nnz = 14,581,760), one MI355X.
+"""Benchmark of the sparse solve path on one MI355X.
 
+Headline (the `value`): ILU(0)-preconditioned CG to a 1e-10 relative residual
+on the 128^3 7-point Poisson operator (n = 2,097,152, nnz = 14,581,760), FP64.
 A step is one complete solve from a matrix and right-hand side already in HBM:
 ILU(0) analysis on the device (diagonal, level sets, level-ordered layout),
 numeric factorisation, then device-resident PCG until ||r|| < 1e-10 ||b||.
-Prints one JSON line; value = mean time to solution (seconds, lower is better).
+value = mean seconds per step (lower is better).
 
-The roofline object covers the CSR SpMV kernel (the bandwidth kernel the
-metric names) timed live with HIP events on the engine stream; the CPU
-baseline is the C restatement of HALA's CPU engine on a bounded sample.
+roofline:      the kernel that dominates a step's time -- the level-scheduled
+               triangular sweep (one launch per triangle, two per PCG pass) --
+               timed live with HIP events on the engine stream;
+roofline_spmv: the CSR SpMV (the bandwidth kernel of the path);
+cpu_baseline:  the C restatement of HALA's CPU engine, one thread, on a
+               bounded sample of the same solve;
+extra:         the other workloads -- plain CG on the 256^2 5-point Laplacian,
+               FP32-inner vs FP64-inner refinement GMRES(30) on the 96^3
+               27-point convection-diffusion operator, 4096 batched systems,
+               and the adjoint gradient on 64^3 variable-coefficient diffusion.
+
+`--kernel-probe` only launches the roofline kernels (and a copy of known size
+for counter calibration) a few times; it is the program the hardware-counter
+runs profile.
 """
 import argparse
 import json
@@ -23,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "PCG/GMRES time-to-1e-10 residual & SpMV HBM GB/s; mixed-prec vs FP64 on 1 GPU"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.txt")
 
 
 def log(*a):
@@ -30,23 +44,52 @@ def log(*a):
 
 
 def spmv_bytes(n, nnz, vbytes=8, xbytes=8):
-    # values + column indices + row pointers + x read once + y written
+    """Values + column indices + row pointers + x read once + y written."""
     return vbytes * nnz + 4 * nnz + 4 * (n + 1) + xbytes * n + xbytes * n
 
 
-def time_spmv(H, eng, A, vals, x, y, reps=50):
+def sweep_bytes(n, nnz):
+    """One triangular sweep of the ILU(0) apply (half of L then U over the combined CSR):
+    12 nnz + 8 (n+1) + 4 n (diagonal index) + 32 n (in/out of both sweeps + diagonal), per launch."""
+    return (12 * nnz + 8 * (n + 1) + 4 * n + 32 * n) / 2.0
+
+
+def event_ms(fn, reps, warm=3):
     import torch
 
-    for _ in range(5):
-        A.gemv("N", 1.0, x, 0.0, y, vals=vals)
+    for _ in range(warm):
+        fn()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     s.record()
     for _ in range(reps):
-        A.gemv("N", 1.0, x, 0.0, y, vals=vals)
+        fn()
     e.record()
     torch.cuda.synchronize()
-    return s.elapsed_time(e) / reps  # ms
+    return s.elapsed_time(e) / reps
+
+
+def wall(fn, eng):
+    eng.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    eng.synchronize()
+    return time.perf_counter() - t0, r
+
+
+def read_traffic():
+    """HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs, if recorded."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return {}
+    out = {}
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            try:
+                out[parts[0]] = float(parts[1])
+            except ValueError:
+                pass
+    return out
 
 
 def cpu_baseline(p, i, v, b, tol, iters_gpu, sample_iters):
@@ -67,6 +110,157 @@ def cpu_baseline(p, i, v, b, tol, iters_gpu, sample_iters):
                        f"restatement and the GPU path are bit-identical)")
 
 
+# ------------------------------------------------------------------ extras
+def extra_c1(H, eng, st, oracle):
+    """Plain CG, 256^2 5-point Laplacian, b ~ U[-1,1] (seed 1), x0 = 0, 1e-10 relative."""
+    p, i, v = st.laplace2d(256, 256)
+    n = len(p) - 1
+    b = st.uniform(1, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    gp, gi, gv, gb = eng.load(p), eng.load(i), eng.load(v), eng.load(b)
+    A = H.make_sparse_matrix(eng, n, gp, gi, gv)
+    gx = eng.vector(n)
+
+    def run():
+        gx.zero_()
+        return H.solve_cg(eng, tol, gp, gi, gv, None, gb, gx, matrix=A)
+
+    run()
+    ts = []
+    for _ in range(3):
+        t, it = wall(run, eng)
+        ts.append(t)
+    x = gx.cpu().numpy()
+    rel = float(np.linalg.norm(b - oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))) / np.linalg.norm(b))
+    t = float(np.median(ts))
+    return {"workload": "cg_poisson2d_5pt_256^2", "n": n, "nnz": len(i), "iterations": it, "tts_ms": round(t * 1e3, 3),
+            "iters_per_s": round((it - 1) / t, 1), "final_relres": rel}
+
+
+def extra_c3(H, eng, st, oracle):
+    """GMRES(30) refinement on 96^3 27-point convection-diffusion: FP32 inner vs FP64 inner."""
+    import torch
+
+    p, i, v = st.convdiff27(96, seed=3)
+    n = len(p) - 1
+    b = st.uniform(33, n)
+    gp, gi, gv, gb = eng.load(p), eng.load(i), eng.load(v), eng.load(b)
+    gv32 = gv.float()
+    A = H.make_sparse_matrix(eng, n, gp, gi, gv)
+    t_setup, ilu = wall(lambda: H.make_ilu(eng, gp, gi, gv, want_f32=True), eng)
+    gh = torch.zeros(64, dtype=torch.float64, device=gb.device)
+    out = {"workload": "gmres30_ir_convdiff3d_27pt_96^3", "n": n, "nnz": len(i), "rtol": 1e-10, "inner_rtol": 1e-4,
+           "ilu_setup_ms": round(t_setup * 1e3, 2)}
+    xs = {}
+    for name, v32 in (("fp32_inner", gv32), ("fp64_inner", None)):
+        gx = eng.vector(n)
+
+        def run():
+            gx.zero_()
+            return H.solve_gmres_ir(eng, A, gv, v32, ilu, gb, gx, rtol=1e-10, inner_rtol=1e-4, restart=30,
+                                    max_refine=40, hist=gh)
+
+        run()
+        ts = []
+        for _ in range(2):
+            t, (tot, nh) = wall(run, eng)
+            ts.append(t)
+        x = gx.cpu().numpy()
+        xs[name] = x
+        rel = float(np.linalg.norm(b - oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))) / np.linalg.norm(b))
+        out[name] = {"tts_ms": round(1e3 * float(np.median(ts)), 2), "inner_prec_applications": tot,
+                     "refinement_steps": nh - 1, "final_relres": rel}
+    out["speedup_fp32_inner_vs_fp64_inner"] = round(out["fp64_inner"]["tts_ms"] / out["fp32_inner"]["tts_ms"], 3)
+    out["rel_error_fp32_inner_vs_fp64_solution"] = float(
+        np.linalg.norm(xs["fp32_inner"] - xs["fp64_inner"]) / np.linalg.norm(xs["fp64_inner"]))
+    # bandwidth of the inner FP32 product vs the FP64 one on this operator
+    gy = eng.vector(n)
+    ms64 = event_ms(lambda: A.gemv("N", 1.0, gb, 0.0, gy, vals=gv), 30)
+    ms32 = event_ms(lambda: A.gemv("N", 1.0, gb, 0.0, gy, vals=gv32), 30)
+    out["spmv_f64_gbs"] = round(spmv_bytes(n, len(i)) / (ms64 * 1e-3) / 1e9, 1)
+    out["spmv_f32_inner_gbs"] = round(spmv_bytes(n, len(i), vbytes=4) / (ms32 * 1e-3) / 1e9, 1)
+    ms_a64 = event_ms(lambda: ilu.apply(gb, gy), 10)
+    ms_a32 = event_ms(lambda: ilu.apply_f32x64(gb, gy), 10)
+    out["ilu_apply_f64_ms"] = round(ms_a64, 4)
+    out["ilu_apply_f32_ms"] = round(ms_a32, 4)
+    return out
+
+
+def extra_c4(H, eng, st):
+    """4096 independent 2D Laplacians + shift, Jacobi-PCG to 1e-10, one persistent graph-captured launch."""
+    import torch
+
+    d = st.batch_laplace(4096, seed=4, side_lo=14, side_hi=70)
+    N = len(d["dvals"])
+    b = st.uniform(104, N)
+    g = {k: eng.load(d[k]) for k in ("row_off", "pntr", "indx", "vals", "dvals")}
+    gb = eng.load(b)
+    bs = H.batch_solver(eng, g["row_off"], g["pntr"], g["indx"])
+    gx = eng.vector(N)
+    t_tune, (team, wg, _) = wall(lambda: bs.tune(g["vals"], g["dvals"], gb, gx, rtol=1e-10, reps=1), eng)
+    iters = torch.zeros(4096, dtype=torch.int32, device=gb.device)
+
+    def run():
+        gx.zero_()  # x0 = 0 every time (the solver starts from x)
+        bs.solve(g["vals"], g["dvals"], gb, gx, rtol=1e-10, iters=iters)
+
+    ms = event_ms(run, 5, warm=2)
+    it = iters.cpu().numpy()
+    nnz = len(d["indx"])
+    return {"workload": "batched_jacobi_pcg_4096x_laplace2d_14..70", "systems": 4096, "rows": N, "nnz": nnz,
+            "team": team, "wg_per_cu": wg, "tune_s": round(t_tune, 2), "solve_ms": round(ms, 3),
+            "iterations_total": int(it.sum()), "iterations_max": int(it.max()),
+            "system_iters_per_s": round(float(it.sum()) / (ms * 1e-3), 1)}
+
+
+def extra_c5(H, eng, st):
+    """Adjoint gradient of 0.5||x - x_t||^2 w.r.t. face conductivities, 64^3 diffusion."""
+    import torch
+
+    from hala_amd.autograd import SparseLinearProblem
+
+    m = 64
+    d = st.diffusion3d(m, seed=5)
+    dt = st.diffusion3d(m, seed=6)
+    f = st.gaussian_source(m)
+    prob = SparseLinearProblem(eng, d["pntr"], d["indx"], d["cptr"], d["cnnz"], d["ccoef"], f, rtol=1e-10)
+    xt, _ = prob.forward_solve(eng.load(dt["kappa"]))
+    xt = xt.clone()
+    k0 = eng.load(d["kappa"])
+
+    def fwd_bwd():
+        x, _ = prob.forward_solve(k0)
+        return prob.gradient(k0, x - xt)
+
+    fwd_bwd()
+    t_fb, g = wall(fwd_bwd, eng)
+    t_f, _ = wall(lambda: prob.forward_solve(k0), eng)
+    return {"workload": "adjoint_grad_diffusion3d_7pt_64^3", "n": len(d["pntr"]) - 1, "nnz": len(d["indx"]),
+            "kappa_faces": len(d["kappa"]), "forward_ms": round(t_f * 1e3, 2),
+            "forward_plus_gradient_ms": round(t_fb * 1e3, 2), "grad_norm": float(torch.linalg.norm(g))}
+
+
+# ------------------------------------------------------------------ main
+def kernel_probe(H, eng, st, m):
+    """A few launches of each roofline kernel plus a copy of known size (counter calibration)."""
+    import torch
+
+    p, i, v = st.laplace3d(m, m, m)
+    n = len(p) - 1
+    gp, gi, gv = eng.load(p), eng.load(i), eng.load(v)
+    A = H.make_sparse_matrix(eng, n, gp, gi, gv)
+    ilu = H.make_ilu(eng, gp, gi, gv)
+    gb, gy = eng.load(st.uniform(2, n)), eng.vector(n)
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=gb.device)  # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        A.gemv("N", 1.0, gb, 0.0, gy)
+        ilu.apply(gb, gy)
+        H.vcopy(eng, src, dst)  # copy_kernel: reads and writes exactly 256 MiB
+    eng.synchronize()
+    print(json.dumps({"probe": "done", "n": n, "nnz": len(i), "copy_bytes": 2 * src.numel() * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +269,8 @@ def main():
     ap.add_argument("--side", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--kernel-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the sparse solve path is measured on a single "
@@ -87,6 +283,11 @@ def main():
     import oracle
     from hala_amd import stencils as st
 
+    eng = H.gpu_engine(0)
+    if args.kernel_probe:
+        kernel_probe(H, eng, st, args.side)
+        return
+
     m = args.side
     log(f"[bench] generating {m}^3 7-point Poisson")
     p, i, v = st.laplace3d(m, m, m)
@@ -94,7 +295,6 @@ def main():
     b = st.uniform(2, n)
     tol = 1e-10 * oracle.cnrm2(b)
 
-    eng = H.gpu_engine(0)
     gp, gi, gv, gb = eng.load(p), eng.load(i), eng.load(v), eng.load(b)
     A = H.make_sparse_matrix(eng, n, gp, gi, gv)
     gx = eng.vector(n)
@@ -108,16 +308,16 @@ def main():
         it = H.solve_cg_ilu(eng, tol, gp, gi, gv, ilu, gb, gx, matrix=A)
         eng.synchronize()
         t2 = time.perf_counter()
-        return it, t1 - t0, t2 - t1
+        return it, t1 - t0, t2 - t1, ilu
 
     for w in range(args.warmup):
-        it, su, so = step()
+        it, su, so, _ = step()
         log(f"[bench] warmup {w}: {it} iterations, setup {su*1e3:.1f} ms, solve {so*1e3:.1f} ms")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     setups, solves = [], []
     for k in range(args.steps):
-        it, su, so = step()
+        it, su, so, ilu = step()
         setups.append(su)
         solves.append(so)
         log(f"[bench] step {k}: {it} iterations, setup {su*1e3:.1f} ms, solve {so*1e3:.1f} ms")
@@ -129,17 +329,23 @@ def main():
     r = b - oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))
     relres = float(np.linalg.norm(r) / np.linalg.norm(b))
 
-    # SpMV roofline (FP64 and FP32-value variants), live events on the engine stream
+    # dominant kernel: the triangular sweep (two launches per ILU apply), live events on the engine stream
     gy = eng.vector(n)
-    ms64 = time_spmv(H, eng, A, gv, gb, gy)
+    ms_apply = event_ms(lambda: ilu.apply(gb, gy), 20)
+    ms_sweep = ms_apply / 2.0
+    by_sweep = sweep_bytes(n, nnz)
+    ach_sweep = by_sweep / (ms_sweep * 1e-3) / 1e9
+    # the CSR SpMV (FP64, and FP32 values)
+    ms64 = event_ms(lambda: A.gemv("N", 1.0, gb, 0.0, gy, vals=gv), 50)
     by64 = spmv_bytes(n, nnz)
     gv32 = gv.float()
-    ms32 = time_spmv(H, eng, A, gv32, gb, gy)
+    ms32 = event_ms(lambda: A.gemv("N", 1.0, gb, 0.0, gy, vals=gv32), 50)
     by32 = spmv_bytes(n, nnz, vbytes=4)
-    ach = by64 / (ms64 * 1e-3) / 1e9
+    ach64 = by64 / (ms64 * 1e-3) / 1e9
+    traffic = read_traffic()
 
     out = {
-        "metric": "ILU(0)-PCG time-to-1e-10 relative residual (128^3 7-point Poisson), FP64",
+        "metric": METRIC,
         "value": round(tts, 6),
         "unit": "s",
         "n_gpus": 1,
@@ -152,19 +358,32 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded splitmix64 rhs U[-1,1]; stencil matrix)",
         "config": {"workload": f"ilu0_pcg_poisson3d_7pt_{m}^3", "n": n, "nnz": nnz, "rtol": 1e-10,
-                   "x0": "zero", "parallelism": "single_gpu"},
+                   "x0": "zero", "step": "ilu0 analysis + factorisation + pcg solve", "parallelism": "single_gpu"},
         "iterations": it,
         "setup_ms": round(1e3 * float(np.mean(setups)), 3),
         "solve_ms": round(1e3 * float(np.mean(solves)), 3),
         "iters_per_s": round((it - 1) / float(np.mean(solves)), 2),
         "final_relres": relres,
-        "roofline": {"kernel": "spmv_rows_kernel<f64> (CSR, LDS-staged 256-row blocks)", "bound": "hbm",
-                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": by64, "avg_launch_ms": round(ms64, 5)},
+        "roofline": {"kernel": "tri_exec_kernel<f64> (sync-free level-scheduled ILU(0) sweep, one triangle)",
+                     "bound": "hbm", "achieved": round(ach_sweep, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_exec_bytes"),
+                     "bytes_per_launch": by_sweep, "avg_launch_ms": round(ms_sweep, 5),
+                     "note": "latency-bound: 382 dependency levels per triangle"},
+        "roofline_spmv": {"kernel": "spmv_rows_kernel<f64> (CSR, LDS-staged 256-row blocks)", "bound": "hbm",
+                          "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic.get("spmv_bytes"),
+                          "bytes_per_launch": by64, "avg_launch_ms": round(ms64, 5)},
         "spmv_f32_gbs": round(by32 / (ms32 * 1e-3) / 1e9, 1),
         "spmv_f32_avg_ms": round(ms32, 5),
     }
+    if not args.no_extra:
+        extra = {}
+        for name, fn in (("c1", lambda: extra_c1(H, eng, st, oracle)), ("c3", lambda: extra_c3(H, eng, st, oracle)),
+                         ("c4", lambda: extra_c4(H, eng, st)), ("c5", lambda: extra_c5(H, eng, st))):
+            log(f"[bench] extra {name}")
+            extra[name] = fn()
+            log(f"[bench] extra {name}: {json.dumps(extra[name])}")
+        out["extra"] = extra
     if not args.no_cpu:
         log(f"[bench] CPU baseline ({args.cpu_iters} PCG passes on the host)")
         cb = cpu_baseline(p, i, v, b, tol, it, args.cpu_iters)

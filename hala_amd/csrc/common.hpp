@@ -103,6 +103,7 @@ struct hala_options {
     int tri_sleep = 1;         // s_sleep slots between polls of an unready dependency
     int cg_batch_max = 32;     // solver passes launched between stop-flag reads
     int spmv_blocks_per_cu = 8;  // workgroups per CU of the grid-stride fused SpMV + dot
+    int tri_streams = 1;       // ticket counters of the triangular-solve pool (1..TK_MAX)
 };
 
 struct hala_ctx_s {
@@ -291,6 +292,14 @@ inline unsigned *counters(hala_ctx c) { return reinterpret_cast<unsigned *>(c->s
 // Arrival counters of reduction site `site` (LB_GROUPS + 1 zeroed words each).
 constexpr int LB_SITES = 160;
 inline unsigned *lb_counter(hala_ctx c, int site) { return c->lbc.as<unsigned>() + site * (LB_GROUPS + 1); }
+// Ticket counters of the triangular-solve pool: TK_MAX counters plus one
+// "waves finished" counter, each on its own 4 KiB page so that their atomics
+// land in different memory channels instead of queueing on one address.
+constexpr int TK_STRIDE = 1024;  // in unsigned words
+constexpr int TK_MAX = 4;
+constexpr int TK_BASE = (LB_SITES * (LB_GROUPS + 1) + TK_STRIDE - 1) / TK_STRIDE * TK_STRIDE;
+constexpr int LBC_WORDS = TK_BASE + (TK_MAX + 1) * TK_STRIDE;
+inline unsigned *tri_tickets(hala_ctx c) { return c->lbc.as<unsigned>() + TK_BASE; }
 enum lb_site : int { LB_DOT = 0, LB_CG_START = 1, LB_CG_SPMV = 2, LB_CG_UPDATE = 3, LB_CG_RZ = 4, LB_GM_NRM = 5,
                      LB_GM_ORTH = 6, LB_IR_NRM = 7, LB_GEMV = 16 /* .. 16 + 32 columns */,
                      LB_GM_COL = 64 /* .. 64 + 64 Krylov columns */ };

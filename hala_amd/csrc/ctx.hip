@@ -104,7 +104,7 @@ int hala_ctx_create(int device, void *stream, hala_ctx *out) {
         try {
             c->scal.reserve(S_TOTAL * sizeof(double));
             HALA_HIP(hipMemsetAsync(c->scal.p, 0, S_TOTAL * sizeof(double), c->stream));
-            c->lbc.reserve((size_t)LB_SITES * (LB_GROUPS + 1) * sizeof(unsigned));
+            c->lbc.reserve((size_t)LBC_WORDS * sizeof(unsigned));
             HALA_HIP(hipMemsetAsync(c->lbc.p, 0, c->lbc.bytes, c->stream));
             partials64(c, 64 * 1024);
             partials32(c, 64 * 1024);
@@ -158,6 +158,7 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "tri_waves_per_cu") return &c->opt.tri_waves_per_cu;
     if (k == "tri_pool_waves") return &c->opt.tri_pool_waves;
     if (k == "tri_sleep") return &c->opt.tri_sleep;
+    if (k == "tri_streams") return &c->opt.tri_streams;
     if (k == "cg_batch_max") return &c->opt.cg_batch_max;
     if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
     return nullptr;
@@ -169,6 +170,8 @@ int hala_ctx_set_option(hala_ctx c, const char *name, int value) {
         int *slot = option_slot(c, name);
         HALA_REQUIRE(slot, HALA_ERR_ARG, std::string("unknown option ") + (name ? name : "(null)"));
         HALA_REQUIRE(value >= 0 && value <= (1 << 20), HALA_ERR_ARG, "option value out of range");
+        HALA_REQUIRE(slot != &c->opt.tri_streams || (value >= 1 && value <= TK_MAX), HALA_ERR_ARG,
+                     "tri_streams must lie in 1.." + std::to_string(TK_MAX));
         *slot = value;
     });
 }

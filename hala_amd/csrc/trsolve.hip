@@ -6,9 +6,6 @@
 
 namespace hala_amd {
 
-constexpr int WAVE = 64;
-constexpr int MAXD = 16;  // dependencies held in registers per row (longer rows loop)
-constexpr unsigned SPIN_LIMIT = 1u << 22;  // polls before a wait gives up (a few seconds)
 
 int *wait_error_flag(hala_ctx c) { return reinterpret_cast<int *>(counters(c) + 120); }
 
@@ -341,169 +338,5 @@ void load_values32(hala_ctx c, tri_plan &p, const double *factor) {
     }
     p.has32 = true;
 }
-
-// ---------------------------------------------------------------------------
-// execution
-// ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ T wait_value(const T *p, int *err, int sleep) {
-    T v = ld_agent(p);
-    if (!is_sent(v)) return v;
-    // exponential backoff between polls, capped at `sleep` slots of 64 cycles:
-    // a lane far from ready polls rarely, which keeps the uncached traffic
-    // (and so the latency seen by lanes about to become ready) low
-    unsigned spins = 0;
-    int nap = 1;
-    for (;;) {
-        for (int i = 0; i < nap; i++) __builtin_amdgcn_s_sleep(1);
-        if (nap < sleep) nap = nap * 2 < sleep ? nap * 2 : sleep;
-        v = ld_agent(p);
-        if (!is_sent(v)) return v;
-        if (++spins > SPIN_LIMIT) {
-            atomicOr(err, 1);
-            return T(0);
-        }
-    }
-}
-
-// Everything one triangular-solve launch needs (passed by value).
-template <class T, class TIN, class TOUT>
-struct tri_args {
-    int nchunk, nwaves, nlev, sleep;
-    const int *chunk_row, *chunk_off, *chunk_len, *ent_dep;
-    const T *ent_val, *dg_val;
-    T alpha;
-    const TIN *in;
-    T *out;
-    TOUT *out_nat;
-    T *reset;
-    const int *reset_slot;
-    unsigned *ticket;
-    int *err;
-    const int *gate;
-};
-
-// Persistent pool: each wavefront takes chunk tickets in order until none are
-// left, so a chunk is only ever held by a running wave and the lowest
-// unfinished chunk always has every dependency done (no deadlock for any pool
-// size, also with the one-ahead ticket prefetch).  Per chunk a wave first
-// loads everything that does not depend on other rows, then issues the loads
-// of all its dependencies at once and only spins on those still holding the
-// sentinel.  The last wave to leave re-arms the ticket counters.
-template <class T, class TIN, class TOUT, int MODE, bool UNIT>
-__global__ __launch_bounds__(256) void tri_exec_kernel(tri_args<T, TIN, TOUT> a) {
-    if (a.gate && *a.gate) return;
-    const int lane = threadIdx.x & (WAVE - 1);
-    int w = 0;
-    if (lane == 0) w = (int)atomicAdd(a.ticket, 1u);
-    w = __shfl(w, 0, WAVE);
-    while (w < a.nchunk) {
-        int wn = 0;
-        if (lane == 0) wn = (int)atomicAdd(a.ticket, 1u);  // next ticket, latency overlapped
-        const int slot = w * WAVE + lane;
-        const int row = a.chunk_row[slot];
-        const int len = a.chunk_len[w];
-        const int64_t base = (int64_t)a.chunk_off[w] * WAVE + lane;
-        T acc = T(0), s = T(0);
-        if (row >= 0 && MODE == TRI_SUB) acc = (T)a.in[row];
-        if (len <= MAXD) {
-            int d[MAXD];
-            T v[MAXD], xv[MAXD];
-#pragma unroll
-            for (int k = 0; k < MAXD; k++) {
-                d[k] = -1;
-                if (row >= 0 && k < len) {
-                    d[k] = a.ent_dep[base + (int64_t)k * WAVE];
-                    v[k] = a.ent_val[base + (int64_t)k * WAVE];
-                }
-            }
-            bool ready = true;
-#pragma unroll
-            for (int k = 0; k < MAXD; k++)
-                if (d[k] >= 0) {
-                    xv[k] = ld_agent(a.out + d[k]);
-                    ready = ready && !is_sent(xv[k]);
-                }
-            if (!__all(ready)) {
-#pragma unroll
-                for (int k = 0; k < MAXD; k++)
-                    if (d[k] >= 0 && is_sent(xv[k])) xv[k] = wait_value(a.out + d[k], a.err, a.sleep);
-            }
-#pragma unroll
-            for (int k = 0; k < MAXD; k++)
-                if (d[k] >= 0) {
-                    if (MODE == TRI_SUB) acc = acc - v[k] * xv[k];
-                    else s = s + v[k] * xv[k];
-                }
-        } else if (row >= 0) {
-            for (int k = 0; k < len; k++) {
-                int dk = a.ent_dep[base + (int64_t)k * WAVE];
-                if (dk < 0) break;
-                T vk = a.ent_val[base + (int64_t)k * WAVE];
-                T xk = wait_value(a.out + dk, a.err, a.sleep);
-                if (MODE == TRI_SUB) acc = acc - vk * xk;
-                else s = s + vk * xk;
-            }
-        }
-        if (row >= 0) {
-            if (MODE == TRI_SUM) acc = a.alpha * (T)a.in[row] - s;
-            if (!UNIT) acc = acc / a.dg_val[slot];
-            st_agent(a.out + slot, acc);
-            if (a.out_nat) a.out_nat[row] = (TOUT)acc;
-            if (a.reset) a.reset[a.reset_slot[row]] = sentinel<T>();
-        }
-        w = __shfl(wn, 0, WAVE);
-    }
-    if (lane == 0) {
-        unsigned *left = a.ticket + 1;
-        unsigned dd = atomicAdd(left, 1u);
-        if (dd == (unsigned)a.nwaves - 1) {
-            st_agent(a.ticket, 0u);
-            st_agent(left, 0u);
-        }
-    }
-}
-
-template <class T, class TIN, class TOUT>
-void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
-              const int *reset_slot, const int *gate) {
-    constexpr bool is64 = sizeof(T) == 8;
-    HALA_REQUIRE(is64 ? p.has64 : p.has32, HALA_ERR_ARG, "triangular solve: values not loaded for this precision");
-    tri_args<T, TIN, TOUT> a;
-    a.out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
-    a.ent_val = is64 ? reinterpret_cast<const T *>(p.ent_val64.p) : reinterpret_cast<const T *>(p.ent_val32.p);
-    a.dg_val = is64 ? reinterpret_cast<const T *>(p.dg_val64.p) : reinterpret_cast<const T *>(p.dg_val32.p);
-    int pool = c->opt.tri_pool_waves > 0 ? c->opt.tri_pool_waves : c->num_cus * c->opt.tri_waves_per_cu;
-    int blocks = (std::min(pool, (p.nchunk + 3) / 4 * 4) + 3) / 4;
-    if (blocks < 1) blocks = 1;
-    a.nchunk = p.nchunk;
-    a.nwaves = blocks * 4;
-    a.nlev = p.nlev;
-    a.sleep = c->opt.tri_sleep;
-    a.chunk_row = p.chunk_row.as<int>();
-    a.chunk_off = p.chunk_off.as<int>();
-    a.chunk_len = p.chunk_len.as<int>();
-    a.ent_dep = p.ent_dep.as<int>();
-    a.alpha = alpha;
-    a.in = in;
-    a.out_nat = out_nat;
-    a.reset = reset;
-    a.reset_slot = reset_slot;
-    a.ticket = counters(c) + 102;  // [102] ticket, [103] waves finished
-    a.err = wait_error_flag(c);
-    a.gate = gate;
-    if (mode == TRI_SUB) {
-        if (p.unit) tri_exec_kernel<T, TIN, TOUT, TRI_SUB, true><<<blocks, 256, 0, c->stream>>>(a);
-        else tri_exec_kernel<T, TIN, TOUT, TRI_SUB, false><<<blocks, 256, 0, c->stream>>>(a);
-    } else {
-        if (p.unit) tri_exec_kernel<T, TIN, TOUT, TRI_SUM, true><<<blocks, 256, 0, c->stream>>>(a);
-        else tri_exec_kernel<T, TIN, TOUT, TRI_SUM, false><<<blocks, 256, 0, c->stream>>>(a);
-    }
-    HALA_CHECK_LAUNCH();
-}
-
-template void tri_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, double *, const int *, const int *);
-template void tri_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, float *, const int *, const int *);
-template void tri_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, float *, const int *, const int *);
 
 }  // namespace hala_amd

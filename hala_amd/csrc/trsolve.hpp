@@ -87,6 +87,10 @@ void tri_rearm(hala_ctx c, tri_plan &p);
 
 // Device error flag raised by a dependency wait that gave up; read and cleared by check_wait_error.
 int *wait_error_flag(hala_ctx c);
+
+constexpr int WAVE = 64;
+constexpr int MAXD = 16;                   // dependencies held in registers per row (longer rows loop)
+constexpr unsigned SPIN_LIMIT = 1u << 22;  // polls before a wait gives up (a few seconds)
 void check_wait_error(hala_ctx c);
 
 }  // namespace hala_amd

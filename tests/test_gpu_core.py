@@ -209,6 +209,39 @@ def test_ilu_symbolic_numeric_apply(engine, name):
         assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply_t(p, i, dg, f, x)))
 
 
+TRI_MODES = [  # (tri_streams, tri_pool_waves)
+    (1, 0), (2, 0), (4, 0), (4, 8), (3, 12), (1, 100000)]
+
+
+@pytest.mark.parametrize("mode", TRI_MODES, ids=lambda m: f"streams{m[0]}_pool{m[1]}")
+def test_tri_launch_modes_bitexact(engine, mode):
+    """Every way of running the triangular-solve pool gives the same bits."""
+    keys = ("tri_streams", "tri_pool_waves")
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        for k, val in zip(keys, mode):
+            engine.set_option(k, val)
+        for name in ("lap3d_24", "cd27_16"):
+            p, i, v = ILU_MATS[name]()
+            n = len(p) - 1
+            M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
+            dg, _ = oracle.diag(p, i)
+            f = oracle.ilu0(p, i, v, dg)
+            x = st.uniform(22, n)
+            gr = engine.vector(n)
+            for _ in range(3):
+                M.apply(dev(engine, x), gr)
+                assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply(p, i, dg, f, x))), name
+            M.apply_f32x64(dev(engine, x), gr)
+            ref = oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x.astype(np.float32)).astype(np.float64)
+            assert np.array_equal(bits(host(gr)), bits(ref)), name
+            M.apply_t(dev(engine, x), gr)
+            assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply_t(p, i, dg, f, x))), name
+    finally:
+        for k, val in zip(keys, saved):
+            engine.set_option(k, val)
+
+
 def test_ilu_reference_known_answer(engine):
     # tests/sparse_tests.hpp:423-440
     p = np.array([0, 3, 6, 9], np.int32)
