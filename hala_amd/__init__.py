@@ -174,6 +174,12 @@ class gpu_ilu:
         call("hala_ilu_info", self.handle, C.byref(inf))
         return {k: getattr(inf, k) for k, _ in IluInfo._fields_}
 
+    def sweep_info(self, upper=False):
+        """Layout of one triangular sweep: column-blocked or level pool, and its shape."""
+        o = (C.c_int * 6)()
+        call("hala_ilu_sweep_info", self.handle, 1 if upper else 0, o)
+        return {"column": bool(o[0]), "ncol": o[1], "cells": o[2], "window": o[3], "max_ext": o[4], "why": o[5]}
+
     def apply(self, x, r, num_rhs=1):
         assert num_rhs == 1
         if _dt(x) == 64:

@@ -77,6 +77,7 @@ SIGNATURES = {
     "hala_ilu_create": [vp, i32, i64, vp, vp, pvp],
     "hala_ilu_destroy": [vp],
     "hala_ilu_info": [vp, C.POINTER(IluInfo)],
+    "hala_ilu_sweep_info": [vp, i32, pi32],
     "hala_ilu_factor_f64": [vp, vp, i32],
     "hala_ilu_apply_f64": [vp, vp, vp],
     "hala_ilu_apply_f32": [vp, vp, vp],

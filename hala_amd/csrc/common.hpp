@@ -68,6 +68,24 @@ struct dbuf {
     dbuf() = default;
     dbuf(const dbuf &) = delete;
     dbuf &operator=(const dbuf &) = delete;
+    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    dbuf &operator=(dbuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    void swap(dbuf &o) noexcept {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
     ~dbuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -104,6 +122,8 @@ struct hala_options {
     int cg_batch_max = 32;     // solver passes launched between stop-flag reads
     int spmv_blocks_per_cu = 8;  // workgroups per CU of the grid-stride fused SpMV + dot
     int tri_streams = 1;       // ticket counters of the triangular-solve pool (1..TK_MAX)
+    int tri_col = 1;           // column-blocked triangular sweeps where the pattern allows (0: level pool only)
+    int tri_col_cols = 0;      // columns of the blocked sweep (0: smallest that fits)
 };
 
 struct hala_ctx_s {

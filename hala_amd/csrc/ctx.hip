@@ -159,6 +159,8 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "tri_pool_waves") return &c->opt.tri_pool_waves;
     if (k == "tri_sleep") return &c->opt.tri_sleep;
     if (k == "tri_streams") return &c->opt.tri_streams;
+    if (k == "tri_col") return &c->opt.tri_col;
+    if (k == "tri_col_cols") return &c->opt.tri_col_cols;
     if (k == "cg_batch_max") return &c->opt.cg_batch_max;
     if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
     return nullptr;

@@ -169,6 +169,19 @@ int hala_ilu_info(hala_ilu M, hala_ilu_info_t *info) {
     });
 }
 
+int hala_ilu_sweep_info(hala_ilu M, int upper, int *out) {
+    return guard([&] {
+        HALA_REQUIRE(M && out, HALA_ERR_ARG, "hala_ilu_sweep_info: null argument");
+        const tri_colplan &q = upper ? M->U.col : M->L.col;
+        out[0] = q.ok ? 1 : 0;
+        out[1] = q.ncol;
+        out[2] = q.CP;
+        out[3] = q.K;
+        out[4] = q.maxext;
+        out[5] = q.why;
+    });
+}
+
 int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
     return guard([&] {
         HALA_REQUIRE(M && vals, HALA_ERR_ARG, "hala_ilu_factor_f64: null argument");

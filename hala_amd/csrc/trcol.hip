@@ -1,0 +1,725 @@
+// trcol.hip -- column-blocked execution of the level-scheduled triangular sweeps.
+//
+// The rows are cut into `ncol` contiguous blocks ("columns"): every dependency
+// of a forward sweep points to a lower row and of a backward sweep to a higher
+// row, so it lies in the same column or in one handled before it.  One
+// workgroup owns a column and walks the levels in order; the rows of its
+// column at one level (a "cell", a contiguous piece of that level's row list)
+// get one thread each.  A dependency produced by the same workgroup within the
+// last K levels is read from an LDS window; every other dependency ("external")
+// is polled in the level-ordered output with agent-scope loads issued several
+// levels ahead and staged into LDS just before its level.  A level ends with an
+// LDS-only wait and a workgroup barrier, so the critical path pays a global
+// round trip only where it crosses to another column, instead of at every
+// level.  Per row the terms are applied in the row's own order, so results
+// are bit-identical to the level-pool sweep and to the sequential reference.
+#include <algorithm>
+#include <climits>
+
+#include "trsolve.hpp"
+
+namespace hala_amd {
+
+namespace {
+
+constexpr int CI = 5;       // ints per (column, level) cell record
+constexpr int KMAX = 8;     // levels held in the LDS window at most
+constexpr int BTMAX = 1024;  // padding of slot-indexed arrays (idle-lane reads)
+constexpr int CPMAX = 256;   // rows of a cell at most (= threads of a column workgroup)
+constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
+constexpr int64_t LDS_BUDGET = 64 * 1024;
+
+__device__ __forceinline__ int colof(int r, int n, int ncol, int backward) {
+    const int64_t k = backward ? (int64_t)(n - 1 - r) : (int64_t)r;
+    return (int)(k * ncol / n);
+}
+
+// cell populations and the largest level gap of a same-column dependency
+__global__ void col_count(int n, int ncol, int backward, const int *__restrict__ lev, const int *__restrict__ dptr,
+                          const int *__restrict__ ddep, int *__restrict__ cellcnt, int *__restrict__ maxgap) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int c = colof(r, n, ncol, backward);
+    atomicAdd(cellcnt + (int64_t)lev[r] * ncol + c, 1);
+    int g = 0;
+    for (int e = dptr[r]; e < dptr[r + 1]; e++) {
+        int d = ddep[e];
+        if (colof(d, n, ncol, backward) == c) g = max(g, lev[r] - lev[d]);
+    }
+    if (g > 0) atomicMax(maxgap, g);
+}
+
+// per level: start position of every cell inside the level's row list
+// (rows ascend inside a level: columns ascend for a forward sweep and
+// descend for a backward one)
+__global__ void col_starts(int nlev, int ncol, int backward, const int *__restrict__ cellcnt,
+                           int *__restrict__ cstart) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlev) return;
+    int run = 0;
+    for (int i = 0; i < ncol; i++) {
+        int c = backward ? ncol - 1 - i : i;
+        cstart[(int64_t)l * ncol + c] = run;
+        run += cellcnt[(int64_t)l * ncol + c];
+    }
+}
+
+__device__ __forceinline__ bool is_local(int r, int d, int c, int cd, const int *lev, int K) {
+    return cd == c && lev[r] - lev[d] < K;
+}
+
+// external dependencies per slot
+__global__ void col_extcnt(int nslots, int n, int ncol, int backward, int K, const int *__restrict__ chunk_row,
+                           const int *__restrict__ lev, const int *__restrict__ dptr, const int *__restrict__ ddep,
+                           int *__restrict__ ecnt) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int r = chunk_row[s], k = 0;
+    if (r >= 0) {
+        const int c = colof(r, n, ncol, backward);
+        for (int e = dptr[r]; e < dptr[r + 1]; e++) {
+            int d = ddep[e];
+            if (!is_local(r, d, c, colof(d, n, ncol, backward), lev, K)) k++;
+        }
+    }
+    ecnt[s] = k;
+}
+
+// cell records (column-major) and the level range of every column
+__global__ void col_cells(int nlev, int ncol, const int *__restrict__ lchunk, const int *__restrict__ cellcnt,
+                          const int *__restrict__ cstart, const int *__restrict__ epos, int *__restrict__ ci,
+                          int *__restrict__ colrange, int *__restrict__ maxext) {
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nlev * ncol) return;
+    const int l = (int)(id / ncol), c = (int)(id % ncol);
+    const int S0 = lchunk[l] * WAVE, Wl = (lchunk[l + 1] - lchunk[l]) * WAVE;
+    const int q0 = cstart[id], size = cellcnt[id];
+    const int slot0 = S0 + q0;
+    const int e0 = epos[slot0], en = epos[slot0 + size] - e0;
+    int *o = ci + ((int64_t)c * nlev + l) * CI;
+    o[0] = slot0;
+    o[1] = q0;
+    o[2] = size | (en << 16);
+    o[3] = e0;
+    o[4] = Wl;
+    if (size > 0) {
+        atomicMin(colrange + 2 * c, l);
+        atomicMax(colrange + 2 * c + 1, l);
+    }
+    if (en > 0) atomicMax(maxext, en);
+}
+
+// dependency codes: >= 0 LDS window index, -1 none, <= -2 index into the cell's external list
+__global__ void col_records(int nslots, int n, int ncol, int backward, int K, int CP, int L, int nlev,
+                            const int *__restrict__ chunk_row, const int *__restrict__ lev,
+                            const int *__restrict__ lchunk, const int *__restrict__ cstart,
+                            const int *__restrict__ slot_of_row, const int *__restrict__ dptr,
+                            const int *__restrict__ ddep, const int *__restrict__ dsrc, const int *__restrict__ epos,
+                            int *__restrict__ code, int *__restrict__ src, int *__restrict__ ext_slot) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int r = chunk_row[s];
+    if (r < 0) return;
+    const int l = lev[r];
+    const int S0 = lchunk[l] * WAVE, Wl = (lchunk[l + 1] - lchunk[l]) * WAVE;
+    const int q = s - S0;
+    const int c = colof(r, n, ncol, backward);
+    const int64_t base = (int64_t)s * L;  // thread-major: the row's L entries are contiguous
+    const int a = dptr[r], len = dptr[r + 1] - a;
+    int j = 0;
+    for (int k = 0; k < L; k++) {
+        int cd = -1, sr = -1;
+        if (k < len) {
+            const int d = ddep[a + k];
+            const int dc = colof(d, n, ncol, backward);
+            const int dl = lev[d];
+            const int ds = slot_of_row[d];
+            if (is_local(r, d, c, dc, lev, K)) {
+                const int dS0 = lchunk[dl] * WAVE;
+                cd = (dl % K) * CP + (ds - dS0 - cstart[(int64_t)dl * ncol + dc]);
+            } else {
+                ext_slot[epos[s] + j] = ds;
+                cd = -2 - (epos[s] + j);  // raw reference; replaced by the cell's unique index below
+                j++;
+            }
+            sr = dsrc[a + k];
+        }
+        code[base + k] = cd;
+        src[base + k] = sr;
+    }
+}
+
+// One workgroup per cell: sort the cell's external references, keep each
+// slot once (written back to the front of the cell's segment) and record for
+// every raw reference its index in that unique list.
+__global__ __launch_bounds__(256) void col_dedup(int nlev, int ncol, int *__restrict__ ci,
+                                                 int *__restrict__ ext_slot, int *__restrict__ uidx,
+                                                 int *__restrict__ maxext, int *__restrict__ fail) {
+    __shared__ int key[DEDUP_MAX], orig[DEDUP_MAX], flag[DEDUP_MAX];
+    __shared__ int nu_s;
+    const int cell = blockIdx.x;
+    const int l = cell / ncol, c = cell % ncol;
+    int *rec = ci + ((int64_t)c * nlev + l) * CI;
+    const int e0 = rec[3], en = rec[2] >> 16, size = rec[2] & 0xffff;
+    if (en == 0) return;
+    if (en > DEDUP_MAX) {
+        if (threadIdx.x == 0) atomicOr(fail, 1);
+        return;
+    }
+    int np = 1;
+    while (np < en) np <<= 1;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        const int v = i < en ? ext_slot[e0 + i] : INT_MAX;
+        key[i] = v;
+        if (i < en) orig[i] = v;
+    }
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                const int o = i ^ j;
+                if (o > i) {
+                    const int a = key[i], b = key[o];
+                    const bool up = (i & k) == 0;
+                    if (up ? (a > b) : (a < b)) {
+                        key[i] = b;
+                        key[o] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < en; i += blockDim.x) flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // compaction in order (en is small)
+        int u = 0;
+        for (int i = 0; i < en; i++)
+            if (flag[i]) key[u++] = key[i];
+        nu_s = u;
+    }
+    __syncthreads();
+    const int nu = nu_s;
+    for (int i = threadIdx.x; i < nu; i += blockDim.x) ext_slot[e0 + i] = key[i];
+    for (int i = threadIdx.x; i < en; i += blockDim.x) {
+        const int v = orig[i];
+        int lo = 0, hi = nu - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (key[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        uidx[e0 + i] = lo;
+    }
+    if (threadIdx.x == 0) {
+        rec[2] = size | (nu << 16);
+        atomicMax(maxext, nu);
+    }
+}
+
+// raw external references in the codes -> the cell's unique index
+__global__ void col_fixcodes(int64_t nrec, int *__restrict__ code, const int *__restrict__ uidx) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrec) return;
+    const int cd = code[e];
+    if (cd <= -2) code[e] = -2 - uidx[-cd - 2];
+}
+
+__global__ void row_lens(int n, const int *__restrict__ dptr, int *__restrict__ len) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) len[r] = dptr[r + 1] - dptr[r];
+}
+
+__global__ void init_range(int ncol, int nlev, int *__restrict__ colrange) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncol) {
+        colrange[2 * c] = nlev;
+        colrange[2 * c + 1] = -1;
+    }
+}
+
+// Packed row records, one per slot, RB bytes each (a multiple of 16, read
+// with 16-byte loads): int16 dependency codes [LR] | diagonal T | values T[LR].
+template <class T>
+__global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, const int *__restrict__ code,
+                         const int *__restrict__ src, const int *__restrict__ dg_src, const double *__restrict__ f,
+                         unsigned char *__restrict__ rec) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots_pad) return;
+    unsigned char *o = rec + (size_t)s * RB;
+    short *oc = reinterpret_cast<short *>(o);
+    T *od = reinterpret_cast<T *>(o + 2 * LR);
+    T *ov = od + 1;
+    const bool in = s < nslots;
+    for (int k = 0; k < LR; k++) {
+        const int e = in ? src[(int64_t)s * LR + k] : -1;
+        oc[k] = in ? (short)code[(int64_t)s * LR + k] : (short)-1;
+        ov[k] = e >= 0 ? (T)f[e] : T(0);
+    }
+    const int d = in ? dg_src[s] : -1;
+    *od = d >= 0 ? (T)f[d] : T(1);
+}
+
+// window + external double buffer + cell records (padded to whole pipeline blocks plus the read-ahead)
+__global__ void fill_i32_k(int *__restrict__ x, int64_t n, int v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// widest level, in slots (64-row chunks)
+int maxw_slots(const tri_plan &p) {
+    int w = 0;
+    for (int l = 0; l < p.nlev; l++) w = std::max(w, (p.h_level_ptr[l + 1] - p.h_level_ptr[l] + WAVE - 1) / WAVE * WAVE);
+    return w;
+}
+
+tri_colplan &tri_colplan_reset(tri_colplan &q) {
+    q.ok = false;
+    q.code.release();
+    q.src.release();
+    q.ext_slot.release();
+    return q;
+}
+
+int64_t lds_bytes(int K, int CP, int nlevcol, size_t tsize) {
+    const int pad = nlevcol + 3 * 8 + 2;  // read-ahead of the deepest pipeline (P = 8)
+    return (int64_t)K * CP * (int64_t)tsize + 2 * (int64_t)CP * (int64_t)tsize + (int64_t)pad * CI * 4;
+}
+
+}  // namespace
+
+bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
+    p.col.ok = false;
+    p.col.why = 1;
+    const int n = p.n, nlev = p.nlev, nslots = p.nchunk * WAVE;
+    if (n < 2 * WAVE || nlev < 2) return false;
+    int maxw = 0;
+    for (int l = 0; l < nlev; l++) maxw = std::max(maxw, p.h_level_ptr[l + 1] - p.h_level_ptr[l]);
+    dbuf lens;
+    lens.reserve((size_t)n * sizeof(int));
+    row_lens<<<(n + 255) / 256, 256, 0, c->stream>>>(n, p.dptr.as<int>(), lens.as<int>());
+    HALA_CHECK_LAUNCH();
+    const int maxlen = reduce_max_i32(c, lens.as<int>(), n);
+    p.col.why = 2;
+    if (maxlen > MAXD || maxlen < 1) return false;
+    const int g = (n + 255) / 256;
+    dbuf cellcnt, gap, cstart, ecnt, epos, ci, colrange, maxext;
+    gap.reserve(sizeof(int));
+    maxext.reserve(sizeof(int));
+    int ncol = ncol_req > 0 ? ncol_req : 4;
+    while (ncol_req <= 0 && (int64_t)maxw > (int64_t)ncol * CPMAX) ncol *= 2;
+    for (; ncol <= 512; ncol *= 2) {
+        if (ncol_req > 0 && ncol != ncol_req) break;
+        const int64_t ncell = (int64_t)nlev * ncol;
+        cellcnt.reserve((size_t)ncell * sizeof(int));
+        HALA_HIP(hipMemsetAsync(cellcnt.p, 0, (size_t)ncell * sizeof(int), c->stream));
+        HALA_HIP(hipMemsetAsync(gap.p, 0, sizeof(int), c->stream));
+        col_count<<<g, 256, 0, c->stream>>>(n, ncol, p.backward ? 1 : 0, p.lev.as<int>(), p.dptr.as<int>(),
+                                           p.ddep.as<int>(), cellcnt.as<int>(), gap.as<int>());
+        HALA_CHECK_LAUNCH();
+        const int maxcell = reduce_max_i32(c, cellcnt.as<int>(), (int)ncell);
+        const int CP = (maxcell + WAVE - 1) / WAVE * WAVE;
+        p.col.ncol = ncol;
+        p.col.CP = CP;
+        p.col.why = 3;
+        if (CP > CPMAX) continue;
+        const int K = std::min(fetch_i32(c, gap.as<int>()) + 1, KMAX);
+        // external counts, cell starts, cell records
+        ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
+        epos.reserve((size_t)(nslots + 1) * sizeof(int));
+        col_extcnt<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, n, ncol, p.backward ? 1 : 0, K,
+                                                               p.chunk_row.as<int>(), p.lev.as<int>(),
+                                                               p.dptr.as<int>(), p.ddep.as<int>(), ecnt.as<int>());
+        HALA_CHECK_LAUNCH();
+        exclusive_scan_i32(c, ecnt.as<int>(), epos.as<int>(), nslots);
+        const int next = fetch_i32(c, epos.as<int>() + nslots);
+        cstart.reserve((size_t)ncell * sizeof(int));
+        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, p.backward ? 1 : 0, cellcnt.as<int>(),
+                                                              cstart.as<int>());
+        HALA_CHECK_LAUNCH();
+        ci.reserve((size_t)ncell * CI * sizeof(int));
+        colrange.reserve((size_t)2 * ncol * sizeof(int));
+        init_range<<<(ncol + 63) / 64, 64, 0, c->stream>>>(ncol, nlev, colrange.as<int>());
+        HALA_HIP(hipMemsetAsync(maxext.p, 0, sizeof(int), c->stream));
+        col_cells<<<(int)((ncell + 255) / 256), 256, 0, c->stream>>>(nlev, ncol, p.lvl_chunk.as<int>(),
+                                                                    cellcnt.as<int>(), cstart.as<int>(),
+                                                                    epos.as<int>(), ci.as<int>(),
+                                                                    colrange.as<int>(), maxext.as<int>());
+        HALA_CHECK_LAUNCH();
+        std::vector<int> hr(2 * ncol);
+        HALA_HIP(hipMemcpyAsync(hr.data(), colrange.p, hr.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HALA_HIP(hipStreamSynchronize(c->stream));
+        int nlevcol = 0;
+        for (int k = 0; k < ncol; k++) nlevcol = std::max(nlevcol, hr[2 * k + 1] - hr[2 * k] + 1);
+        p.col.why = 4;
+        if (lds_bytes(K, CP, nlevcol, sizeof(double)) > LDS_BUDGET) continue;
+        // dependency records, level by level in k-major order
+        tri_colplan &q = p.col;
+        q.ncol = ncol;
+        q.K = K;
+        q.CP = CP;
+        q.nlevcol = nlevcol;
+        // record width: the kernel variant's dependency capacity (4 or 16)
+        q.L = maxlen <= 4 ? 4 : MAXD;
+        q.nrec = (int64_t)nslots * q.L;
+        q.next = next;
+        q.code.reserve((size_t)q.nrec * sizeof(int));
+        q.src.reserve((size_t)q.nrec * sizeof(int));
+        fill_i32_k<<<(int)((q.nrec + 255) / 256), 256, 0, c->stream>>>(q.code.as<int>(), q.nrec, -1);
+        fill_i32_k<<<(int)((q.nrec + 255) / 256), 256, 0, c->stream>>>(q.src.as<int>(), q.nrec, -1);
+        HALA_CHECK_LAUNCH();
+        HALA_CHECK_LAUNCH();
+        q.ext_slot.reserve((size_t)(next + BTMAX) * sizeof(int));
+        HALA_HIP(hipMemsetAsync(q.ext_slot.p, 0, q.ext_slot.bytes, c->stream));
+        col_records<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
+            nslots, n, ncol, p.backward ? 1 : 0, K, CP, q.L, nlev, p.chunk_row.as<int>(), p.lev.as<int>(),
+            p.lvl_chunk.as<int>(), cstart.as<int>(), p.slot_of_row.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
+            p.dsrc.as<int>(), epos.as<int>(), q.code.as<int>(), q.src.as<int>(), q.ext_slot.as<int>());
+        HALA_CHECK_LAUNCH();
+        // each external value staged once per cell
+        {
+            dbuf uidx, fail;
+            uidx.reserve((size_t)(next > 0 ? next : 1) * sizeof(int));
+            fail.reserve(sizeof(int));
+            HALA_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), c->stream));
+            HALA_HIP(hipMemsetAsync(maxext.p, 0, sizeof(int), c->stream));
+            col_dedup<<<(unsigned)ncell, 256, 0, c->stream>>>(nlev, ncol, ci.as<int>(), q.ext_slot.as<int>(),
+                                                            uidx.as<int>(), maxext.as<int>(), fail.as<int>());
+            HALA_CHECK_LAUNCH();
+            col_fixcodes<<<(int)((q.nrec + 255) / 256), 256, 0, c->stream>>>(q.nrec, q.code.as<int>(),
+                                                                             uidx.as<int>());
+            HALA_CHECK_LAUNCH();
+            const int mext = fetch_i32(c, maxext.as<int>());
+            q.maxext = mext;
+            const bool dfail = fetch_i32(c, fail.as<int>()) != 0;
+            if (dfail || mext > CP) {
+                tri_colplan_reset(q);
+                q.why = dfail ? 5 : 6;
+                continue;
+            }
+        }
+        q.ci.swap(ci);
+        q.colrange.swap(colrange);
+        q.xs64.reserve((size_t)(nslots + BTMAX) * sizeof(double));
+        HALA_HIP(hipStreamSynchronize(c->stream));
+        q.ok = true;
+        q.why = 0;
+        return true;
+    }
+    return false;
+}
+
+int rec_bytes(int LR, size_t tsize) {
+    const int b = 2 * LR + (int)tsize * (LR + 1);
+    return (b + 15) / 16 * 16;
+}
+
+void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
+    tri_colplan &q = p.col;
+    if (!q.ok) return;
+    const int nslots = p.nchunk * WAVE, npad = nslots + BTMAX;
+    const int RB = rec_bytes(q.L, f32 ? sizeof(float) : sizeof(double));
+    dbuf &dst = f32 ? q.val32 : q.val64;
+    dst.reserve((size_t)npad * RB);
+    const int g = (npad + 255) / 256;
+    if (f32) {
+        col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, q.code.as<int>(), q.src.as<int>(),
+                                                 p.dg_src.as<int>(), factor, dst.as<unsigned char>());
+        if (!q.xs32.p) q.xs32.reserve((size_t)npad * sizeof(float));
+    } else {
+        col_pack<double><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, q.code.as<int>(), q.src.as<int>(),
+                                                  p.dg_src.as<int>(), factor, dst.as<unsigned char>());
+    }
+    HALA_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+namespace {
+
+// Before a column sweep, over the whole chip: input into slot order, the
+// sweep's own polled output back to sentinels, and the partner plan's polled
+// output (if any) reset for its next sweep.
+template <class T, class TIN>
+__global__ void col_gather_in(int nslots, const int *__restrict__ chunk_row, const TIN *__restrict__ in,
+                              T *__restrict__ xs, T *__restrict__ out, T *__restrict__ reset,
+                              const int *__restrict__ reset_slot, const int *gate) {
+    if (gate && *gate) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    out[s] = sentinel<T>();
+    const int r = chunk_row[s];
+    if (r < 0) return;
+    xs[s] = (T)in[r];
+    if (reset) reset[reset_slot[r]] = sentinel<T>();
+}
+
+// After a column sweep: the slot-ordered result into natural order.
+template <class T, class TOUT>
+__global__ void col_scatter_out(int n, const int *__restrict__ slot_of_row, const T *__restrict__ out,
+                                TOUT *__restrict__ out_nat, const int *gate) {
+    if (gate && *gate) return;
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out_nat[r] = (TOUT)out[slot_of_row[r]];
+}
+
+template <class T, class TOUT>
+struct col_args {
+    int nlev, K, CP;
+    const int *ci, *colrange;
+    const int4 *rec;   // packed row records (see col_pack), NQ quads per slot
+    const int *ext_slot;
+    const T *xs;
+    T alpha;
+    T *out;
+    unsigned *ticket;  // [0] column ticket, [1] workgroups done
+    int *err;
+    const int *gate;
+};
+
+template <class T, int MAXD>
+struct col_rec {
+    static constexpr int NQ = (2 * MAXD + (int)sizeof(T) * (MAXD + 1) + 15) / 16;
+    int slot;   // -1 for a lane without a row
+    T xin;
+    int4 q[NQ];  // packed record: int16 codes [MAXD] | diagonal | values [MAXD]
+};
+
+// 32-bit word w of a packed record (static w: a register pick)
+template <int NQ>
+__device__ __forceinline__ int rec_word(const int4 (&q)[NQ], int w) {
+    const int4 v = q[w >> 2];
+    switch (w & 3) {
+    case 0: return v.x;
+    case 1: return v.y;
+    case 2: return v.z;
+    default: return v.w;
+    }
+}
+template <int NQ>
+__device__ __forceinline__ int rec_code(const int4 (&q)[NQ], int k) {
+    const int w = rec_word(q, k >> 1);
+    return (k & 1) ? (w >> 16) : (int)(short)(w & 0xffff);
+}
+template <class T, int NQ>
+__device__ __forceinline__ T rec_val(const int4 (&q)[NQ], int byte) {
+    if (sizeof(T) == 8)
+        return (T)__hiloint2double(rec_word(q, byte / 4 + 1), rec_word(q, byte / 4));
+    return (T)__int_as_float(rec_word(q, byte / 4));
+}
+
+// Slow path, kept out of line so that the steady-state loop has no loops or
+// conditional memory operations (the waits for the prefetched registers then
+// stay precise): spin until the value is published, bounded.
+template <class T>
+__device__ __attribute__((noinline)) T poll_slow(const T *p, int *err) {
+    T v = ld_agent(p);
+    unsigned spins = 0;
+    while (is_sent(v)) {
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_agent(p);
+        ++spins;
+        if (spins > SPIN_LIMIT || ((spins & 1023u) == 0 && ld_agent(err) != 0)) {
+            atomicOr(err, 1);
+            return T(0);
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One workgroup per column.  Pipeline (relative level lam = level - first):
+//   rec[lam % P]  row record of level lam, loaded P levels ahead;
+//   ev[lam % P]   external value for level lam + 1, polled P levels ahead
+//                 (evs: its slot, -1 for none);
+//   esr[lam % P]  external slot for level lam + P + 1, loaded P levels before that;
+//   extb[lam & 1] external values of level lam, staged in LDS.
+// Every step issues the same memory operations (lanes without work use
+// clamped addresses and a dummy target), and the level count is padded to a
+// multiple of P with empty levels: the loop body is straight-line code.
+template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int BT>
+__global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
+    if (a.gate && *a.gate) return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int CP = a.CP, K = a.K;
+    T *win = reinterpret_cast<T *>(smem);
+    T *extb = win + (size_t)K * CP;
+    int *cis = reinterpret_cast<int *>(extb + 2 * CP);
+    __shared__ int col_s;
+    const int t = threadIdx.x;
+    if (t == 0) col_s = (int)atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const int col = col_s;
+    const int first = a.colrange[2 * col], last = a.colrange[2 * col + 1];
+
+    const int nl = last - first + 1;
+    const int nlp = (nl + P - 1) / P * P;         // padded level count
+    const int ncis = nlp + 2 * P + 2;               // records read ahead reach lam + 2P + 1
+    for (int i = t; i < ncis * CI; i += blockDim.x)
+        cis[i] = (i < nl * CI) ? a.ci[((int64_t)col * a.nlev + first) * CI + i] : 0;
+    __syncthreads();
+
+    // Raw loads at unclamped addresses (every array is padded past its end
+    // by a workgroup's worth, so lanes without a row still read in bounds);
+    // masks are applied where the values are used, so no wait is forced here.
+    constexpr int NQ = col_rec<T, MAXD>::NQ;
+    auto load_rec = [&](col_rec<T, MAXD> &r, int lam) {
+        const int *ci = cis + lam * CI;
+        const int slot0 = ci[0], size = ci[2] & 0xffff;
+        const int s = slot0 + t;
+        r.slot = t < size ? s : -1;
+        r.xin = a.xs[s];
+        const int4 *src = a.rec + (size_t)s * NQ;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) r.q[i] = src[i];
+    };
+    // this lane's external slot for level lam (a valid slot even without one) and whether it has one
+    auto ext_has = [&](int lam) { return t < (cis[lam * CI + 2] >> 16); };
+    auto ext_raw = [&](int lam) { return a.ext_slot[cis[lam * CI + 3] + t]; };
+
+    col_rec<T, MAXD> rec[P];
+    T ev[P];
+    int evs[P], esr[P];
+    bool evh[P], esh[P];
+
+    // prologue: external slots of levels 1..P and level 0's externals first,
+    // then the prefetches in exactly the order the loop issues them, so the
+    // loop starts with the same outstanding loads it keeps in steady state
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        evh[i] = ext_has(i + 1);
+        evs[i] = ext_raw(i + 1);
+    }
+    if (ext_has(0)) extb[t] = poll_slow(a.out + ext_raw(0), a.err);
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        ev[i] = ld_agent(a.out + evs[i]);
+        esh[i] = ext_has(P + 1 + i);
+        esr[i] = ext_raw(P + 1 + i);
+        load_rec(rec[i], i);
+    }
+    lds_barrier();
+
+    for (int lam0 = 0; lam0 < nlp; lam0 += P) {
+#pragma unroll
+        for (int s = 0; s < P; s++) {
+            const int lam = lam0 + s;
+            const int lvl = first + lam;
+            const col_rec<T, MAXD> &r = rec[s];
+            const bool valid = r.slot >= 0;
+            const int eoff = K * CP + (lam & 1) * CP;  // the external buffer follows the window in LDS
+            T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
+#pragma unroll
+            for (int k = 0; k < MAXD; k++) {
+                const int cd = valid ? rec_code(r.q, k) : -1;
+                const T v = rec_val<T>(r.q, 2 * MAXD + (int)sizeof(T) * (k + 1));
+                const T x = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
+                if (MODE == TRI_SUB) {
+                    const T nacc = acc - v * x;
+                    acc = cd != -1 ? nacc : acc;
+                } else {
+                    const T nsum = sum + v * x;
+                    sum = cd != -1 ? nsum : sum;
+                }
+            }
+            if (MODE == TRI_SUM) acc = a.alpha * r.xin - sum;
+            if (!UNIT) acc = acc / rec_val<T>(r.q, 2 * MAXD);
+            win[(lvl % K) * CP + t] = acc;
+            if (valid) st_agent(a.out + r.slot, acc);
+            // externals of the next level into LDS
+            T v = ev[s];
+            if (evh[s] && is_sent(v)) v = poll_slow(a.out + evs[s], a.err);
+            extb[((lam + 1) & 1) * CP + t] = v;
+            // prefetch: externals of level lam + P + 1, their slots for lam + 2P + 1, record of lam + P
+            evh[s] = esh[s];
+            evs[s] = esr[s];
+            ev[s] = ld_agent(a.out + esr[s]);
+            esh[s] = ext_has(lam + 2 * P + 1);
+            esr[s] = ext_raw(lam + 2 * P + 1);
+            load_rec(rec[s], lam + P);
+            lds_barrier();
+        }
+    }
+    if (t == 0) {
+        unsigned d = atomicAdd(a.ticket + 1, 1u);
+        if (d == gridDim.x - 1) {
+            st_agent(a.ticket, 0u);
+            st_agent(a.ticket + 1, 0u);
+        }
+    }
+}
+
+template <class T, class TOUT, int MODE, bool UNIT>
+void launch_col(hala_ctx c, tri_plan &p, col_args<T, TOUT> &a, size_t lds) {
+    const tri_colplan &q = p.col;
+    dim3 grid(q.ncol), block(q.CP);
+    if (q.L == 4)
+        tri_col_kernel<T, TOUT, MODE, UNIT, 4, 8, CPMAX><<<grid, block, lds, c->stream>>>(a);
+    else
+        tri_col_kernel<T, TOUT, MODE, UNIT, 16, 4, CPMAX><<<grid, block, lds, c->stream>>>(a);
+}
+
+}  // namespace
+
+bool col_usable(hala_ctx c, const tri_plan &p, bool is64) {
+    const tri_colplan &q = p.col;
+    if (!q.ok || !c->opt.tri_col) return false;
+    if (q.CP > CPMAX) return false;
+    return is64 ? (q.val64.p != nullptr) : (q.val32.p != nullptr);
+}
+
+template <class T, class TIN, class TOUT>
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
+                  const int *reset_slot, const int *gate) {
+    constexpr bool is64 = sizeof(T) == 8;
+    tri_colplan &q = p.col;
+    const int nslots = p.nchunk * WAVE;
+    T *xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
+    T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
+    col_gather_in<T, TIN><<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in, xs, out,
+                                                                      reset, reset_slot, gate);
+    HALA_CHECK_LAUNCH();
+    col_args<T, TOUT> a;
+    a.nlev = p.nlev;
+    a.K = q.K;
+    a.CP = q.CP;
+    a.ci = q.ci.as<int>();
+    a.colrange = q.colrange.as<int>();
+    a.rec = is64 ? reinterpret_cast<const int4 *>(q.val64.p) : reinterpret_cast<const int4 *>(q.val32.p);
+    a.ext_slot = q.ext_slot.as<int>();
+    a.xs = xs;
+    a.alpha = alpha;
+    a.out = out;
+
+    a.ticket = counters(c) + 104;
+    a.err = wait_error_flag(c);
+    a.gate = gate;
+    const size_t lds = (size_t)lds_bytes(q.K, q.CP, q.nlevcol, sizeof(T));
+    if (mode == TRI_SUB) {
+        if (p.unit) launch_col<T, TOUT, TRI_SUB, true>(c, p, a, lds);
+        else launch_col<T, TOUT, TRI_SUB, false>(c, p, a, lds);
+    } else {
+        if (p.unit) launch_col<T, TOUT, TRI_SUM, true>(c, p, a, lds);
+        else launch_col<T, TOUT, TRI_SUM, false>(c, p, a, lds);
+    }
+    HALA_CHECK_LAUNCH();
+    if (out_nat) {
+        col_scatter_out<T, TOUT><<<(p.n + 255) / 256, 256, 0, c->stream>>>(p.n, p.slot_of_row.as<int>(), out, out_nat,
+                                                                          gate);
+        HALA_CHECK_LAUNCH();
+    }
+}
+
+template void tri_col_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *,
+                                                   double *, const int *, const int *);
+template void tri_col_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, float *,
+                                                  const int *, const int *);
+template void tri_col_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, float *,
+                                                const int *, const int *);
+
+}  // namespace hala_amd

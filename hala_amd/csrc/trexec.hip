@@ -187,6 +187,10 @@ void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *o
               const int *reset_slot, const int *gate) {
     constexpr bool is64 = sizeof(T) == 8;
     HALA_REQUIRE(is64 ? p.has64 : p.has32, HALA_ERR_ARG, "triangular solve: values not loaded for this precision");
+    if (col_usable(c, p, is64)) {
+        tri_col_exec<T, TIN, TOUT>(c, p, mode, alpha, in, out_nat, reset, reset_slot, gate);
+        return;
+    }
     tri_args<T, TIN, TOUT> a;
     a.out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
     a.ent_val = is64 ? reinterpret_cast<const T *>(p.ent_val64.p) : reinterpret_cast<const T *>(p.ent_val32.p);

@@ -247,7 +247,8 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     HALA_HIP(hipMemcpyAsync(p.h_level_ptr.data(), p.level_ptr.p, (size_t)(p.nlev + 1) * sizeof(int),
                             hipMemcpyDeviceToHost, c->stream));
     // chunks: ceil(level size / 64) per level
-    dbuf nch, chunk_ptr;
+    dbuf nch;
+    dbuf &chunk_ptr = p.lvl_chunk;
     nch.reserve((size_t)(p.nlev + 1) * sizeof(int));
     chunk_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));
     level_chunks<<<(p.nlev + 255) / 256, 256, 0, c->stream>>>(p.nlev, p.level_ptr.as<int>(), nch.as<int>());
@@ -255,11 +256,11 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     exclusive_scan_i32(c, nch.as<int>(), chunk_ptr.as<int>(), p.nlev);
     p.nchunk = fetch_i32(c, chunk_ptr.as<int>() + p.nlev);
     const int nslots = p.nchunk * WAVE;
-    p.chunk_row.reserve((size_t)nslots * sizeof(int));
+    p.chunk_row.reserve((size_t)(nslots + SLOT_PAD) * sizeof(int));
     p.slot_of_row.reserve((size_t)n * sizeof(int));
     p.chunk_len.reserve((size_t)(p.nchunk + 1) * sizeof(int));
     p.chunk_off.reserve((size_t)(p.nchunk + 1) * sizeof(int));
-    fill_i32_k<<<(nslots + 255) / 256, 256, 0, c->stream>>>(p.chunk_row.as<int>(), nslots, -1);
+    fill_i32_k<<<(nslots + SLOT_PAD + 255) / 256, 256, 0, c->stream>>>(p.chunk_row.as<int>(), nslots + SLOT_PAD, -1);
     fill_i32_k<<<(p.nchunk + 256) / 256, 256, 0, c->stream>>>(p.chunk_len.as<int>(), p.nchunk + 1, 0);
     HALA_CHECK_LAUNCH();
     place_rows<<<g, 256, 0, c->stream>>>(n, p.perm.as<int>(), p.lev.as<int>(), p.level_ptr.as<int>(),
@@ -284,6 +285,7 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     if (want32) p.out32.reserve((size_t)nslots * sizeof(float));
     tri_rearm(c, p);
     HALA_HIP(hipStreamSynchronize(c->stream));
+    if (c->opt.tri_col) build_col_plan(c, p, c->opt.tri_col_cols);
 }
 
 void tri_rearm(hala_ctx c, tri_plan &p) {
@@ -315,7 +317,7 @@ template <class T>
 static void load_values(hala_ctx c, tri_plan &p, const double *factor, dbuf &ent, dbuf &dg) {
     int64_t nslots = (int64_t)p.nchunk * WAVE;
     ent.reserve((size_t)(p.nent > 0 ? p.nent : 1) * sizeof(T));
-    dg.reserve((size_t)nslots * sizeof(T));
+    dg.reserve((size_t)(nslots + SLOT_PAD) * sizeof(T));
     if (p.nent > 0) {
         gather_vals<T><<<(int)((p.nent + 255) / 256), 256, 0, c->stream>>>(p.nent, p.ent_src.as<int>(), factor,
                                                                           ent.as<T>());
@@ -328,10 +330,12 @@ static void load_values(hala_ctx c, tri_plan &p, const double *factor, dbuf &ent
 
 void load_values64(hala_ctx c, tri_plan &p, const double *factor) {
     load_values<double>(c, p, factor, p.ent_val64, p.dg_val64);
+    col_load_values(c, p, factor, false);
     p.has64 = true;
 }
 void load_values32(hala_ctx c, tri_plan &p, const double *factor) {
     load_values<float>(c, p, factor, p.ent_val32, p.dg_val32);
+    col_load_values(c, p, factor, true);
     if (!p.out32.p) {
         p.out32.reserve((size_t)p.nchunk * WAVE * sizeof(float));
         tri_rearm(c, p);

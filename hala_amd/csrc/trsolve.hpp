@@ -27,6 +27,25 @@ enum tri_mode : int {
     TRI_SUM = 1   // s = 0; s += v * x_dep; acc = alpha*in - s; [acc /= diag]
 };
 
+// Column-blocked layout of a plan (see trcol.hip): rows cut into ncol
+// contiguous blocks, each swept level by level by one workgroup.
+struct tri_colplan {
+    bool ok = false;
+    int ncol = 0;     // columns (one workgroup each)
+    int K = 0;        // levels kept in the LDS window
+    int CP = 0;       // rows of the largest cell, rounded up to 64 (= threads per workgroup)
+    int L = 0;        // dependency record width (longest row)
+    int nlevcol = 0;  // most levels spanned by one column
+    int64_t nrec = 0;
+    int next = 0;     // external-list entries
+    int maxext = 0;   // most distinct external values one cell stages
+    int why = 0;      // when not built: 1 small, 2 long rows, 3 cells too wide, 4 LDS, 5/6 externals
+    dbuf ci, colrange;           // cell records (column-major, 5 ints each), level range per column
+    dbuf code, src, ext_slot;    // dependency codes and factor positions (k-major per level); external slots
+    dbuf val64, val32;           // dependency values in record order
+    dbuf xs64, xs32;             // input, gathered into slot order per sweep
+};
+
 struct tri_plan {
     int n = 0;
     bool backward = false;
@@ -40,6 +59,7 @@ struct tri_plan {
     dbuf lev, level_ptr, perm;
     std::vector<int> h_level_ptr;
     dbuf chunk_row, chunk_len, chunk_off, slot_of_row;
+    dbuf lvl_chunk;  // first chunk of every level (nlev + 1)
 
     dbuf ent_dep, ent_src, dg_src;
     // values
@@ -47,6 +67,7 @@ struct tri_plan {
     bool has64 = false, has32 = false;
     // polled outputs in slot order (all-sentinel between solves)
     dbuf out64, out32;
+    tri_colplan col;
 };
 
 // dependency selection for build_deps()
@@ -82,6 +103,17 @@ template <class T, class TIN, class TOUT>
 void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
               const int *reset_slot, const int *gate = nullptr);
 
+// Column-blocked layout (trcol.hip): built after build_schedule when the
+// pattern allows it (returns false and leaves p.col.ok unset otherwise);
+// ncol_req = 0 picks the column count.  tri_exec uses it when present and
+// enabled (option tri_col).
+bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req);
+void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32);
+bool col_usable(hala_ctx c, const tri_plan &p, bool is64);
+template <class T, class TIN, class TOUT>
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
+                  const int *reset_slot, const int *gate);
+
 // Refills the polled output buffer(s) with sentinels.
 void tri_rearm(hala_ctx c, tri_plan &p);
 
@@ -89,6 +121,7 @@ void tri_rearm(hala_ctx c, tri_plan &p);
 int *wait_error_flag(hala_ctx c);
 
 constexpr int WAVE = 64;
+constexpr int SLOT_PAD = 1024;             // slot-indexed arrays run this far past their end (idle-lane reads)
 constexpr int MAXD = 16;                   // dependencies held in registers per row (longer rows loop)
 constexpr unsigned SPIN_LIMIT = 1u << 22;  // polls before a wait gives up (a few seconds)
 void check_wait_error(hala_ctx c);

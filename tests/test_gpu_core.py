@@ -209,14 +209,15 @@ def test_ilu_symbolic_numeric_apply(engine, name):
         assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply_t(p, i, dg, f, x)))
 
 
-TRI_MODES = [  # (tri_streams, tri_pool_waves)
-    (1, 0), (2, 0), (4, 0), (4, 8), (3, 12), (1, 100000)]
+TRI_MODES = [  # (tri_col, tri_col_cols, tri_streams, tri_pool_waves)
+    (0, 0, 1, 0), (0, 0, 2, 0), (0, 0, 4, 0), (0, 0, 4, 8), (0, 0, 3, 12), (0, 0, 1, 100000),
+    (1, 0, 1, 0), (1, 4, 1, 0), (1, 16, 1, 0), (1, 64, 1, 0)]
 
 
-@pytest.mark.parametrize("mode", TRI_MODES, ids=lambda m: f"streams{m[0]}_pool{m[1]}")
+@pytest.mark.parametrize("mode", TRI_MODES, ids=lambda m: f"col{m[0]}x{m[1]}_streams{m[2]}_pool{m[3]}")
 def test_tri_launch_modes_bitexact(engine, mode):
-    """Every way of running the triangular-solve pool gives the same bits."""
-    keys = ("tri_streams", "tri_pool_waves")
+    """Every way of running the triangular sweeps (level pool, column-blocked) gives the same bits."""
+    keys = ("tri_col", "tri_col_cols", "tri_streams", "tri_pool_waves")
     saved = [engine.get_option(k) for k in keys]
     try:
         for k, val in zip(keys, mode):
