@@ -27,6 +27,9 @@ constexpr int KMAX = 8;     // levels held in the LDS window at most
 constexpr int BTMAX = 1024;  // padding of slot-indexed arrays (idle-lane reads)
 constexpr int CPMAX = 256;   // rows of a cell at most (= threads of a column workgroup)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
+#ifndef COL_PE
+#define COL_PE 1  // levels an external value is polled ahead of its use
+#endif
 constexpr int64_t LDS_BUDGET = 64 * 1024;
 
 __device__ __forceinline__ int colof(int r, int n, int ncol, int backward) {
@@ -534,16 +537,20 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // One workgroup per column.  Pipeline (relative level lam = level - first):
-//   rec[lam % P]  row record of level lam, loaded P levels ahead;
-//   ev[lam % P]   external value for level lam + 1, polled P levels ahead
-//                 (evs: its slot, -1 for none);
-//   esr[lam % P]  external slot for level lam + P + 1, loaded P levels before that;
-//   extb[lam & 1] external values of level lam, staged in LDS.
+//   rec[lam % P]   row record of level lam, loaded P levels ahead;
+//   esr[lam % P]   external slot for level lam + P + 1, loaded P levels ahead;
+//   ev[lam % PE]   external value for level lam + 1, polled PE levels ahead
+//                  (evs: its slot; evh: whether there is one).  PE < P: an
+//                  external is produced by another column only shortly
+//                  before it is needed, so it is read just in time (one load
+//                  round trip ahead), while the static records go far ahead;
+//   extb[lam & 1]  external values of level lam, staged in LDS.
 // Every step issues the same memory operations (lanes without work use
 // clamped addresses and a dummy target), and the level count is padded to a
 // multiple of P with empty levels: the loop body is straight-line code.
-template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int BT>
+template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int PE, int BT>
 __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
+    static_assert(P % PE == 0 && PE < P, "external ring must divide the record ring and be shorter");
     if (a.gate && *a.gate) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int CP = a.CP, K = a.K;
@@ -583,22 +590,24 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     auto ext_raw = [&](int lam) { return a.ext_slot[cis[lam * CI + 3] + t]; };
 
     col_rec<T, MAXD> rec[P];
-    T ev[P];
-    int evs[P], esr[P];
-    bool evh[P], esh[P];
+    T ev[PE];
+    int evs[PE], esr[P];
+    bool evh[PE], esh[P];
 
     // prologue: external slots of levels 1..P and level 0's externals first,
     // then the prefetches in exactly the order the loop issues them, so the
     // loop starts with the same outstanding loads it keeps in steady state
 #pragma unroll
-    for (int i = 0; i < P; i++) {
+    for (int i = 0; i < PE; i++) {
         evh[i] = ext_has(i + 1);
         evs[i] = ext_raw(i + 1);
     }
     if (ext_has(0)) extb[t] = poll_slow(a.out + ext_raw(0), a.err);
+    // esr[i]: slots of level i + P + 1 (read at step i + P - PE); levels
+    // PE + 1 .. P are read directly in the first block of steps
 #pragma unroll
     for (int i = 0; i < P; i++) {
-        ev[i] = ld_agent(a.out + evs[i]);
+        if (i < PE) ev[i] = ld_agent(a.out + evs[i]);
         esh[i] = ext_has(P + 1 + i);
         esr[i] = ext_raw(P + 1 + i);
         load_rec(rec[i], i);
@@ -632,15 +641,30 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
             win[(lvl % K) * CP + t] = acc;
             if (valid) st_agent(a.out + r.slot, acc);
             // externals of the next level into LDS
-            T v = ev[s];
-            if (evh[s] && is_sent(v)) v = poll_slow(a.out + evs[s], a.err);
+            const int se = s % PE;  // static once unrolled
+            T v = ev[se];
+            if (evh[se] && is_sent(v)) v = poll_slow(a.out + evs[se], a.err);
             extb[((lam + 1) & 1) * CP + t] = v;
-            // prefetch: externals of level lam + P + 1, their slots for lam + 2P + 1, record of lam + P
-            evh[s] = esh[s];
-            evs[s] = esr[s];
-            ev[s] = ld_agent(a.out + esr[s]);
-            esh[s] = ext_has(lam + 2 * P + 1);
-            esr[s] = ext_raw(lam + 2 * P + 1);
+            // prefetch: externals of level lam + PE + 1 (slots loaded earlier, or read
+            // directly while the slot ring is not yet that far), slots of level
+            // lam + P + 1, record of level lam + P
+            {
+                const int le = lam + PE + 1;
+                bool h;
+                int sl;
+                if (lam0 == 0 && s + PE < P) {  // slot ring not yet filled this far
+                    h = ext_has(le);
+                    sl = ext_raw(le);
+                } else {
+                    h = esh[(s + PE) % P];
+                    sl = esr[(s + PE) % P];
+                }
+                evh[se] = h;
+                evs[se] = sl;
+                ev[se] = ld_agent(a.out + sl);
+            }
+            esh[s] = ext_has(lam + P + 1);
+            esr[s] = ext_raw(lam + P + 1);
             load_rec(rec[s], lam + P);
             lds_barrier();
         }
@@ -659,9 +683,9 @@ void launch_col(hala_ctx c, tri_plan &p, col_args<T, TOUT> &a, size_t lds) {
     const tri_colplan &q = p.col;
     dim3 grid(q.ncol), block(q.CP);
     if (q.L == 4)
-        tri_col_kernel<T, TOUT, MODE, UNIT, 4, 8, CPMAX><<<grid, block, lds, c->stream>>>(a);
+        tri_col_kernel<T, TOUT, MODE, UNIT, 4, 8, (COL_PE < 8 ? COL_PE : 4), CPMAX><<<grid, block, lds, c->stream>>>(a);
     else
-        tri_col_kernel<T, TOUT, MODE, UNIT, 16, 4, CPMAX><<<grid, block, lds, c->stream>>>(a);
+        tri_col_kernel<T, TOUT, MODE, UNIT, 16, 4, (COL_PE < 4 ? COL_PE : 2), CPMAX><<<grid, block, lds, c->stream>>>(a);
 }
 
 }  // namespace
