@@ -25,7 +25,8 @@ namespace {
 constexpr int CI = 5;       // ints per (column, level) cell record
 constexpr int KMAX = 8;     // levels held in the LDS window at most
 constexpr int BTMAX = 1024;  // padding of slot-indexed arrays (idle-lane reads)
-constexpr int CPMAX = 256;   // rows of a cell at most (= threads of a column workgroup)
+constexpr int CPMAX = 256;   // rows of a cell at most (= compute threads of a column workgroup)
+constexpr int ELMAX = 8;     // external values per loader lane at most (512 per cell)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
 #ifndef COL_PE
 #define COL_PE 1  // levels an external value is polled ahead of its use
@@ -283,9 +284,19 @@ tri_colplan &tri_colplan_reset(tri_colplan &q) {
     return q;
 }
 
-int64_t lds_bytes(int K, int CP, int nlevcol, size_t tsize) {
+// window (K levels x CP rows) + external double buffer (2 x 64 EL values) +
+// slots of the polls in flight (COL_PE x 64 EL) + cell records
+int64_t lds_bytes(int K, int CP, int EL, int nlevcol, size_t tsize) {
     const int pad = nlevcol + 3 * 8 + 2;  // read-ahead of the deepest pipeline (P = 8)
-    return (int64_t)K * CP * (int64_t)tsize + 2 * (int64_t)CP * (int64_t)tsize + (int64_t)pad * CI * 4;
+    const int64_t xc = (int64_t)WAVE * EL;
+    return (int64_t)K * CP * (int64_t)tsize + 2 * xc * (int64_t)tsize + COL_PE * xc * 4 + (int64_t)pad * CI * 4;
+}
+
+// loader-lane capacity for a cell with m distinct externals (0 when m is too large)
+int ext_lanes(int m) {
+    int el = 2;
+    while (el * WAVE < m) el *= 2;
+    return el <= ELMAX ? el : 0;
 }
 
 }  // namespace
@@ -354,7 +365,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         int nlevcol = 0;
         for (int k = 0; k < ncol; k++) nlevcol = std::max(nlevcol, hr[2 * k + 1] - hr[2 * k] + 1);
         p.col.why = 4;
-        if (lds_bytes(K, CP, nlevcol, sizeof(double)) > LDS_BUDGET) continue;
+        if (lds_bytes(K, CP, 2, nlevcol, sizeof(double)) > LDS_BUDGET) continue;
         // dependency records, level by level in k-major order
         tri_colplan &q = p.col;
         q.ncol = ncol;
@@ -393,10 +404,20 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
             HALA_CHECK_LAUNCH();
             const int mext = fetch_i32(c, maxext.as<int>());
             q.maxext = mext;
+            // loader capacities built: 2 or 4 values per lane with the short-row
+            // records, 4 or 8 with the long-row ones
+            q.EL = ext_lanes(mext);
+            if (q.L > 4 && q.EL == 2) q.EL = 4;
+            if (q.L == 4 && q.EL > 4) q.EL = 0;
             const bool dfail = fetch_i32(c, fail.as<int>()) != 0;
-            if (dfail || mext > CP) {
+            if (dfail || q.EL == 0) {
                 tri_colplan_reset(q);
                 q.why = dfail ? 5 : 6;
+                continue;
+            }
+            if (lds_bytes(K, CP, q.EL, nlevcol, sizeof(double)) > LDS_BUDGET) {
+                tri_colplan_reset(q);
+                q.why = 4;
                 continue;
             }
         }
@@ -536,27 +557,37 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// One workgroup per column.  Pipeline (relative level lam = level - first):
-//   rec[lam % P]   row record of level lam, loaded P levels ahead;
-//   esr[lam % P]   external slot for level lam + P + 1, loaded P levels ahead;
-//   ev[lam % PE]   external value for level lam + 1, polled PE levels ahead
-//                  (evs: its slot; evh: whether there is one).  PE < P: an
-//                  external is produced by another column only shortly
-//                  before it is needed, so it is read just in time (one load
-//                  round trip ahead), while the static records go far ahead;
-//   extb[lam & 1]  external values of level lam, staged in LDS.
-// Every step issues the same memory operations (lanes without work use
-// clamped addresses and a dummy target), and the level count is padded to a
-// multiple of P with empty levels: the loop body is straight-line code.
-template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int PE, int BT>
+// One workgroup per column: CP/64 compute waves (one row of a cell per
+// thread) and one loader wave.  Relative level lam = level - first.
+//
+// Compute waves: rec[lam % P] is the row record of level lam, loaded P levels
+// ahead.  Their only global loads are these records, so the wait for a
+// record never waits for anything younger.
+//
+// Loader wave: lane l stages the cell's distinct externals l, l + 64, ...
+// (EL per lane).  An external is written by another column only shortly
+// before it is needed, so it is polled PE levels ahead (ev/evs rings), with
+// its slot read from the external list P levels before that (esr ring).  The
+// loader is a wave of its own because a wave's loads retire in issue order:
+// a compute wave that also polled would wait, every level, for a poll issued
+// one or two levels earlier, and the record prefetch would be lost.
+//
+// extb[lam & 1] holds the externals of level lam in LDS; a level ends with an
+// LDS-only wait and the workgroup barrier.  Every step of either loop issues
+// the same memory operations (idle lanes read valid padding), and the level
+// count is padded to a multiple of P with empty levels, so both loop bodies
+// are straight-line code.
+template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int PE, int EL, int BT>
 __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
-    static_assert(P % PE == 0 && PE < P, "external ring must divide the record ring and be shorter");
+    static_assert(P % PE == 0, "poll ring must divide the record ring");
+    constexpr int XC = WAVE * EL;  // external values per level
     if (a.gate && *a.gate) return;
     extern __shared__ __align__(16) unsigned char smem[];
     const int CP = a.CP, K = a.K;
     T *win = reinterpret_cast<T *>(smem);
     T *extb = win + (size_t)K * CP;
-    int *cis = reinterpret_cast<int *>(extb + 2 * CP);
+    int *eslot = reinterpret_cast<int *>(extb + 2 * XC);  // slots of the polls in flight
+    int *cis = eslot + PE * XC;
     __shared__ int col_s;
     const int t = threadIdx.x;
     if (t == 0) col_s = (int)atomicAdd(a.ticket, 1u);
@@ -565,127 +596,141 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     const int first = a.colrange[2 * col], last = a.colrange[2 * col + 1];
 
     const int nl = last - first + 1;
-    const int nlp = (nl + P - 1) / P * P;         // padded level count
-    const int ncis = nlp + 2 * P + 2;               // records read ahead reach lam + 2P + 1
+    const int nlp = (nl + P - 1) / P * P;  // padded level count
+    const int ncis = nlp + 2 * P + 2;      // read-ahead reaches lam + PE + P + 1
     for (int i = t; i < ncis * CI; i += blockDim.x)
         cis[i] = (i < nl * CI) ? a.ci[((int64_t)col * a.nlev + first) * CI + i] : 0;
     __syncthreads();
 
-    // Raw loads at unclamped addresses (every array is padded past its end
-    // by a workgroup's worth, so lanes without a row still read in bounds);
-    // masks are applied where the values are used, so no wait is forced here.
-    constexpr int NQ = col_rec<T, MAXD>::NQ;
-    auto load_rec = [&](col_rec<T, MAXD> &r, int lam) {
-        const int *ci = cis + lam * CI;
-        const int slot0 = ci[0], size = ci[2] & 0xffff;
-        const int s = slot0 + t;
-        r.slot = t < size ? s : -1;
-        r.xin = a.xs[s];
-        const int4 *src = a.rec + (size_t)s * NQ;
+    // wave index made scalar: each wave takes one branch as a whole
+    const int wave = __builtin_amdgcn_readfirstlane(t / WAVE);
+    if (wave == CP / WAVE) {
+        // ---------------- loader wave ----------------
+        const int l = t - CP;
+        auto has = [&](int lam, int e) { return l + WAVE * e < (cis[lam * CI + 2] >> 16); };
+        // slot of external l + 64 e of level lam (a valid slot even past the cell's list)
+        auto raw = [&](int lam, int e) { return a.ext_slot[cis[lam * CI + 3] + l + WAVE * e]; };
+        // polled address: workgroup-uniform base + 32-bit byte offset (no 64-bit
+        // register pair per slot, so a freshly loaded slot is used where it lands)
+        const char *obase = reinterpret_cast<const char *>(a.out);
+        auto at = [&](int slot) { return reinterpret_cast<const T *>(obase + (unsigned)slot * (unsigned)sizeof(T)); };
+        T ev[PE][EL];
+        int esr[P][EL];
+        // level 0 directly; polls of levels 1..PE; slots of levels PE+1 .. PE+P
 #pragma unroll
-        for (int i = 0; i < NQ; i++) r.q[i] = src[i];
-    };
-    // this lane's external slot for level lam (a valid slot even without one) and whether it has one
-    auto ext_has = [&](int lam) { return t < (cis[lam * CI + 2] >> 16); };
-    auto ext_raw = [&](int lam) { return a.ext_slot[cis[lam * CI + 3] + t]; };
-
-    col_rec<T, MAXD> rec[P];
-    T ev[PE];
-    int evs[PE], esr[P];
-    bool evh[PE], esh[P];
-
-    // prologue: external slots of levels 1..P and level 0's externals first,
-    // then the prefetches in exactly the order the loop issues them, so the
-    // loop starts with the same outstanding loads it keeps in steady state
+        for (int e = 0; e < EL; e++)
+            if (has(0, e)) extb[l + WAVE * e] = poll_slow(at(raw(0, e)), a.err);
 #pragma unroll
-    for (int i = 0; i < PE; i++) {
-        evh[i] = ext_has(i + 1);
-        evs[i] = ext_raw(i + 1);
-    }
-    if (ext_has(0)) extb[t] = poll_slow(a.out + ext_raw(0), a.err);
-    // esr[i]: slots of level i + P + 1 (read at step i + P - PE); levels
-    // PE + 1 .. P are read directly in the first block of steps
+        for (int i = 0; i < PE; i++)
 #pragma unroll
-    for (int i = 0; i < P; i++) {
-        if (i < PE) ev[i] = ld_agent(a.out + evs[i]);
-        esh[i] = ext_has(P + 1 + i);
-        esr[i] = ext_raw(P + 1 + i);
-        load_rec(rec[i], i);
-    }
-    lds_barrier();
-
-    for (int lam0 = 0; lam0 < nlp; lam0 += P) {
-#pragma unroll
-        for (int s = 0; s < P; s++) {
-            const int lam = lam0 + s;
-            const int lvl = first + lam;
-            const col_rec<T, MAXD> &r = rec[s];
-            const bool valid = r.slot >= 0;
-            const int eoff = K * CP + (lam & 1) * CP;  // the external buffer follows the window in LDS
-            T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
-#pragma unroll
-            for (int k = 0; k < MAXD; k++) {
-                const int cd = valid ? rec_code(r.q, k) : -1;
-                const T v = rec_val<T>(r.q, 2 * MAXD + (int)sizeof(T) * (k + 1));
-                const T x = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
-                if (MODE == TRI_SUB) {
-                    const T nacc = acc - v * x;
-                    acc = cd != -1 ? nacc : acc;
-                } else {
-                    const T nsum = sum + v * x;
-                    sum = cd != -1 ? nsum : sum;
-                }
+            for (int e = 0; e < EL; e++) {
+                const int sl = raw(i + 1, e);
+                eslot[i * XC + l + WAVE * e] = sl;
+                ev[i][e] = ld_agent(at(sl));
             }
-            if (MODE == TRI_SUM) acc = a.alpha * r.xin - sum;
-            if (!UNIT) acc = acc / rec_val<T>(r.q, 2 * MAXD);
-            win[(lvl % K) * CP + t] = acc;
-            if (valid) st_agent(a.out + r.slot, acc);
-            // externals of the next level into LDS
-            const int se = s % PE;  // static once unrolled
-            T v = ev[se];
-            if (evh[se] && is_sent(v)) v = poll_slow(a.out + evs[se], a.err);
-            extb[((lam + 1) & 1) * CP + t] = v;
-            // prefetch: externals of level lam + PE + 1 (slots loaded earlier, or read
-            // directly while the slot ring is not yet that far), slots of level
-            // lam + P + 1, record of level lam + P
-            {
-                const int le = lam + PE + 1;
-                bool h;
-                int sl;
-                if (lam0 == 0 && s + PE < P) {  // slot ring not yet filled this far
-                    h = ext_has(le);
-                    sl = ext_raw(le);
-                } else {
-                    h = esh[(s + PE) % P];
-                    sl = esr[(s + PE) % P];
+#pragma unroll
+        for (int i = 0; i < P; i++)
+#pragma unroll
+            for (int e = 0; e < EL; e++) esr[i][e] = raw(PE + 1 + i, e);
+        lds_barrier();
+        for (int lam0 = 0; lam0 < nlp; lam0 += P) {
+#pragma unroll
+            for (int s = 0; s < P; s++) {
+                const int lam = lam0 + s;
+                const int se = s % PE;  // static once unrolled
+                // externals of level lam + 1 into LDS
+                T *dst = extb + ((lam + 1) & 1) * XC + l;
+#pragma unroll
+                for (int e = 0; e < EL; e++) {
+                    T v = ev[se][e];
+                    if (has(lam + 1, e) && is_sent(v)) v = poll_slow(at(eslot[se * XC + l + WAVE * e]), a.err);
+                    dst[WAVE * e] = v;
                 }
-                evh[se] = h;
-                evs[se] = sl;
-                ev[se] = ld_agent(a.out + sl);
+                // polls of level lam + PE + 1, slots of level lam + PE + P + 1
+#pragma unroll
+                for (int e = 0; e < EL; e++) {
+                    eslot[se * XC + l + WAVE * e] = esr[s][e];
+                    ev[se][e] = ld_agent(at(esr[s][e]));
+                }
+#pragma unroll
+                for (int e = 0; e < EL; e++) esr[s][e] = raw(lam + PE + P + 1, e);
+                lds_barrier();
             }
-            esh[s] = ext_has(lam + P + 1);
-            esr[s] = ext_raw(lam + P + 1);
-            load_rec(rec[s], lam + P);
-            lds_barrier();
+        }
+    } else {
+        // ---------------- compute waves ----------------
+        constexpr int NQ = col_rec<T, MAXD>::NQ;
+        auto load_rec = [&](col_rec<T, MAXD> &r, int lam) {
+            const int *ci = cis + lam * CI;
+            const int slot0 = ci[0], size = ci[2] & 0xffff;
+            const int s = slot0 + t;
+            r.slot = t < size ? s : -1;
+            r.xin = a.xs[s];  // slot arrays are padded past their end: idle lanes read in bounds
+            const int4 *src = a.rec + (size_t)s * NQ;
+#pragma unroll
+            for (int i = 0; i < NQ; i++) r.q[i] = src[i];
+        };
+        col_rec<T, MAXD> rec[P];
+#pragma unroll
+        for (int i = 0; i < P; i++) load_rec(rec[i], i);
+        lds_barrier();
+        for (int lam0 = 0; lam0 < nlp; lam0 += P) {
+#pragma unroll
+            for (int s = 0; s < P; s++) {
+                const int lam = lam0 + s;
+                const int lvl = first + lam;
+                const col_rec<T, MAXD> &r = rec[s];
+                const bool valid = r.slot >= 0;
+                const int eoff = K * CP + (lam & 1) * XC;  // the external buffer follows the window
+                T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
+#pragma unroll
+                for (int k = 0; k < MAXD; k++) {
+                    const int cd = valid ? rec_code(r.q, k) : -1;
+                    const T v = rec_val<T>(r.q, 2 * MAXD + (int)sizeof(T) * (k + 1));
+                    const T x = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
+                    if (MODE == TRI_SUB) {
+                        const T nacc = acc - v * x;
+                        acc = cd != -1 ? nacc : acc;
+                    } else {
+                        const T nsum = sum + v * x;
+                        sum = cd != -1 ? nsum : sum;
+                    }
+                }
+                if (MODE == TRI_SUM) acc = a.alpha * r.xin - sum;
+                if (!UNIT) acc = acc / rec_val<T>(r.q, 2 * MAXD);
+                win[(lvl % K) * CP + t] = acc;
+                if (valid) st_agent(a.out + r.slot, acc);
+                load_rec(rec[s], lam + P);
+                lds_barrier();
+            }
+        }
+        if (t == 0) {
+            unsigned d = atomicAdd(a.ticket + 1, 1u);
+            if (d == gridDim.x - 1) {
+                st_agent(a.ticket, 0u);
+                st_agent(a.ticket + 1, 0u);
+            }
         }
     }
-    if (t == 0) {
-        unsigned d = atomicAdd(a.ticket + 1, 1u);
-        if (d == gridDim.x - 1) {
-            st_agent(a.ticket, 0u);
-            st_agent(a.ticket + 1, 0u);
-        }
-    }
+}
+
+template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int EL>
+void launch_col_el(hala_ctx c, const tri_colplan &q, col_args<T, TOUT> &a, size_t lds) {
+    constexpr int PE = COL_PE < P ? COL_PE : P;
+    tri_col_kernel<T, TOUT, MODE, UNIT, MAXD, P, PE, EL, CPMAX + WAVE>
+        <<<dim3(q.ncol), dim3(q.CP + WAVE), lds, c->stream>>>(a);
 }
 
 template <class T, class TOUT, int MODE, bool UNIT>
 void launch_col(hala_ctx c, tri_plan &p, col_args<T, TOUT> &a, size_t lds) {
     const tri_colplan &q = p.col;
-    dim3 grid(q.ncol), block(q.CP);
-    if (q.L == 4)
-        tri_col_kernel<T, TOUT, MODE, UNIT, 4, 8, (COL_PE < 8 ? COL_PE : 4), CPMAX><<<grid, block, lds, c->stream>>>(a);
-    else
-        tri_col_kernel<T, TOUT, MODE, UNIT, 16, 4, (COL_PE < 4 ? COL_PE : 2), CPMAX><<<grid, block, lds, c->stream>>>(a);
+    if (q.L == 4) {
+        if (q.EL == 2) launch_col_el<T, TOUT, MODE, UNIT, 4, 8, 2>(c, q, a, lds);
+        else launch_col_el<T, TOUT, MODE, UNIT, 4, 8, 4>(c, q, a, lds);
+    } else {
+        if (q.EL == 4) launch_col_el<T, TOUT, MODE, UNIT, 16, 4, 4>(c, q, a, lds);
+        else launch_col_el<T, TOUT, MODE, UNIT, 16, 4, 8>(c, q, a, lds);
+    }
 }
 
 }  // namespace
@@ -723,7 +768,7 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOU
     a.ticket = counters(c) + 104;
     a.err = wait_error_flag(c);
     a.gate = gate;
-    const size_t lds = (size_t)lds_bytes(q.K, q.CP, q.nlevcol, sizeof(T));
+    const size_t lds = (size_t)lds_bytes(q.K, q.CP, q.EL, q.nlevcol, sizeof(T));
     if (mode == TRI_SUB) {
         if (p.unit) launch_col<T, TOUT, TRI_SUB, true>(c, p, a, lds);
         else launch_col<T, TOUT, TRI_SUB, false>(c, p, a, lds);

@@ -33,12 +33,13 @@ struct tri_colplan {
     bool ok = false;
     int ncol = 0;     // columns (one workgroup each)
     int K = 0;        // levels kept in the LDS window
-    int CP = 0;       // rows of the largest cell, rounded up to 64 (= threads per workgroup)
+    int CP = 0;       // rows of the largest cell, rounded up to 64 (= compute threads per workgroup)
     int L = 0;        // dependency record width (longest row)
     int nlevcol = 0;  // most levels spanned by one column
     int64_t nrec = 0;
     int next = 0;     // external-list entries
     int maxext = 0;   // most distinct external values one cell stages
+    int EL = 0;       // external values per lane of the loader wave (power of two, >= maxext / 64)
     int why = 0;      // when not built: 1 small, 2 long rows, 3 cells too wide, 4 LDS, 5/6 externals
     dbuf ci, colrange;           // cell records (column-major, 5 ints each), level range per column
     dbuf code, src, ext_slot;    // dependency codes and factor positions (k-major per level); external slots

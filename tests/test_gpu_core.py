@@ -243,6 +243,47 @@ def test_tri_launch_modes_bitexact(engine, mode):
             engine.set_option(k, val)
 
 
+COL_CASES = {  # name: (matrix, columns, lowest and highest distinct externals of a cell expected)
+    # short rows (<= 4 dependencies): loader lanes hold 2 or 4 values each
+    "lap3d_40_c16": (lambda: st.laplace3d(40, 40, 40), 16, 1, 128),
+    "lap3d_200x200x6_c8": (lambda: st.laplace3d(200, 200, 6), 8, 129, 256),
+    # long rows (27-point): 4 or 8 values per loader lane
+    "cd27_24_c16": (lambda: st.convdiff27(24, seed=3), 16, 1, 256),
+    "cd27_72_c32": (lambda: st.convdiff27(72, seed=3), 32, 257, 512),
+}
+
+
+@pytest.mark.parametrize("name", sorted(COL_CASES))
+def test_column_sweep_capacities_bitexact(engine, name):
+    """Column-blocked sweeps run (not the level-pool fallback) at every staging capacity, bit-exact."""
+    gen, ncol, lo, hi = COL_CASES[name]
+    keys = ("tri_col", "tri_col_cols")
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        engine.set_option("tri_col", 1)
+        engine.set_option("tri_col_cols", ncol)
+        p, i, v = gen()
+        n = len(p) - 1
+        M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
+        for upper in (False, True):
+            info = M.sweep_info(upper)
+            assert info["column"] and info["ncol"] == ncol, info
+            assert lo <= info["max_ext"] <= hi, info
+        dg, _ = oracle.diag(p, i)
+        f = oracle.ilu0(p, i, v, dg)
+        x = st.uniform(23, n)
+        gr = engine.vector(n)
+        for _ in range(2):
+            M.apply(dev(engine, x), gr)
+            assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply(p, i, dg, f, x)))
+        M.apply_f32x64(dev(engine, x), gr)
+        ref = oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x.astype(np.float32)).astype(np.float64)
+        assert np.array_equal(bits(host(gr)), bits(ref))
+    finally:
+        for k, val in zip(keys, saved):
+            engine.set_option(k, val)
+
+
 def test_ilu_reference_known_answer(engine):
     # tests/sparse_tests.hpp:423-440
     p = np.array([0, 3, 6, 9], np.int32)
