@@ -8,9 +8,10 @@ ILU(0) analysis on the device (diagonal, level sets, level-ordered layout),
 numeric factorisation, then device-resident PCG until ||r|| < 1e-10 ||b||.
 value = mean seconds per step (lower is better).
 
-roofline:      the kernel that dominates a step's time -- the level-scheduled
-               triangular sweep (one launch per triangle, two per PCG pass) --
-               timed live with HIP events on the engine stream;
+roofline:      what dominates a step's time -- the triangular sweep of the
+               ILU(0) apply (two per PCG pass; each is a gather pre-pass, the
+               column-blocked sweep kernel and a scatter post-pass) -- timed
+               live with HIP events on the engine stream;
 roofline_spmv: the CSR SpMV (the bandwidth kernel of the path);
 cpu_baseline:  the C restatement of HALA's CPU engine, one thread, on a
                bounded sample of the same solve;
@@ -364,11 +365,13 @@ def main():
         "solve_ms": round(1e3 * float(np.mean(solves)), 3),
         "iters_per_s": round((it - 1) / float(np.mean(solves)), 2),
         "final_relres": relres,
-        "roofline": {"kernel": "tri_exec_kernel<f64> (sync-free level-scheduled ILU(0) sweep, one triangle)",
+        "roofline": {"kernel": "ILU(0) triangular sweep, one triangle: col_gather_in + tri_col_kernel<f64> + "
+                               "col_scatter_out",
                      "bound": "hbm", "achieved": round(ach_sweep, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_exec_bytes"),
+                     "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_sweep_bytes"),
                      "bytes_per_launch": by_sweep, "avg_launch_ms": round(ms_sweep, 5),
-                     "note": "latency-bound: 382 dependency levels per triangle"},
+                     "note": f"latency-bound: {ilu.info()['nlev_lower']} dependency levels per triangle, "
+                             f"{ilu.sweep_info(False)['ncol']} row-block columns"},
         "roofline_spmv": {"kernel": "spmv_rows_kernel<f64> (CSR, LDS-staged 256-row blocks)", "bound": "hbm",
                           "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic.get("spmv_bytes"),

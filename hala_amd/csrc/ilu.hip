@@ -62,19 +62,15 @@ __global__ void set_int(int *p, int v) { *p = v; }
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
-    tri_exec<double, double, double>(c, M->L, TRI_SUB, 1.0, x, M->tmp64.as<double>(), M->U.out64.as<double>(),
-                                     M->U.slot_of_row.as<int>(), gate);
-    tri_exec<double, double, double>(c, M->U, TRI_SUB, 1.0, M->tmp64.as<double>(), r, M->L.out64.as<double>(),
-                                     M->L.slot_of_row.as<int>(), gate);
+    tri_exec<double, double, double>(c, M->L, TRI_SUB, 1.0, x, M->tmp64.as<double>(), gate);
+    tri_exec<double, double, double>(c, M->U, TRI_SUB, 1.0, M->tmp64.as<double>(), r, gate);
 }
 
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs hala_ilu_factor_f64(..., want_f32=1)");
-    tri_exec<float, double, double>(c, M->L, TRI_SUB, 1.0f, x, M->tmp64.as<double>(), M->U.out32.as<float>(),
-                                    M->U.slot_of_row.as<int>(), gate);
-    tri_exec<float, double, double>(c, M->U, TRI_SUB, 1.0f, M->tmp64.as<double>(), r, M->L.out32.as<float>(),
-                                    M->L.slot_of_row.as<int>(), gate);
+    tri_exec<float, double, double>(c, M->L, TRI_SUB, 1.0f, x, M->tmp64.as<double>(), gate);
+    tri_exec<float, double, double>(c, M->U, TRI_SUB, 1.0f, M->tmp64.as<double>(), r, gate);
 }
 
 void ilu_prepare_adjoint(hala_ilu M) {
@@ -100,10 +96,8 @@ void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
     ilu_prepare_adjoint(M);
-    tri_exec<double, double, double>(c, M->Ut, TRI_SUB, 1.0, x, M->tmp64.as<double>(), M->Lt.out64.as<double>(),
-                                     M->Lt.slot_of_row.as<int>(), gate);
-    tri_exec<double, double, double>(c, M->Lt, TRI_SUB, 1.0, M->tmp64.as<double>(), r, M->Ut.out64.as<double>(),
-                                     M->Ut.slot_of_row.as<int>(), gate);
+    tri_exec<double, double, double>(c, M->Ut, TRI_SUB, 1.0, x, M->tmp64.as<double>(), gate);
+    tri_exec<double, double, double>(c, M->Lt, TRI_SUB, 1.0, M->tmp64.as<double>(), r, gate);
 }
 
 }  // namespace hala_amd
@@ -234,10 +228,8 @@ int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
         hala_ctx c = M->ctx;
         set_device(c);
         M->tmp32.reserve((size_t)M->n * sizeof(float));
-        tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>(), M->U.out32.as<float>(),
-                                      M->U.slot_of_row.as<int>());
-        tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r, M->L.out32.as<float>(),
-                                      M->L.slot_of_row.as<int>());
+        tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>());
+        tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r);
     });
 }
 

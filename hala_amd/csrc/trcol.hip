@@ -25,7 +25,9 @@ namespace {
 constexpr int CI = 5;       // ints per (column, level) cell record
 constexpr int KMAX = 8;     // levels held in the LDS window at most
 constexpr int BTMAX = 1024;  // padding of slot-indexed arrays (idle-lane reads)
-constexpr int CPMAX = 256;   // rows of a cell at most (= compute threads of a column workgroup)
+constexpr int CPMAX = 512;   // rows of a cell at most (= compute threads of a column workgroup)
+constexpr int CPMAX_LONG = 256;  // the same for long-row records (register budget of their prefetch)
+constexpr int STEP_BYTES = 20 * 1024;  // automatic column count: record bytes one level of a column reads at most
 constexpr int ELMAX = 8;     // external values per loader lane at most (512 per cell)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
 #ifndef COL_PE
@@ -292,6 +294,12 @@ int64_t lds_bytes(int K, int CP, int EL, int nlevcol, size_t tsize) {
     return (int64_t)K * CP * (int64_t)tsize + 2 * xc * (int64_t)tsize + COL_PE * xc * 4 + (int64_t)pad * CI * 4;
 }
 
+// bytes of one packed row record: int16 codes [LR] | diagonal | values [LR], 16-byte multiple
+int rec_bytes(int LR, size_t tsize) {
+    const int b = 2 * LR + (int)tsize * (LR + 1);
+    return (b + 15) / 16 * 16;
+}
+
 // loader-lane capacity for a cell with m distinct externals (0 when m is too large)
 int ext_lanes(int m) {
     int el = 2;
@@ -319,8 +327,11 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     dbuf cellcnt, gap, cstart, ecnt, epos, ci, colrange, maxext;
     gap.reserve(sizeof(int));
     maxext.reserve(sizeof(int));
+    // record width: the kernel variant's dependency capacity (4 or 16)
+    const int L = maxlen <= 4 ? 4 : MAXD;
+    const int cpmax = L == 4 ? CPMAX : CPMAX_LONG;
     int ncol = ncol_req > 0 ? ncol_req : 4;
-    while (ncol_req <= 0 && (int64_t)maxw > (int64_t)ncol * CPMAX) ncol *= 2;
+    while (ncol_req <= 0 && (int64_t)maxw > (int64_t)ncol * cpmax) ncol *= 2;
     for (; ncol <= 512; ncol *= 2) {
         if (ncol_req > 0 && ncol != ncol_req) break;
         const int64_t ncell = (int64_t)nlev * ncol;
@@ -335,7 +346,9 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         p.col.ncol = ncol;
         p.col.CP = CP;
         p.col.why = 3;
-        if (CP > CPMAX) continue;
+        if (CP > cpmax) continue;
+        // automatic choice: narrower cells until a level's records fit the per-step budget
+        if (ncol_req <= 0 && ncol < 512 && (int64_t)CP * rec_bytes(L, sizeof(double)) > STEP_BYTES) continue;
         const int K = std::min(fetch_i32(c, gap.as<int>()) + 1, KMAX);
         // external counts, cell starts, cell records
         ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
@@ -372,8 +385,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         q.K = K;
         q.CP = CP;
         q.nlevcol = nlevcol;
-        // record width: the kernel variant's dependency capacity (4 or 16)
-        q.L = maxlen <= 4 ? 4 : MAXD;
+        q.L = L;
         q.nrec = (int64_t)nslots * q.L;
         q.next = next;
         q.code.reserve((size_t)q.nrec * sizeof(int));
@@ -432,11 +444,6 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     return false;
 }
 
-int rec_bytes(int LR, size_t tsize) {
-    const int b = 2 * LR + (int)tsize * (LR + 1);
-    return (b + 15) / 16 * 16;
-}
-
 void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
     tri_colplan &q = p.col;
     if (!q.ok) return;
@@ -461,21 +468,17 @@ void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// Before a column sweep, over the whole chip: input into slot order, the
-// sweep's own polled output back to sentinels, and the partner plan's polled
-// output (if any) reset for its next sweep.
+// Before a column sweep, over the whole chip: the sweep's polled output back
+// to sentinels and the input into slot order.
 template <class T, class TIN>
 __global__ void col_gather_in(int nslots, const int *__restrict__ chunk_row, const TIN *__restrict__ in,
-                              T *__restrict__ xs, T *__restrict__ out, T *__restrict__ reset,
-                              const int *__restrict__ reset_slot, const int *gate) {
+                              T *__restrict__ xs, T *__restrict__ out, const int *gate) {
     if (gate && *gate) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     out[s] = sentinel<T>();
     const int r = chunk_row[s];
-    if (r < 0) return;
-    xs[s] = (T)in[r];
-    if (reset) reset[reset_slot[r]] = sentinel<T>();
+    if (r >= 0) xs[s] = (T)in[r];
 }
 
 // After a column sweep: the slot-ordered result into natural order.
@@ -493,7 +496,7 @@ struct col_args {
     const int *ci, *colrange;
     const int4 *rec;   // packed row records (see col_pack), NQ quads per slot
     const int *ext_slot;
-    const T *xs;
+    const T *xs;  // input in slot order
     T alpha;
     T *out;
     unsigned *ticket;  // [0] column ticket, [1] workgroups done
@@ -560,9 +563,12 @@ __device__ __forceinline__ void lds_barrier() {
 // One workgroup per column: CP/64 compute waves (one row of a cell per
 // thread) and one loader wave.  Relative level lam = level - first.
 //
-// Compute waves: rec[lam % P] is the row record of level lam, loaded P levels
-// ahead.  Their only global loads are these records, so the wait for a
-// record never waits for anything younger.
+// Compute waves: rec[lam % P] is the row record of level lam and its input
+// value, loaded P levels ahead.  These are their only global loads, so the
+// wait for a record never waits for anything younger.  (The input is gathered
+// into slot order by a pre-pass over the whole chip: read through the row map
+// inside this kernel, each load instruction would touch 64 cache lines on the
+// critical path.)
 //
 // Loader wave: lane l stages the cell's distinct externals l, l + 64, ...
 // (EL per lane).  An external is written by another column only shortly
@@ -671,8 +677,13 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
             for (int i = 0; i < NQ; i++) r.q[i] = src[i];
         };
         col_rec<T, MAXD> rec[P];
+        // in the loop's own issue order, so that the loop is entered with the
+        // outstanding loads it keeps in steady state
 #pragma unroll
-        for (int i = 0; i < P; i++) load_rec(rec[i], i);
+        for (int i = 0; i < P; i++) {
+            load_rec(rec[i], i);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         lds_barrier();
         for (int lam0 = 0; lam0 < nlp; lam0 += P) {
 #pragma unroll
@@ -717,8 +728,8 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
 template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int EL>
 void launch_col_el(hala_ctx c, const tri_colplan &q, col_args<T, TOUT> &a, size_t lds) {
     constexpr int PE = COL_PE < P ? COL_PE : P;
-    tri_col_kernel<T, TOUT, MODE, UNIT, MAXD, P, PE, EL, CPMAX + WAVE>
-        <<<dim3(q.ncol), dim3(q.CP + WAVE), lds, c->stream>>>(a);
+    constexpr int BT = (MAXD == 4 ? CPMAX : CPMAX_LONG) + WAVE;
+    tri_col_kernel<T, TOUT, MODE, UNIT, MAXD, P, PE, EL, BT><<<dim3(q.ncol), dim3(q.CP + WAVE), lds, c->stream>>>(a);
 }
 
 template <class T, class TOUT, int MODE, bool UNIT>
@@ -738,20 +749,19 @@ void launch_col(hala_ctx c, tri_plan &p, col_args<T, TOUT> &a, size_t lds) {
 bool col_usable(hala_ctx c, const tri_plan &p, bool is64) {
     const tri_colplan &q = p.col;
     if (!q.ok || !c->opt.tri_col) return false;
-    if (q.CP > CPMAX) return false;
+    if (q.CP > (q.L == 4 ? CPMAX : CPMAX_LONG)) return false;
     return is64 ? (q.val64.p != nullptr) : (q.val32.p != nullptr);
 }
 
-template <class T, class TIN, class TOUT>
-void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
-                  const int *reset_slot, const int *gate) {
+template <class T, class TOUT>
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate) {
     constexpr bool is64 = sizeof(T) == 8;
     tri_colplan &q = p.col;
     const int nslots = p.nchunk * WAVE;
     T *xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
     T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
-    col_gather_in<T, TIN><<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in, xs, out,
-                                                                      reset, reset_slot, gate);
+    col_gather_in<T, TOUT><<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in, xs, out,
+                                                                       gate);
     HALA_CHECK_LAUNCH();
     col_args<T, TOUT> a;
     a.nlev = p.nlev;
@@ -784,11 +794,8 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOU
     }
 }
 
-template void tri_col_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *,
-                                                   double *, const int *, const int *);
-template void tri_col_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, float *,
-                                                  const int *, const int *);
-template void tri_col_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, float *,
-                                                const int *, const int *);
+template void tri_col_exec<double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, const int *);
+template void tri_col_exec<float, double>(hala_ctx, tri_plan &, int, float, const double *, double *, const int *);
+template void tri_col_exec<float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, const int *);
 
 }  // namespace hala_amd

@@ -2,6 +2,7 @@
 // pool over level-ordered chunks).
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "trsolve.hpp"
 
@@ -43,8 +44,6 @@ struct tri_args {
     const TIN *in;
     T *out;
     TOUT *out_nat;
-    T *reset;
-    const int *reset_slot;
     unsigned *ticket;  // nstream counters TK_STRIDE words apart; "waves finished" at TK_MAX * TK_STRIDE
     int nstream;
     int *err;
@@ -83,11 +82,9 @@ __global__ __launch_bounds__(256) void tri_exec_kernel(tri_args<T, TIN, TOUT> a)
         // wait, so that nothing but the dependency values sits between the
         // last producer's store and this row's own store.
         T xin = T(0), dgv = T(1);
-        int rslot = 0;
         if (row >= 0) {
             xin = (T)a.in[row];
             if (!UNIT) dgv = a.dg_val[slot];
-            if (a.reset) rslot = a.reset_slot[row];
         }
         // TRI_SUB: acc = in, acc -= v*x term by term; TRI_SUM: acc = alpha*in - sum(v*x)
         auto publish = [&](T acc, T sum) {
@@ -95,7 +92,6 @@ __global__ __launch_bounds__(256) void tri_exec_kernel(tri_args<T, TIN, TOUT> a)
             if (!UNIT) acc = acc / dgv;
             st_agent(a.out + slot, acc);
             if (a.out_nat) a.out_nat[row] = (TOUT)acc;
-            if (a.reset) a.reset[rslot] = sentinel<T>();
         };
         if (len <= MAXD) {
             int d[MAXD];
@@ -183,14 +179,15 @@ __global__ __launch_bounds__(256) void tri_exec_kernel(tri_args<T, TIN, TOUT> a)
 
 
 template <class T, class TIN, class TOUT>
-void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
-              const int *reset_slot, const int *gate) {
+void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, const int *gate) {
     constexpr bool is64 = sizeof(T) == 8;
     HALA_REQUIRE(is64 ? p.has64 : p.has32, HALA_ERR_ARG, "triangular solve: values not loaded for this precision");
-    if (col_usable(c, p, is64)) {
-        tri_col_exec<T, TIN, TOUT>(c, p, mode, alpha, in, out_nat, reset, reset_slot, gate);
+    static_assert(std::is_same<TIN, TOUT>::value, "input and output share a type");
+    if (col_usable(c, p, is64)) {  // (its gather pass re-arms the polled output)
+        tri_col_exec<T, TOUT>(c, p, mode, alpha, in, out_nat, gate);
         return;
     }
+    tri_rearm_prec(c, p, is64);
     tri_args<T, TIN, TOUT> a;
     a.out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
     a.ent_val = is64 ? reinterpret_cast<const T *>(p.ent_val64.p) : reinterpret_cast<const T *>(p.ent_val32.p);
@@ -208,8 +205,6 @@ void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *o
     a.alpha = alpha;
     a.in = in;
     a.out_nat = out_nat;
-    a.reset = reset;
-    a.reset_slot = reset_slot;
     a.ticket = tri_tickets(c);
     a.err = wait_error_flag(c);
     a.gate = gate;
@@ -224,8 +219,8 @@ void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *o
     HALA_CHECK_LAUNCH();
 }
 
-template void tri_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, double *, const int *, const int *);
-template void tri_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, float *, const int *, const int *);
-template void tri_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, float *, const int *, const int *);
+template void tri_exec<double, double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, const int *);
+template void tri_exec<float, double, double>(hala_ctx, tri_plan &, int, float, const double *, double *, const int *);
+template void tri_exec<float, float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, const int *);
 
 }  // namespace hala_amd

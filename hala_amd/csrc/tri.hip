@@ -104,8 +104,7 @@ int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, cons
                      "hala_trsv_f64: trans must be N, T or C");
         tri_plan &p = tr ? T->T : T->N;
         load_values64(c, p, vals);
-        tri_rearm(c, p);
-        tri_exec<double, double, double>(c, p, TRI_SUM, alpha, b, x, nullptr, nullptr);
+        tri_exec<double, double, double>(c, p, TRI_SUM, alpha, b, x);
         check_wait_error(c);
     });
 }

@@ -288,6 +288,14 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     if (c->opt.tri_col) build_col_plan(c, p, c->opt.tri_col_cols);
 }
 
+void tri_rearm_prec(hala_ctx c, tri_plan &p, bool is64) {
+    int64_t nslots = (int64_t)p.nchunk * WAVE;
+    int g = (int)((nslots + 255) / 256);
+    if (is64) fill_sent_k<double><<<g, 256, 0, c->stream>>>(p.out64.as<double>(), nslots);
+    else fill_sent_k<float><<<g, 256, 0, c->stream>>>(p.out32.as<float>(), nslots);
+    HALA_CHECK_LAUNCH();
+}
+
 void tri_rearm(hala_ctx c, tri_plan &p) {
     int64_t nslots = (int64_t)p.nchunk * WAVE;
     int g = (int)((nslots + 255) / 256);

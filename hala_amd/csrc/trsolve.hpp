@@ -43,7 +43,7 @@ struct tri_colplan {
     int why = 0;      // when not built: 1 small, 2 long rows, 3 cells too wide, 4 LDS, 5/6 externals
     dbuf ci, colrange;           // cell records (column-major, 5 ints each), level range per column
     dbuf code, src, ext_slot;    // dependency codes and factor positions (k-major per level); external slots
-    dbuf val64, val32;           // dependency values in record order
+    dbuf val64, val32;           // packed row records (see col_pack)
     dbuf xs64, xs32;             // input, gathered into slot order per sweep
 };
 
@@ -66,7 +66,7 @@ struct tri_plan {
     // values
     dbuf ent_val64, ent_val32, dg_val64, dg_val32;
     bool has64 = false, has32 = false;
-    // polled outputs in slot order (all-sentinel between solves)
+    // polled outputs in slot order (re-armed to sentinels at the start of every sweep)
     dbuf out64, out32;
     tri_colplan col;
 };
@@ -97,12 +97,11 @@ void load_values64(hala_ctx c, tri_plan &p, const double *factor);
 void load_values32(hala_ctx c, tri_plan &p, const double *factor);
 
 // Solves with plan p.  in: natural-order input (TIN), out_nat: natural-order
-// output (TOUT, nullable).  reset/reset_slot: the partner plan's polled buffer
-// and slot map, set back to sentinels for its next solve (nullable).
+// output (TOUT).  The plan's polled output of this precision is re-armed first,
+// so sweeps of any kind may follow each other on the same plans.
 // gate (nullable, device): when *gate != 0 at launch the solve does nothing.
 template <class T, class TIN, class TOUT>
-void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
-              const int *reset_slot, const int *gate = nullptr);
+void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, const int *gate = nullptr);
 
 // Column-blocked layout (trcol.hip): built after build_schedule when the
 // pattern allows it (returns false and leaves p.col.ok unset otherwise);
@@ -111,12 +110,12 @@ void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *o
 bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req);
 void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32);
 bool col_usable(hala_ctx c, const tri_plan &p, bool is64);
-template <class T, class TIN, class TOUT>
-void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *out_nat, T *reset,
-                  const int *reset_slot, const int *gate);
+template <class T, class TOUT>
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate);
 
-// Refills the polled output buffer(s) with sentinels.
+// Refills the polled output buffer(s) with sentinels (both precisions, or one).
 void tri_rearm(hala_ctx c, tri_plan &p);
+void tri_rearm_prec(hala_ctx c, tri_plan &p, bool is64);
 
 // Device error flag raised by a dependency wait that gave up; read and cleared by check_wait_error.
 int *wait_error_flag(hala_ctx c);
