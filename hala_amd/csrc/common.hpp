@@ -308,6 +308,18 @@ enum lb_site : int { LB_DOT = 0, LB_CG_START = 1, LB_CG_SPMV = 2, LB_CG_UPDATE =
                      LB_GM_COL = 64 /* .. 64 + 64 Krylov columns */ };
 constexpr int GM_MAX_RESTART = 64;
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the
+// chip's 8 XCDs (workgroup b runs on XCD b % 8), each XCD with its own L2.
+// A kernel launched with xcd_grid(nvirt) workgroups handles virtual block
+// xcd_block(nvirt) (>= nvirt: nothing): every XCD takes a contiguous eighth
+// of the virtual blocks, so blocks that touch the same lines share an L2.
+constexpr int NXCD = 8;
+inline int xcd_grid(int nvirt) { return (nvirt + NXCD - 1) / NXCD * NXCD; }
+__device__ __forceinline__ int xcd_block(int nvirt) {
+    const int per = (nvirt + NXCD - 1) / NXCD;
+    return (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+}
+
 inline int grid_for(int64_t work, int per_block, int cap) {
     int64_t g = (work + per_block - 1) / per_block;
     if (g < 1) g = 1;

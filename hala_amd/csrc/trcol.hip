@@ -470,11 +470,13 @@ namespace {
 
 // Before a column sweep, over the whole chip: the sweep's polled output back
 // to sentinels and the input into slot order.
+// (Slots run level by level; a line of the input holds rows of ~16
+// consecutive levels, so blocks are spread XCD-contiguously to reuse it in L2.)
 template <class T, class TIN>
 __global__ void col_gather_in(int nslots, const int *__restrict__ chunk_row, const TIN *__restrict__ in,
                               T *__restrict__ xs, T *__restrict__ out, const int *gate) {
     if (gate && *gate) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block((nslots + 255) / 256) * 256 + threadIdx.x;
     if (s >= nslots) return;
     out[s] = sentinel<T>();
     const int r = chunk_row[s];
@@ -486,7 +488,7 @@ template <class T, class TOUT>
 __global__ void col_scatter_out(int n, const int *__restrict__ slot_of_row, const T *__restrict__ out,
                                 TOUT *__restrict__ out_nat, const int *gate) {
     if (gate && *gate) return;
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = xcd_block((n + 255) / 256) * 256 + threadIdx.x;
     if (r < n) out_nat[r] = (TOUT)out[slot_of_row[r]];
 }
 
@@ -760,8 +762,8 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     const int nslots = p.nchunk * WAVE;
     T *xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
     T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
-    col_gather_in<T, TOUT><<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in, xs, out,
-                                                                       gate);
+    col_gather_in<T, TOUT><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in,
+                                                                                 xs, out, gate);
     HALA_CHECK_LAUNCH();
     col_args<T, TOUT> a;
     a.nlev = p.nlev;
@@ -788,8 +790,8 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     }
     HALA_CHECK_LAUNCH();
     if (out_nat) {
-        col_scatter_out<T, TOUT><<<(p.n + 255) / 256, 256, 0, c->stream>>>(p.n, p.slot_of_row.as<int>(), out, out_nat,
-                                                                          gate);
+        col_scatter_out<T, TOUT><<<xcd_grid((p.n + 255) / 256), 256, 0, c->stream>>>(p.n, p.slot_of_row.as<int>(), out,
+                                                                                    out_nat, gate);
         HALA_CHECK_LAUNCH();
     }
 }
