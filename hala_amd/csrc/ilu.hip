@@ -69,6 +69,10 @@ void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs hala_ilu_factor_f64(..., want_f32=1)");
+    if (c->opt.ilu32_jacobi > 0) {
+        ilu_apply32x64_jacobi(M, x, r, gate, c->opt.ilu32_jacobi);
+        return;
+    }
     tri_exec<float, double, double>(c, M->L, TRI_SUB, 1.0f, x, M->tmp64.as<double>(), gate);
     tri_exec<float, double, double>(c, M->U, TRI_SUB, 1.0f, M->tmp64.as<double>(), r, gate);
 }
@@ -199,6 +203,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         int b = fetch_i32(c, bad);
         M->factored = false;
         M->adjoint_loaded = false;
+        M->jac_values = false;
         HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
                      "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
         load_values64(c, M->L, f);

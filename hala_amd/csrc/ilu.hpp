@@ -16,6 +16,10 @@ struct hala_ilu_s {
     hala_amd::csr_transpose tp;
     hala_amd::dbuf tmp64, tmp32;  // natural-order hand-off between the two solves
     bool factored = false, has32 = false, adjoint = false, adjoint_loaded = false;
+    // Jacobi-applied FP32 factor (ilujac.hip): packed strict-L / strict-U parts
+    // with FP32 values, FP32 diagonal, four FP32 iterates (built on first use)
+    hala_amd::dbuf jl_ptr, jl_idx, jl_val, ju_ptr, ju_idx, ju_val, jd_val, jwork;
+    bool jac_pattern = false, jac_values = false;
 };
 
 struct hala_tri_s {
@@ -39,4 +43,7 @@ void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullp
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_prepare_adjoint(hala_ilu M);
+// r = (LU)^-1 x with the FP32 factor applied by k Jacobi sweeps per triangle (ilujac.hip)
+void ilu_apply32x64_jacobi(hala_ilu M, const double *x, double *r, const int *gate, int k);
+void ilu_jacobi_load(hala_ilu M);
 }  // namespace hala_amd

@@ -27,11 +27,12 @@ struct spmv_tile {
 // nonzeros are streamed in coalesced tiles: every thread forms products
 // vals[j] * x[indx[j]] into LDS, then each thread adds the products of its own
 // row in CSR order.  Returns the row sum (TA(0) for rows past the end).
+// SUB: start from `init` and subtract the products one at a time instead.
 // Must be called by all 256 threads of the block.
-template <class TV, class TX, class TA>
+template <class TV, class TX, class TA, bool SUB = false>
 __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
                                                const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod,
-                                               int blk) {
+                                               int blk, TA init = TA(0)) {
     constexpr int TILE = spmv_tile<TA>::value;
     const int r0 = blk * SPMV_ROWS;
     const int row = r0 + threadIdx.x;
@@ -43,7 +44,7 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
         rs = pntr[row];
         re = pntr[row + 1];
     }
-    TA acc = TA(0);
+    TA acc = init;
     for (int base = bstart; base < bend; base += TILE) {
         const int cnt = min(TILE, bend - base);
         int e = threadIdx.x;
@@ -63,7 +64,7 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
         }
         __syncthreads();
         const int a = max(rs, base), b = min(re, base + cnt);
-        for (int j = a; j < b; j++) acc = acc + prod[j - base];
+        for (int j = a; j < b; j++) acc = SUB ? acc - prod[j - base] : acc + prod[j - base];
         __syncthreads();
     }
     return acc;

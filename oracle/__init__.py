@@ -38,12 +38,13 @@ _SIG = {
     "hala_or_ilu0_kij": (None, [i32, vp, vp, vp, vp]),
     "hala_or_ilu_apply": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_ilu_apply_f32": (None, [i32, vp, vp, vp, vp, vp]),
+    "hala_or_ilu_apply_f32_jacobi": (None, [i32, vp, vp, vp, vp, i32, vp, vp]),
     "hala_or_ilu_apply_t": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_trsv": (None, [ch, ch, ch, i32, f64, vp, vp, vp, vp, vp]),
     "hala_or_cg": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, f64, i32, vp, i32]),
     "hala_or_cg_team": (i32, [i32, vp, vp, vp, vp, vp, vp, f64, i32, i32, vp, i32]),
     "hala_or_gmres": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, f64, i32, i32, vp, i32, pi32]),
-    "hala_or_gmres_ir": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, vp, i32, pi32]),
+    "hala_or_gmres_ir": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, vp, i32, pi32, i32]),
     "hala_or_rotg": (None, [vp, vp, vp, vp]),
     "hala_or_rot": (None, [vp, vp, f64, f64]),
     "hala_or_tpsv_un": (None, [i32, vp, vp]),
@@ -176,6 +177,15 @@ def ilu_apply_f32(pntr, indx, dg, ilu32, x32):
     return x
 
 
+def ilu_apply_f32_jacobi(pntr, indx, dg, ilu32, x32, k):
+    """(LU)^-1 x with the FP32 factor applied by k Jacobi sweeps per triangle (hala_or_ilu_apply_f32_jacobi)."""
+    x = np.ascontiguousarray(x32, np.float32).copy()
+    w = np.empty(3 * x.size, np.float32)
+    lib().hala_or_ilu_apply_f32_jacobi(len(pntr) - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)),
+                                       P(np.ascontiguousarray(ilu32, np.float32)), int(k), P(x), P(w))
+    return x
+
+
 def ilu_apply_t(pntr, indx, dg, ilu, x):
     x = c64(x).copy()
     lib().hala_or_ilu_apply_t(len(pntr) - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)), P(c64(ilu)), P(x))
@@ -225,8 +235,10 @@ def gmres(pntr, indx, vals, b, x0=None, tol=0.0, restart=30, max_outer=2**31 - 1
     return tot, x, hist[: nh.value].copy()
 
 
-def gmres_ir(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=50, fp32=True):
-    """Iterative refinement around ILU(0)-GMRES; fp32 selects the FP32 product and ILU inside."""
+def gmres_ir(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=50, fp32=True,
+             jacobi=0):
+    """Iterative refinement around ILU(0)-GMRES; fp32 selects the FP32 product and ILU inside;
+    jacobi > 0 applies that FP32 ILU by Jacobi sweeps."""
     n = len(pntr) - 1
     x = np.zeros(n)
     hist = np.zeros(max_refine + 2)
@@ -235,7 +247,7 @@ def gmres_ir(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restar
     i32_ = np.ascontiguousarray(ilu64, np.float32) if fp32 else None
     tot = lib().hala_or_gmres_ir(n, P(ci32(pntr)), P(ci32(indx)), P(c64(vals)), P(v32), P(ci32(dg)), P(c64(ilu64)),
                                  P(i32_), P(c64(b)), P(x), rtol, inner_rtol, restart, max_refine, P(hist), hist.size,
-                                 C.byref(nh))
+                                 C.byref(nh), int(jacobi))
     return tot, x, hist[: nh.value].copy()
 
 

@@ -253,6 +253,49 @@ void hala_or_ilu_apply_f32(int n, const int *pntr, const int *indx, const int *d
 }
 
 /*
+ * x <- (LU)^-1 x with the FP32 factor applied by k >= 1 Jacobi sweeps per
+ * triangle: a method of this build for the FP32 inner solve of iterative
+ * refinement, not a HALA operation.  L (unit): y0 = x,
+ *   y(m+1)_i = x_i - l_ij y(m)_j - ...   (j < i, one term at a time in row order);
+ * U: z0_i = y_i / u_ii,
+ *   z(m+1)_i = (y_i - u_ij z(m)_j - ...) / u_ii   (j > i, row order),
+ * with y = y(k).  Every row applies its terms in the exact sweep's order, so
+ * with k at least the level count of a triangle minus one the result equals
+ * hala_or_ilu_apply_f32 bit for bit.  w: 3n floats of scratch.
+ */
+void hala_or_ilu_apply_f32_jacobi(int n, const int *pntr, const int *indx, const int *diag, const float *ilu, int k,
+                                  float *x, float *w) {
+    float *ya = w, *yb = w + n, *zb = w + 2 * (size_t)n;
+    const float *yo = x; /* y0 = x */
+    float *yn = ya;
+    for (int m = 0; m < k; m++) {
+        for (int i = 0; i < n; i++) {
+            float acc = x[i];
+            for (int j = pntr[i]; j < diag[i]; j++) acc = acc - ilu[j] * yo[indx[j]];
+            yn[i] = acc;
+        }
+        yo = yn;
+        yn = (yn == ya) ? yb : ya;
+    }
+    /* y(k) is in yo; z0 into x (x no longer needed), then alternate x / zb */
+    const float *y = yo;
+    float *zo = (yo == ya) ? yb : ya; /* the free y buffer holds z0 */
+    for (int i = 0; i < n; i++) zo[i] = y[i] / ilu[diag[i]];
+    float *zn = zb;
+    for (int m = 0; m < k; m++) {
+        for (int i = 0; i < n; i++) {
+            float acc = y[i];
+            for (int j = diag[i] + 1; j < pntr[i + 1]; j++) acc = acc - ilu[j] * zo[indx[j]];
+            zn[i] = acc / ilu[diag[i]];
+        }
+        float *t = zo;
+        zo = zn;
+        zn = t;
+    }
+    memcpy(x, zo, (size_t)n * sizeof(float));
+}
+
+/*
  * x <- (LU)^-T x = L^-T U^-T x (the adjoint preconditioner; HALA has no
  * transposed ILU apply).  U^T w = x column by column: w_i = x_i / U_ii, then
  * x_k -= U_ik w_i for k > i; L^T y = w from the last row up: x_k -= L_ik x_i
@@ -471,6 +514,8 @@ typedef struct {
     const float *ilu32;
     int prec; /* IDENTITY or ILU */
     float *w32a, *w32b;
+    int jacobi;  /* FP32 ILU: k > 0 Jacobi sweeps per triangle instead of the exact sweeps */
+    float *wj;   /* 3n floats for the Jacobi apply */
 } gm_op_t;
 
 /* t = alpha*A*x + beta*t in FP64, or the FP32 product converted back. */
@@ -494,7 +539,10 @@ static void gm_prec(gm_op_t *o, const double *t, double *r) {
         hala_or_ilu_apply(n, o->pntr, o->indx, o->diag, o->ilu, r);
     } else {
         for (int i = 0; i < n; i++) o->w32a[i] = (float)t[i];
-        hala_or_ilu_apply_f32(n, o->pntr, o->indx, o->diag, o->ilu32, o->w32a);
+        if (o->jacobi > 0)
+            hala_or_ilu_apply_f32_jacobi(n, o->pntr, o->indx, o->diag, o->ilu32, o->jacobi, o->w32a, o->wj);
+        else
+            hala_or_ilu_apply_f32(n, o->pntr, o->indx, o->diag, o->ilu32, o->w32a);
         for (int i = 0; i < n; i++) r[i] = (double)o->w32a[i];
     }
 }
@@ -598,7 +646,7 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
 int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,
                   const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
                   double *hist, int max_hist, int *nhist) {
-    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0};
+    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0, 0, 0};
     if (nhist) *nhist = 0;
     return gmres_core(&o, b, x, tol, 0.0, 0, restart, max_outer, hist, max_hist, nhist);
 }
@@ -617,13 +665,15 @@ int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, i
 int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
                      const int *diag, const double *ilu, const float *ilu32, const double *b, double *x,
                      double rtol, double inner_rtol, int restart, int max_refine, double *hist, int max_hist,
-                     int *nhist) {
+                     int *nhist, int jacobi) {
     const int inner_cycles = 20;
     float *w32a = (float *)malloc((size_t)n * sizeof(float));
     float *w32b = (float *)malloc((size_t)n * sizeof(float));
+    float *wj = (float *)malloc((size_t)3 * n * sizeof(float));
     double *r = (double *)malloc((size_t)n * sizeof(double));
     double *d = (double *)malloc((size_t)n * sizeof(double));
-    gm_op_t o = {n, pntr, indx, diag, vals, vals32, ilu, vals32 ? ilu32 : 0, HALA_OR_PREC_ILU, w32a, w32b};
+    gm_op_t o = {n, pntr, indx, diag, vals, vals32, ilu, vals32 ? ilu32 : 0, HALA_OR_PREC_ILU, w32a, w32b,
+                 vals32 ? jacobi : 0, wj};
     double bn = hala_or_cnrm2(n, b);
     int total = 0, nh = 0;
     for (int it = 0; it <= max_refine; it++) {
@@ -640,6 +690,7 @@ int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals
     if (nhist) *nhist = nh;
     free(w32a);
     free(w32b);
+    free(wj);
     free(r);
     free(d);
     return total;

@@ -312,6 +312,50 @@ def test_refinement_reaches_fp64_accuracy(fp32):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
 
 
+def _lu_dense(p, i, f, n):
+    a = dense(p, i, f, n)
+    return np.tril(a, -1), np.triu(a, 1), np.diag(a).copy()
+
+
+@pytest.mark.parametrize("case", ["lap2d_12x9", "cd27_8"])
+def test_jacobi_ilu_apply_pins(case):
+    """FP32 ILU applied by k Jacobi sweeps per triangle: equals the exact FP32 apply once k reaches
+    the level depth (same per-row term order); one sweep matches the dense algebra
+    y = x - (L-I)x, z = (y - (U-D) y/d) / d; the error falls with every sweep."""
+    p, i, v = st.laplace2d(12, 9) if case == "lap2d_12x9" else st.convdiff27(8, seed=3)
+    n = len(p) - 1
+    d, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, d)
+    f32 = f.astype(np.float32)
+    x = st.uniform(9, n).astype(np.float32)
+    exact = oracle.ilu_apply_f32(p, i, d, f32, x)
+    nl = max(len(oracle.levels(p, i, d, up)[1]) - 1 for up in (False, True))
+    assert np.array_equal(oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, nl).view(np.int32), exact.view(np.int32))
+    lo, up, dg = _lu_dense(p, i, f32.astype(np.float64), n)
+    x64 = x.astype(np.float64)
+    y = x64 - lo @ x64
+    z = (y - up @ (y / dg)) / dg
+    one = oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, 1).astype(np.float64)
+    assert np.linalg.norm(one - z) <= 1e-5 * np.linalg.norm(z)
+    errs = [np.linalg.norm(oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, k).astype(np.float64) - exact)
+            for k in range(1, 7)]
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+
+
+def test_refinement_with_jacobi_inner_reaches_fp64_accuracy():
+    p, i, v = st.convdiff27(10, seed=3)
+    n = len(p) - 1
+    b = st.uniform(8, n)
+    d, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, d)
+    tot_exact, _, _ = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=True)
+    tot, x, hist = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=True, jacobi=3)
+    assert hist[-1] < 1e-10
+    assert tot >= tot_exact  # an inexact preconditioner needs at least as many inner steps
+    xs = np.linalg.solve(dense(p, i, v, n), b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
 def test_adjoint_gradient_finite_differences():
     m = 4
     d = st.diffusion3d(m, seed=5)

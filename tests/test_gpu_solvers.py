@@ -115,6 +115,64 @@ def test_gmres_ir_bitexact(engine, fp32):
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, -1], ids=["k1", "k2", "k3", "k_levels"])
+def test_ilu_jacobi_apply_bitexact(engine, k):
+    """FP32 ILU applied by Jacobi sweeps (option ilu32_jacobi) matches the oracle bit for bit; with as
+    many sweeps as levels it is the exact FP32 apply."""
+    saved = engine.get_option("ilu32_jacobi")
+    try:
+        for gen in (lambda: st.laplace3d(24, 24, 24), lambda: st.convdiff27(16, seed=3)):
+            p, i, v = gen()
+            n = len(p) - 1
+            dg, _ = oracle.diag(p, i)
+            f32 = oracle.ilu0(p, i, v, dg).astype(np.float32)
+            kk = k if k > 0 else max(len(oracle.levels(p, i, dg, up)[1]) - 1 for up in (False, True))
+            M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
+            x = st.uniform(61, n)
+            gr = engine.vector(n)
+            engine.set_option("ilu32_jacobi", kk)
+            for _ in range(2):
+                M.apply_f32x64(dev(engine, x), gr)
+                ref = oracle.ilu_apply_f32_jacobi(p, i, dg, f32, x.astype(np.float32), kk).astype(np.float64)
+                assert np.array_equal(bits(host(gr)), bits(ref))
+            if k < 0:
+                engine.set_option("ilu32_jacobi", 0)
+                g2 = engine.vector(n)
+                M.apply_f32x64(dev(engine, x), g2)
+                assert np.array_equal(bits(host(g2)), bits(host(gr)))
+    finally:
+        engine.set_option("ilu32_jacobi", saved)
+
+
+def test_gmres_ir_jacobi_inner_bitexact(engine):
+    import torch
+
+    p, i, v = st.convdiff27(16, seed=3)
+    n = len(p) - 1
+    b = st.uniform(44, n)
+    dg, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, dg)
+    tot, x, hist = oracle.gmres_ir(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40,
+                                   fp32=True, jacobi=3)
+    saved = engine.get_option("ilu32_jacobi")
+    try:
+        engine.set_option("ilu32_jacobi", 3)
+        gp, gi, gv = upload(engine, p, i, v)
+        ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
+        A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+        gx = engine.vector(n)
+        gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
+        gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv.float(), ilu, dev(engine, b), gx, rtol=1e-10,
+                                     inner_rtol=1e-4, restart=30, max_refine=40, hist=gh)
+    finally:
+        engine.set_option("ilu32_jacobi", saved)
+    assert gtot == tot and gnh == len(hist)
+    assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    r = b - oracle.spmv("N", n, n, 1.0, p, i, v, host(gx), 0.0, np.zeros(n))
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+
+
 # ---------------------------------------------------------------- batched
 def _batch(engine, nsys, seed, lo, hi):
     d = st.batch_laplace(nsys, seed=seed, side_lo=lo, side_hi=hi)
