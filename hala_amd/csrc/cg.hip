@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void cg_start_kernel(int n, const double *__re
 
 // Ap = A p (beta 0) fused with <p, Ap>; the last block forms alpha = zr / <p,Ap>.
 // One block = 256 rows = one canonical partial.
-template <bool DUMMY>
+template <bool NT>
 __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restrict__ pntr,
                                                       const int *__restrict__ indx, const double *__restrict__ vals,
                                                       const double *__restrict__ p, double *__restrict__ ap,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
     const int nb = (n + SPMV_ROWS - 1) / SPMV_ROWS;
     // grid-stride over 256-row blocks: one arrival (atomic) per workgroup, not per row block
     for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-        double acc = csr_block_rowsum<double, double, double>(n, pntr, indx, vals, p, prod, blk);
+        double acc = csr_block_rowsum<double, double, double, false, NT>(n, pntr, indx, vals, p, prod, blk);
         int i = blk * SPMV_ROWS + threadIdx.x;
         double v = 0.0;
         if (i < n) {
@@ -269,6 +269,7 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
 
     const int gs = (int)((n + SPMV_ROWS - 1) / SPMV_ROWS);
     const int gsd = std::min(gs, c->num_cus * c->opt.spmv_blocks_per_cu);  // grid-stride SpMV+dot
+    const bool nt = stream_nt<double>(P.A->nnz);
     const int ge = (int)((nb + EK - 1) / EK);
     const int gp = grid_for(n, 256 * 4, 16384);
     cg_init_kernel<<<1, 1, 0, c->stream>>>(st, tol, max_iter, hist, max_hist);
@@ -291,7 +292,10 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     HALA_CHECK_LAUNCH();
 
     auto one_pass = [&]() {
-        cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
+        if (nt)
+            cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
+        else
+            cg_spmv_kernel<false><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
         switch (P.prec) {
         case PREC_ID:
             cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);

@@ -75,19 +75,21 @@ int validate_pattern(hala_ctx c, int nrows, int ncols, int64_t nnz, const int *p
 // ---------------------------------------------------------------------------
 // y = alpha*sum + beta*y (beta == 0: y not read).  f32_inner: the FP32 sum is
 // first normalised as 1*sum + 0 in FP32, then combined in the output type.
-template <class TV, class TX, class TY, class TA>
+template <class TV, class TX, class TY, class TA, bool NT>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int nrows, const int *__restrict__ pntr,
                                                         const int *__restrict__ indx, const TV *__restrict__ vals,
                                                         const TX *__restrict__ x, TY alpha, TY beta,
                                                         TY *__restrict__ y, int f32_inner) {
     __shared__ TA prod[spmv_tile<TA>::value];
-    TA acc = csr_block_rowsum<TV, TX, TA>(nrows, pntr, indx, vals, x, prod);
+    TA acc = csr_block_rowsum<TV, TX, TA, false, NT>(nrows, pntr, indx, vals, x, prod, blockIdx.x);
     const int row = blockIdx.x * SPMV_ROWS + threadIdx.x;
     if (row >= nrows) return;
     TY s;
     if (f32_inner) s = (TY)((TA)1 * acc + (TA)0);
     else s = (TY)acc;
-    y[row] = (beta == TY(0)) ? alpha * s + TY(0) : alpha * s + beta * y[row];
+    const TY out = (beta == TY(0)) ? alpha * s + TY(0) : alpha * s + beta * y[row];
+    if (NT) __builtin_nontemporal_store(out, y + row);
+    else y[row] = out;
 }
 
 // One wavefront per row for long rows: lanes stride the row, butterfly sum.
@@ -135,8 +137,12 @@ void spmv_axpby(hala_csr A, const TV *vals, const TX *x, TY alpha, TY beta, TY *
                                                                   y, f32_inner ? 1 : 0);
     } else {
         int g = (A->nrows + SPMV_ROWS - 1) / SPMV_ROWS;
-        spmv_rows_kernel<TV, TX, TY, TA><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals, x, alpha,
-                                                                       beta, y, f32_inner ? 1 : 0);
+        if (stream_nt<TV>(A->nnz))
+            spmv_rows_kernel<TV, TX, TY, TA, true><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals, x,
+                                                                                 alpha, beta, y, f32_inner ? 1 : 0);
+        else
+            spmv_rows_kernel<TV, TX, TY, TA, false><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals,
+                                                                                  x, alpha, beta, y, f32_inner ? 1 : 0);
     }
     HALA_CHECK_LAUNCH();
 }
