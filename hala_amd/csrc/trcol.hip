@@ -30,9 +30,11 @@ constexpr int CPMAX_LONG = 256;  // the same for long-row records (register budg
 constexpr int STEP_BYTES = 20 * 1024;  // automatic column count: record bytes one level of a column reads at most
 constexpr int ELMAX = 8;     // external values per loader lane at most (512 per cell)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
-#ifndef COL_PE
-#define COL_PE 1  // levels an external value is polled ahead of its use
-#endif
+// Levels an external value is polled ahead of its use: 2 for short-row
+// operators (C2 apply 0.577 vs 0.596 ms), 1 for long rows (whose columns
+// already lag their producers by several levels: 1.33 vs 1.54 ms at 2).
+constexpr int PE_MAX = 2;
+constexpr int poll_ahead(int maxd) { return maxd == 4 ? 2 : 1; }
 constexpr int64_t LDS_BUDGET = 64 * 1024;
 
 __device__ __forceinline__ int colof(int r, int n, int ncol, int backward) {
@@ -287,11 +289,11 @@ tri_colplan &tri_colplan_reset(tri_colplan &q) {
 }
 
 // window (K levels x CP rows) + external double buffer (2 x 64 EL values) +
-// slots of the polls in flight (COL_PE x 64 EL) + cell records
+// slots of the polls in flight (up to PE_MAX x 64 EL) + cell records
 int64_t lds_bytes(int K, int CP, int EL, int nlevcol, size_t tsize) {
     const int pad = nlevcol + 3 * 8 + 2;  // read-ahead of the deepest pipeline (P = 8)
     const int64_t xc = (int64_t)WAVE * EL;
-    return (int64_t)K * CP * (int64_t)tsize + 2 * xc * (int64_t)tsize + COL_PE * xc * 4 + (int64_t)pad * CI * 4;
+    return (int64_t)K * CP * (int64_t)tsize + 2 * xc * (int64_t)tsize + PE_MAX * xc * 4 + (int64_t)pad * CI * 4;
 }
 
 // bytes of one packed row record: int16 codes [LR] | diagonal | values [LR], 16-byte multiple
@@ -648,11 +650,28 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 const int se = s % PE;  // static once unrolled
                 // externals of level lam + 1 into LDS
                 T *dst = extb + ((lam + 1) & 1) * XC + l;
+                bool slow = false;
 #pragma unroll
                 for (int e = 0; e < EL; e++) {
                     T v = ev[se][e];
-                    if (has(lam + 1, e) && is_sent(v)) v = poll_slow(at(eslot[se * XC + l + WAVE * e]), a.err);
+                    if (has(lam + 1, e) && is_sent(v)) {
+                        v = poll_slow(at(eslot[se * XC + l + WAVE * e]), a.err);
+                        slow = true;
+                    }
                     dst[WAVE * e] = v;
+                }
+                // A miss means this column has caught up with a producer: every poll
+                // already in flight was issued too early as well.  Waiting for the next
+                // level's values too puts the column PE levels behind again, so the
+                // polls issued from here on find their values written.
+                if (PE > 1 && __ballot(slow) != 0) {
+                    const int sn = (se + 1) % PE;
+#pragma unroll
+                    for (int e = 0; e < EL; e++) {
+                        T v = ev[sn][e];
+                        if (has(lam + 2, e) && is_sent(v)) v = poll_slow(at(eslot[sn * XC + l + WAVE * e]), a.err);
+                        ev[sn][e] = v;
+                    }
                 }
                 // polls of level lam + PE + 1, slots of level lam + PE + P + 1
 #pragma unroll
@@ -729,7 +748,8 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
 
 template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int EL>
 void launch_col_el(hala_ctx c, const tri_colplan &q, col_args<T, TOUT> &a, size_t lds) {
-    constexpr int PE = COL_PE < P ? COL_PE : P;
+    constexpr int PE = poll_ahead(MAXD);
+    static_assert(PE <= PE_MAX, "LDS sized for PE_MAX polls in flight");
     constexpr int BT = (MAXD == 4 ? CPMAX : CPMAX_LONG) + WAVE;
     tri_col_kernel<T, TOUT, MODE, UNIT, MAXD, P, PE, EL, BT><<<dim3(q.ncol), dim3(q.CP + WAVE), lds, c->stream>>>(a);
 }
