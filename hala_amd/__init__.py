@@ -17,7 +17,7 @@ from . import stencils  # noqa: F401
 __all__ = [
     "HalaError", "gpu_engine", "stop_criteria", "make_sparse_matrix", "sparse_gemv", "make_ilu",
     "make_triangular_matrix", "solve_cg", "solve_cg_ilu", "solve_gmres", "solve_gmres_ilu",
-    "solve_gmres_ir", "dot", "norm2", "axpy", "scal", "vcopy", "stencils",
+    "solve_gmres_ir", "solve_batch_cg", "solve_batch_cg_ilu", "dot", "norm2", "axpy", "scal", "vcopy", "stencils",
 ]
 
 
@@ -316,6 +316,31 @@ def solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x, hist=None, matrix=No
     if ilu is None:
         ilu = make_ilu(engine, pntr, indx, vals)
     return solve_cg(engine, stop, pntr, indx, vals, ilu, b, x, hist=hist, matrix=matrix)
+
+
+def solve_batch_cg(engine, stop, pntr, indx, vals, precon, B, X, hist=None, matrix=None):
+    """solve_batch_cg(engine, stop, pntr, indx, vals, precon, B, X): hex/solvers/hala_solvers_cg_batch.hpp:68-110.
+
+    B, X: n*nrhs entries, right-hand side k at [k n, (k+1) n) (a (nrhs, n) tensor works);
+    precon: None (identity copy) or a gpu_ilu.  Returns the iteration count.
+    """
+    stop = _stop(stop)
+    n = int(pntr.numel()) - 1
+    assert B.numel() % n == 0 and X.numel() == B.numel(), "B and X must hold n * nrhs entries"
+    A = matrix or gpu_sparse_matrix(engine, n, n, pntr, indx, vals)
+    it = C.c_int()
+    nh = int(hist.numel()) if hist is not None else 0
+    call("hala_cg_batch_f64", A.handle, _ptr(vals), 0 if precon is None else 1,
+         precon.handle if precon is not None else C.c_void_p(0), int(B.numel()) // n, _ptr(B), _ptr(X), stop.tol,
+         stop.max_iter, C.byref(it), _ptr(hist), nh)
+    return it.value
+
+
+def solve_batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X, hist=None, matrix=None):
+    """solve_batch_cg_ilu: hex/solvers/hala_solvers_cg_batch.hpp:156-249 (ILU built on the fly when None)."""
+    if ilu is None:
+        ilu = make_ilu(engine, pntr, indx, vals)
+    return solve_batch_cg(engine, stop, pntr, indx, vals, ilu, B, X, hist=hist, matrix=matrix)
 
 
 def solve_cg_t(engine, stop, matrix, vals, ilu, b, x, hist=None):

@@ -90,6 +90,7 @@ SIGNATURES = {
     "hala_trsv_f64": [vp, ch, f64, vp, vp, vp],
     "hala_cg_f64": [vp, vp, i32, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_cg_t_f64": [vp, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
+    "hala_cg_batch_f64": [vp, vp, i32, vp, i32, vp, vp, f64, i32, pi32, vp, i32],
     "hala_gmres_f64": [vp, vp, i32, vp, vp, vp, f64, i32, i32, pi32, vp, i32, pi32],
     "hala_gmres_ir_f64": [vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, pi32, vp, i32, pi32],
     "hala_assemble_f64": [vp, i64, vp, vp, vp, vp, vp],

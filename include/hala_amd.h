@@ -181,6 +181,15 @@ int hala_trsv_f64(hala_tri tri, char trans, double alpha, const double *vals, co
  * hist (device, nullable) receives norm2(r) per pass, at most max_hist. */
 int hala_cg_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag, const double *b,
                 double *x, double tol, int max_iter, int *iterations, double *hist, int max_hist);
+/* Multi-RHS PCG with one shared matrix, resident on the device: the
+   solve_batch_cg / solve_batch_cg_ilu of hex/solvers/hala_solvers_cg_batch.hpp:68-249
+   (solve_cg_core with the batch operations of wax/hala_blas_extensions.hpp).
+   B, X: n x nrhs column-major (column k at offset k n); X holds the initial
+   guess.  prec: 0 identity, 1 ILU (ilu).  Every scalar is per column; all
+   columns iterate until iterations == max_iter or max_k ||r_k|| < tol.
+   hist (nullable) receives max_k ||r_k|| per pass. */
+int hala_cg_batch_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, int nrhs, const double *B, double *X,
+                      double tol, int max_iter, int *iterations, double *hist, int max_hist);
 /* Adjoint CG: A^T y = b preconditioned with (LU)^-T. */
 int hala_cg_t_f64(hala_csr A, const double *vals, hala_ilu ilu, const double *b, double *x, double tol,
                   int max_iter, int *iterations, double *hist, int max_hist);

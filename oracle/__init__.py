@@ -43,6 +43,7 @@ _SIG = {
     "hala_or_trsv": (None, [ch, ch, ch, i32, f64, vp, vp, vp, vp, vp]),
     "hala_or_cg": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, f64, i32, vp, i32]),
     "hala_or_cg_team": (i32, [i32, vp, vp, vp, vp, vp, vp, f64, i32, i32, vp, i32]),
+    "hala_or_cg_batch": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, f64, i32, vp, i32]),
     "hala_or_gmres": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, f64, i32, i32, vp, i32, pi32]),
     "hala_or_gmres_ir": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, f64, f64, i32, i32, vp, i32, pi32, i32]),
     "hala_or_rotg": (None, [vp, vp, vp, vp]),
@@ -212,6 +213,22 @@ def cg(pntr, indx, vals, b, x0=None, tol=0.0, max_iter=2**31 - 1, prec=PREC_IDEN
                           P(c64(mdata)) if mdata is not None else None, P(c64(b)), P(x), tol, max_iter, P(hist),
                           max_hist)
     return it, x, hist[: max(it - 1, 0)].copy()
+
+
+def cg_batch(pntr, indx, vals, B, tol=0.0, max_iter=2**31 - 1, prec=PREC_IDENTITY, mdiag=None, mdata=None,
+             max_hist=100000):
+    """Multi-RHS PCG with a shared matrix (hala_or_cg_batch); B is (nrhs, n): row k is right-hand side k."""
+    B = np.ascontiguousarray(B, np.float64)
+    nrhs, n = B.shape
+    X = np.zeros_like(B)
+    hist = np.zeros(max_hist)
+    ilu_ = prec in (PREC_ILU,)
+    it = lib().hala_or_cg_batch(n, P(ci32(pntr)), P(ci32(indx)), P(c64(vals)), prec,
+                                P(ci32(pntr)) if ilu_ else None, P(ci32(indx)) if ilu_ else None,
+                                P(ci32(mdiag)) if mdiag is not None else None,
+                                P(c64(mdata)) if mdata is not None else None, nrhs, P(B), P(X), tol, max_iter,
+                                P(hist), max_hist)
+    return it, X, hist[: max(it - 1, 0)].copy()
 
 
 def cg_team(pntr, indx, vals, dvals, b, tol, team, max_iter=2**31 - 1, max_hist=100000):

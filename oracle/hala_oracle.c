@@ -440,6 +440,71 @@ int hala_or_cg(int n, const int *pntr, const int *indx, const double *vals, int 
     return cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, dot_canon, 0, hist, max_hist);
 }
 
+/*
+ * Multi-RHS PCG with one shared matrix: HALA's solve_batch_cg
+ * (hex/solvers/hala_solvers_cg_batch.hpp:68-110), i.e. solve_cg_core with the
+ * batch operations of wax/hala_blas_extensions.hpp.  Column k of B, X is
+ * entries [k n, (k+1) n); every scalar becomes a vector over the columns:
+ *   zr_k = <r_k, z_k>; alpha_k = zr_k / <p_k, Ap_k> (vdivide);
+ *   x_k += (1.0 alpha_k) p_k, r_k += (-1.0 alpha_k) Ap_k (batch_axpy, :51-66);
+ *   stop when iterations == max_iter or max_k ||r_k|| < tol (batch_max_norm2,
+ *   :354-365: the largest <r_k, r_k>, then sqrt);
+ *   beta_k = <r_k, z_k>new / zr_k; p_k = beta_k p_k (batch_scal), p += z (axpy).
+ * All columns iterate until the stop.  hist receives max_k ||r_k|| per pass.
+ */
+int hala_or_cg_batch(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *mpntr,
+                     const int *mindx, const int *mdiag, const double *mdata, int nrhs, const double *B, double *X,
+                     double tol, int max_iter, double *hist, int max_hist) {
+    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata};
+    const size_t N = (size_t)n * nrhs;
+    double *r = (double *)malloc(N * sizeof(double));
+    double *p = (double *)malloc(N * sizeof(double));
+    double *ap = (double *)malloc(N * sizeof(double));
+    double *z = (double *)malloc(N * sizeof(double));
+    double *zr = (double *)malloc((size_t)nrhs * sizeof(double));
+    double *a = (double *)malloc((size_t)nrhs * sizeof(double));
+    memcpy(r, B, N * sizeof(double));
+    for (int k = 0; k < nrhs; k++)
+        hala_or_spmv('N', n, n, 1.0, pntr, indx, vals, X + (size_t)k * n, 0.0, p + (size_t)k * n);
+    axpy((int)N, -1.0, p, r);
+    for (int k = 0; k < nrhs; k++) prec_apply(&m, r + (size_t)k * n, z + (size_t)k * n);
+    memcpy(p, z, N * sizeof(double));
+    for (int k = 0; k < nrhs; k++) zr[k] = hala_or_cdot(n, r + (size_t)k * n, z + (size_t)k * n);
+    int it = 1;
+    for (;;) {
+        for (int k = 0; k < nrhs; k++)
+            hala_or_spmv('N', n, n, 1.0, pntr, indx, vals, p + (size_t)k * n, 0.0, ap + (size_t)k * n);
+        it++;
+        double mx = 0.0;
+        for (int k = 0; k < nrhs; k++) {
+            const size_t o = (size_t)k * n;
+            a[k] = zr[k] / hala_or_cdot(n, p + o, ap + o);
+            axpy(n, 1.0 * a[k], p + o, X + o);
+            axpy(n, -1.0 * a[k], ap + o, r + o);
+            double d = hala_or_cdot(n, r + o, r + o);
+            if (k == 0 || d > mx) mx = d;
+        }
+        double rn = sqrt(mx);
+        if (hist && it - 2 < max_hist) hist[it - 2] = rn;
+        if (it == max_iter || rn < tol) break;
+        for (int k = 0; k < nrhs; k++) prec_apply(&m, r + (size_t)k * n, z + (size_t)k * n);
+        for (int k = 0; k < nrhs; k++) {
+            const size_t o = (size_t)k * n;
+            double nzr = hala_or_cdot(n, r + o, z + o);
+            scal(n, nzr / zr[k], p + o);
+            zr[k] = nzr;
+        }
+        axpy((int)N, 1.0, z, p);
+    }
+    free(r);
+    free(p);
+    free(ap);
+    free(z);
+    free(zr);
+    free(a);
+    return it;
+}
+
 /* Jacobi-PCG of one batched system, reductions in team order. */
 int hala_or_cg_team(int n, const int *pntr, const int *indx, const double *vals, const double *dvals,
                     const double *b, double *x, double tol, int max_iter, int team, double *hist, int max_hist) {

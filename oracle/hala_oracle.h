@@ -61,6 +61,9 @@ enum { HALA_OR_PREC_IDENTITY = 0, HALA_OR_PREC_ILU = 1, HALA_OR_PREC_JACOBI = 2,
 int hala_or_cg(int n, const int *pntr, const int *indx, const double *vals, int prec,
                const int *mpntr, const int *mindx, const int *mdiag, const double *mdata,
                const double *b, double *x, double tol, int max_iter, double *hist, int max_hist);
+int hala_or_cg_batch(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *mpntr,
+                     const int *mindx, const int *mdiag, const double *mdata, int nrhs, const double *B, double *X,
+                     double tol, int max_iter, double *hist, int max_hist);
 int hala_or_cg_team(int n, const int *pntr, const int *indx, const double *vals, const double *dvals,
                     const double *b, double *x, double tol, int max_iter, int team, double *hist, int max_hist);
 int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,

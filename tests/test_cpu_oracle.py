@@ -356,6 +356,30 @@ def test_refinement_with_jacobi_inner_reaches_fp64_accuracy():
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
 
 
+def test_batch_cg_oracle_pins():
+    """Multi-RHS PCG: one column is the single-RHS PCG bit for bit (same loop); with several columns,
+    a column that needs the full iteration count also matches its single solve; every column solves."""
+    p, i, v = st.laplace3d(9, 8, 7)
+    n = len(p) - 1
+    d, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, d)
+    B = np.stack([st.uniform(90 + k, n) for k in range(3)])
+    tol = 1e-10 * max(np.linalg.norm(b) for b in B)
+    it1, X1, h1 = oracle.cg_batch(p, i, v, B[:1], tol=tol, prec=oracle.PREC_ILU, mdiag=d, mdata=f)
+    it0, x0, h0 = oracle.cg(p, i, v, B[0], tol=tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=d, mdata=f)
+    assert it1 == it0 and np.array_equal(bits(X1[0]), bits(x0)) and np.array_equal(bits(h1), bits(h0))
+    it, X, h = oracle.cg_batch(p, i, v, B, tol=tol, prec=oracle.PREC_ILU, mdiag=d, mdata=f)
+    singles = [oracle.cg(p, i, v, B[k], tol=tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=d, mdata=f)
+               for k in range(3)]
+    assert it == max(s[0] for s in singles)
+    for k in range(3):
+        if singles[k][0] == it:
+            assert np.array_equal(bits(X[k]), bits(singles[k][1]))
+    a = dense(p, i, v, n)
+    for k in range(3):
+        assert np.linalg.norm(np.linalg.solve(a, B[k]) - X[k]) < 1e-8 * np.linalg.norm(X[k])
+
+
 def test_adjoint_gradient_finite_differences():
     m = 4
     d = st.diffusion3d(m, seed=5)

@@ -173,6 +173,36 @@ def test_gmres_ir_jacobi_inner_bitexact(engine):
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["ilu_3rhs", "identity_2rhs", "ilu_9rhs"])
+def test_batch_cg_shared_matrix_bitexact(engine, case):
+    """Multi-RHS PCG with one shared matrix (solve_batch_cg): solutions, iteration count and the
+    max-norm history match the oracle bit for bit; 9 columns span two product launches."""
+    import torch
+
+    nrhs = {"ilu_3rhs": 3, "identity_2rhs": 2, "ilu_9rhs": 9}[case]
+    ilu_ = case != "identity_2rhs"
+    p, i, v = st.laplace3d(14, 13, 12)
+    n = len(p) - 1
+    B = np.stack([st.uniform(80 + k, n) for k in range(nrhs)])
+    tol = 1e-10 * max(np.linalg.norm(b) for b in B)
+    dg, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, dg)
+    it, X, hist = oracle.cg_batch(p, i, v, B, tol=tol, prec=oracle.PREC_ILU if ilu_ else oracle.PREC_IDENTITY,
+                                  mdiag=dg, mdata=f)
+    gp, gi, gv = upload(engine, p, i, v)
+    M = H.make_ilu(engine, gp, gi, gv) if ilu_ else None
+    gB = dev(engine, B.ravel())
+    gX = engine.vector(n * nrhs)
+    gh = torch.zeros(512, dtype=torch.float64, device=gX.device)
+    git = H.solve_batch_cg(engine, tol, gp, gi, gv, M, gB, gX, hist=gh)
+    assert git == it
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
+    assert np.array_equal(bits(host(gX).reshape(nrhs, n)), bits(X))
+    for k in range(nrhs):
+        r = B[k] - oracle.spmv("N", n, n, 1.0, p, i, v, X[k], 0.0, np.zeros(n))
+        assert np.linalg.norm(r) < 10 * tol
+
+
 # ---------------------------------------------------------------- batched
 def _batch(engine, nsys, seed, lo, hi):
     d = st.batch_laplace(nsys, seed=seed, side_lo=lo, side_hi=hi)
