@@ -540,9 +540,13 @@ class gpu_ilu {
     }
     template <class VX, class VR>
     void apply(VX const &x, VR &&r, int num_rhs = 1) const {
-        runtime_assert(num_rhs == 1, "hala_amd: one right-hand side per ILU apply");
-        if ((int)r.size() < n) r.resize(n);
-        check_hala(hala_ilu_apply_f64(ilu.get(), (double const *)x.data(), (double *)r.data()), "gpu_ilu::apply()");
+        // num_rhs columns of n entries, column-major (gpu/hala_gpu_ilu.hpp:140-195)
+        runtime_assert(num_rhs >= 1 && x.size() >= (size_t)n * num_rhs, "gpu_ilu::apply(): x too small");
+        if (r.size() < (size_t)n * num_rhs) r.resize((size_t)n * num_rhs);
+        for (int k = 0; k < num_rhs; k++)
+            check_hala(hala_ilu_apply_f64(ilu.get(), (double const *)x.data() + (size_t)k * n,
+                                          (double *)r.data() + (size_t)k * n),
+                       "gpu_ilu::apply()");
     }
     gpu_engine const &engine() const { return rengine; }
     hala_ilu handle() const { return ilu.get(); }

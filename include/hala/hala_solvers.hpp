@@ -100,6 +100,88 @@ int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP
     return solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x);
 }
 
+// ---- multi-RHS CG with one shared matrix -----------------------------------
+// B, X hold nrhs right-hand sides / solutions of n entries each, column-major
+// (column k at offset k n), as the reference's sparse_gemm.
+template <class V>
+gpu_wrapped_array<double> batch_column(gpu_engine const &engine, V &v, int n, int k) {
+    return wrap_gpu_array((double *)v.data() + (size_t)k * n, (size_t)n, engine.device());
+}
+
+// solve_batch_cg(engine, stop, pntr, indx, vals, precon, B, X):
+// hex/solvers/hala_solvers_cg_batch.hpp:68-136 (solve_cg_core with batch_dot,
+// vdivide, batch_axpy, batch_scal and batch_max_norm2 of
+// wax/hala_blas_extensions.hpp).  The preconditioner receives the whole batch.
+// Host loop over the engine's per-column BLAS, the reference's step order.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                   VV const &vals, preconditioner<double> precon, VB const &B, VX &X) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg: size mismatch");
+    const int nrhs = (int)(B.size() / (size_t)n);
+    prepare_x(engine, (int)B.size(), X);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    gpu_vector<double> r(B.size(), engine.device()), p(B.size(), engine.device()), ap(B.size(), engine.device()),
+        z(B.size(), engine.device());
+    auto col = [&](auto &v, int k) { return batch_column(engine, v, n, k); };
+    vcopy(engine, B, r);
+    for (int k = 0; k < nrhs; k++) matrix.gemv('N', 1.0, col(X, k), 0.0, col(p, k));
+    axpy(engine, -1.0, p, r);
+    precon(r, z);
+    vcopy(engine, z, p);
+    std::vector<double> zr(nrhs), a(nrhs);
+    for (int k = 0; k < nrhs; k++) zr[k] = dot(engine, col(r, k), col(z, k));
+    int iterations = 1;
+    for (;;) {
+        for (int k = 0; k < nrhs; k++) matrix.gemv('N', 1.0, col(p, k), 0.0, col(ap, k));
+        iterations++;
+        for (int k = 0; k < nrhs; k++) a[k] = zr[k] / dot(engine, col(p, k), col(ap, k));
+        for (int k = 0; k < nrhs; k++) axpy(engine, 1.0 * a[k], col(p, k), col(X, k));
+        for (int k = 0; k < nrhs; k++) axpy(engine, -1.0 * a[k], col(ap, k), col(r, k));
+        double mx = 0.0;  // batch_max_norm2: the largest <r_k, r_k>, then its square root
+        for (int k = 0; k < nrhs; k++) {
+            const double d = dot(engine, col(r, k), col(r, k));
+            if (k == 0 || d > mx) mx = d;
+        }
+        if (iterations == stop.max_iter || std::sqrt(mx) < stop.tol) break;
+        precon(r, z);
+        for (int k = 0; k < nrhs; k++) {
+            const double nzr = dot(engine, col(r, k), col(z, k));
+            scal(engine, nzr / zr[k], col(p, k));
+            zr[k] = nzr;
+        }
+        axpy(engine, 1.0, z, p);
+    }
+    return iterations;
+}
+
+// solve_batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X):
+// hex/solvers/hala_solvers_cg_batch.hpp:156-200.  Runs device-resident
+// (hala_cg_batch_f64), the same arithmetic as solve_batch_cg with the ILU
+// applied to every column.
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                       VV const &vals, gpu_ilu<double> const &ilu, VB const &B, VX &X) {
+    const int n = (int)pntr.size() - 1;
+    runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg_ilu: size mismatch");
+    prepare_x(engine, (int)B.size(), X);
+    auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
+    int it = 0;
+    check_hala(hala_cg_batch_f64(matrix.handle(), (double const *)vals.data(), 1, ilu.handle(),
+                                 (int)(B.size() / (size_t)n), (double const *)B.data(), (double *)X.data(), stop.tol,
+                                 stop.max_iter, &it, nullptr, 0),
+               "solve_batch_cg_ilu()");
+    return it;
+}
+
+// solve_batch_cg_ilu(engine, stop, pntr, indx, vals, B, X, policy): factor on the fly (:238-249).
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                       VV const &vals, VB const &B, VX &X, char policy = 'N') {
+    auto ilu = make_ilu(engine, pntr, indx, vals, policy);
+    return solve_batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X);
+}
+
 // Device-resident CG with the identity (z = r) or Jacobi (z = r ./ d) preconditioner.
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_cg_device(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,

@@ -142,6 +142,76 @@ int main() {
         CHECK(diffsum(x2, x) == 0.0);
         std::printf("PASS solve_cg\n");
     }
+    // tests/solvers_tests.hpp:56-104 (batch CG, three right-hand sides)
+    {
+        const int N = 5, nrhs = 3;
+        std::vector<int> pntr = {0, 2, 5, 8, 11, 13};
+        std::vector<int> indx = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4};
+        std::vector<double> vals = {2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0};
+        std::vector<double> xref(N * nrhs), b, x;
+        for (int i = 0; i < N * nrhs; i++) xref[i] = 1.0 + 0.5 * i - 0.125 * (i % 4);
+        auto batch_rhs = [&]() {
+            b.clear();
+            for (int k = 0; k < nrhs; k++) {
+                auto bk = densemul(pntr, indx, vals, std::vector<double>(xref.begin() + k * N, xref.begin() + (k + 1) * N));
+                b.insert(b.end(), bk.begin(), bk.end());
+            }
+        };
+        auto relerr = [&](std::vector<double> const &v) {
+            double e = 0, nr = 0;
+            for (int i = 0; i < N * nrhs; i++) {
+                e += (v[i] - xref[i]) * (v[i] - xref[i]);
+                nr += xref[i] * xref[i];
+            }
+            return std::sqrt(e / nr);
+        };
+        batch_rhs();
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gb = hala::make_gpu_vector(b);
+        auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+        hala::gpu_vector<double> gx;
+        // tri-diagonal: the ILU is the exact factorisation, convergence is instant
+        hala::solve_batch_cg(engine, 2, gp, gi, gv,
+                             [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                                 hala::ilu_apply(ilu, inx, outr, nrhs);
+                             },
+                             gb, gx);
+        gx.unload(x);
+        CHECK(relerr(x) < 1e-14);
+        pntr = {0, 3, 6, 9, 12, 15};
+        indx = {0, 1, 4, 0, 1, 2, 1, 2, 3, 2, 3, 4, 0, 3, 4};
+        vals = {2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 2.0};
+        batch_rhs();
+        gp = hala::make_gpu_vector(pntr);
+        gi = hala::make_gpu_vector(indx);
+        gv = hala::make_gpu_vector(vals);
+        gb = hala::make_gpu_vector(b);
+        auto ilu2 = hala::make_ilu(engine, gp, gi, gv, 'N');
+        gx = hala::gpu_vector<double>(N * nrhs, 0);
+        gx.fill(0.0);
+        int it = hala::solve_batch_cg_ilu(engine, {1.E-9, 6}, gp, gi, gv, ilu2, gb, gx);
+        gx.unload(x);
+        CHECK(it <= 6 && relerr(x) < 1e-8);
+        // the host loop with the same ILU applied column by column takes the same steps
+        hala::gpu_vector<double> gx2(N * nrhs, 0);
+        gx2.fill(0.0);
+        int it2 = hala::solve_batch_cg(engine, {1.E-9, 6}, gp, gi, gv,
+                                       [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                                           hala::ilu_apply(ilu2, inx, outr, nrhs);
+                                       },
+                                       gb, gx2);
+        std::vector<double> x2;
+        gx2.unload(x2);
+        CHECK(it2 == it && diffsum(x2, x) == 0.0);
+        hala::gpu_vector<double> gx3;
+        hala::solve_batch_cg_ilu(engine, 1.E-9, gp, gi, gv, gb, gx3);  // factorisation behind the scene
+        std::vector<double> x3;
+        gx3.unload(x3);
+        CHECK(relerr(x3) < 1e-8);
+        std::printf("PASS solve_batch_cg\n");
+    }
     // tests/solvers_tests.hpp:105-137 (GMRES)
     {
         std::vector<int> pntr = {0, 2, 5, 8, 11, 12};
