@@ -410,7 +410,8 @@ def main():
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_sweep_bytes"),
                      "bytes_per_launch": by_sweep, "avg_launch_ms": round(ms_sweep, 5),
                      "note": f"latency-bound: {ilu.info()['nlev_lower']} dependency levels per triangle, "
-                             f"{ilu.sweep_info(False)['ncol']} row-block columns"},
+                             f"{ilu.sweep_info(False)['ncol']} columns "
+                             f"({ilu.sweep_info(False)['split']} per slab of planes)"},
         "roofline_spmv": {"kernel": "spmv_rows_kernel<f64> (CSR, LDS-staged 256-row blocks)", "bound": "hbm",
                           "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic.get("spmv_bytes"),

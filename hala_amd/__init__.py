@@ -176,9 +176,10 @@ class gpu_ilu:
 
     def sweep_info(self, upper=False):
         """Layout of one triangular sweep: column-blocked or level pool, and its shape."""
-        o = (C.c_int * 6)()
+        o = (C.c_int * 8)()
         call("hala_ilu_sweep_info", self.handle, 1 if upper else 0, o)
-        return {"column": bool(o[0]), "ncol": o[1], "cells": o[2], "window": o[3], "max_ext": o[4], "why": o[5]}
+        return {"column": bool(o[0]), "ncol": o[1], "cells": o[2], "window": o[3], "max_ext": o[4], "why": o[5],
+                "split": o[6], "plane": o[7]}
 
     def apply(self, x, r, num_rhs=1):
         assert num_rhs == 1

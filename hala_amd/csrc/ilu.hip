@@ -177,6 +177,8 @@ int hala_ilu_sweep_info(hala_ilu M, int upper, int *out) {
         out[3] = q.K;
         out[4] = q.maxext;
         out[5] = q.why;
+        out[6] = q.split;
+        out[7] = q.plane;
     });
 }
 

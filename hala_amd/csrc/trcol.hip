@@ -1,8 +1,15 @@
 // trcol.hip -- column-blocked execution of the level-scheduled triangular sweeps.
 //
-// The rows are cut into `ncol` contiguous blocks ("columns"): every dependency
-// of a forward sweep points to a lower row and of a backward sweep to a higher
-// row, so it lies in the same column or in one handled before it.  One
+// The rows are cut into `ncol` blocks ("columns").  Plainly, contiguous row
+// ranges: every dependency of a forward sweep points to a lower row and of a
+// backward sweep to a higher row, so it lies in the same column or in one
+// handled before it.  For a stencil-like pattern whose rows come in "planes"
+// (P rows, the middle of the largest dependency offsets) a column is instead
+// one of B pieces of a slab of planes: the column a row's dependencies lie in
+// is then the same, the previous piece or the previous slab, and a sweep's
+// longest chain of columns shrinks from A*B to A+B-1 (each link costs a
+// global round trip before the next column can start).  Any assignment is
+// correct, since columns only exchange values through the polled output.  One
 // workgroup owns a column and walks the levels in order; the rows of its
 // column at one level (a "cell", a contiguous piece of that level's row list)
 // get one thread each.  A dependency produced by the same workgroup within the
@@ -28,6 +35,10 @@ constexpr int BTMAX = 1024;  // padding of slot-indexed arrays (idle-lane reads)
 constexpr int CPMAX = 512;   // rows of a cell at most (= compute threads of a column workgroup)
 constexpr int CPMAX_LONG = 256;  // the same for long-row records (register budget of their prefetch)
 constexpr int STEP_BYTES = 20 * 1024;  // automatic column count: record bytes one level of a column reads at most
+// the same for columns split across planes: they pass fewer round trips per
+// sweep, so narrower cells pay (128^3 7-point apply: 0.53 ms at 256 rows per
+// cell, 0.39 ms at 128, 0.41 ms at 64; contiguous columns 0.58 ms)
+constexpr int STEP_BYTES_SPLIT = 8 * 1024;
 constexpr int ELMAX = 8;     // external values per loader lane at most (512 per cell)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
 // Levels an external value is polled ahead of its use: 2 for short-row
@@ -37,36 +48,73 @@ constexpr int PE_MAX = 2;
 constexpr int poll_ahead(int maxd) { return maxd == 4 ? 2 : 1; }
 constexpr int64_t LDS_BUDGET = 64 * 1024;
 
-__device__ __forceinline__ int colof(int r, int n, int ncol, int backward) {
+// Column of every row.  Rows are taken in sweep order (r' = n-1-r backward).
+// B == 1: A contiguous blocks.  B > 1: slab a = plane * A / (n / P) of whole
+// planes of P rows, piece b = (r' mod P) * B / P, column a * B + b.
+__global__ void col_map(int n, int backward, int A, int B, int P, int *__restrict__ cmap) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
     const int64_t k = backward ? (int64_t)(n - 1 - r) : (int64_t)r;
-    return (int)(k * ncol / n);
+    if (B == 1) {
+        cmap[r] = (int)(k * A / n);
+    } else {
+        const int64_t nz = n / P, plane = k / P, pos = k % P;
+        cmap[r] = (int)(plane * A / nz) * B + (int)(pos * B / P);
+    }
+}
+
+// per row: the largest dependency offset |r - d|, or (lo > 0) minus the
+// smallest offset above lo (INT_MIN + 1 for none) -- max-reduced afterwards
+__global__ void dep_offsets(int n, int lo, const int *__restrict__ dptr, const int *__restrict__ ddep,
+                            int *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int mx = 0, mn = INT_MAX;
+    for (int e = dptr[r]; e < dptr[r + 1]; e++) {
+        const int o = abs(r - ddep[e]);
+        mx = max(mx, o);
+        if (o > lo) mn = min(mn, o);
+    }
+    out[r] = lo > 0 ? (mn == INT_MAX ? INT_MIN + 1 : -mn) : mx;
+}
+
+// flag: some dependency lies in a higher-numbered column
+__global__ void col_upward(int n, const int *__restrict__ cmap, const int *__restrict__ dptr,
+                           const int *__restrict__ ddep, int *__restrict__ flag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int c = cmap[r];
+    bool up = false;
+    for (int e = dptr[r]; e < dptr[r + 1]; e++) up |= cmap[ddep[e]] > c;
+    if (up) *flag = 1;
 }
 
 // cell populations and the largest level gap of a same-column dependency
-__global__ void col_count(int n, int ncol, int backward, const int *__restrict__ lev, const int *__restrict__ dptr,
-                          const int *__restrict__ ddep, int *__restrict__ cellcnt, int *__restrict__ maxgap) {
+__global__ void col_count(int n, int ncol, const int *__restrict__ cmap, const int *__restrict__ lev,
+                          const int *__restrict__ dptr, const int *__restrict__ ddep, int *__restrict__ cellcnt,
+                          int *__restrict__ maxgap) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    const int c = colof(r, n, ncol, backward);
+    const int c = cmap[r];
     atomicAdd(cellcnt + (int64_t)lev[r] * ncol + c, 1);
     int g = 0;
     for (int e = dptr[r]; e < dptr[r + 1]; e++) {
         int d = ddep[e];
-        if (colof(d, n, ncol, backward) == c) g = max(g, lev[r] - lev[d]);
+        if (cmap[d] == c) g = max(g, lev[r] - lev[d]);
     }
     if (g > 0) atomicMax(maxgap, g);
 }
 
-// per level: start position of every cell inside the level's row list
-// (rows ascend inside a level: columns ascend for a forward sweep and
-// descend for a backward one)
-__global__ void col_starts(int nlev, int ncol, int backward, const int *__restrict__ cellcnt,
+// per level: start position of every cell inside the level's row list (its
+// rows in (column, row) order; desc: rows ascending with columns descending,
+// the contiguous backward map in the (level, row) layout)
+__global__ void col_starts(int nlev, int ncol, int desc, const int *__restrict__ cellcnt,
                            int *__restrict__ cstart) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlev) return;
     int run = 0;
     for (int i = 0; i < ncol; i++) {
-        int c = backward ? ncol - 1 - i : i;
+        int c = desc ? ncol - 1 - i : i;
         cstart[(int64_t)l * ncol + c] = run;
         run += cellcnt[(int64_t)l * ncol + c];
     }
@@ -77,17 +125,17 @@ __device__ __forceinline__ bool is_local(int r, int d, int c, int cd, const int 
 }
 
 // external dependencies per slot
-__global__ void col_extcnt(int nslots, int n, int ncol, int backward, int K, const int *__restrict__ chunk_row,
+__global__ void col_extcnt(int nslots, const int *__restrict__ cmap, int K, const int *__restrict__ chunk_row,
                            const int *__restrict__ lev, const int *__restrict__ dptr, const int *__restrict__ ddep,
                            int *__restrict__ ecnt) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     int r = chunk_row[s], k = 0;
     if (r >= 0) {
-        const int c = colof(r, n, ncol, backward);
+        const int c = cmap[r];
         for (int e = dptr[r]; e < dptr[r + 1]; e++) {
             int d = ddep[e];
-            if (!is_local(r, d, c, colof(d, n, ncol, backward), lev, K)) k++;
+            if (!is_local(r, d, c, cmap[d], lev, K)) k++;
         }
     }
     ecnt[s] = k;
@@ -118,7 +166,7 @@ __global__ void col_cells(int nlev, int ncol, const int *__restrict__ lchunk, co
 }
 
 // dependency codes: >= 0 LDS window index, -1 none, <= -2 index into the cell's external list
-__global__ void col_records(int nslots, int n, int ncol, int backward, int K, int CP, int L, int nlev,
+__global__ void col_records(int nslots, const int *__restrict__ cmap, int ncol, int K, int CP, int L, int nlev,
                             const int *__restrict__ chunk_row, const int *__restrict__ lev,
                             const int *__restrict__ lchunk, const int *__restrict__ cstart,
                             const int *__restrict__ slot_of_row, const int *__restrict__ dptr,
@@ -131,7 +179,7 @@ __global__ void col_records(int nslots, int n, int ncol, int backward, int K, in
     const int l = lev[r];
     const int S0 = lchunk[l] * WAVE, Wl = (lchunk[l + 1] - lchunk[l]) * WAVE;
     const int q = s - S0;
-    const int c = colof(r, n, ncol, backward);
+    const int c = cmap[r];
     const int64_t base = (int64_t)s * L;  // thread-major: the row's L entries are contiguous
     const int a = dptr[r], len = dptr[r + 1] - a;
     int j = 0;
@@ -139,7 +187,7 @@ __global__ void col_records(int nslots, int n, int ncol, int backward, int K, in
         int cd = -1, sr = -1;
         if (k < len) {
             const int d = ddep[a + k];
-            const int dc = colof(d, n, ncol, backward);
+            const int dc = cmap[d];
             const int dl = lev[d];
             const int ds = slot_of_row[d];
             if (is_local(r, d, c, dc, lev, K)) {
@@ -326,21 +374,63 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     p.col.why = 2;
     if (maxlen > MAXD || maxlen < 1) return false;
     const int g = (n + 255) / 256;
-    dbuf cellcnt, gap, cstart, ecnt, epos, ci, colrange, maxext;
+    dbuf cellcnt, gap, cstart, ecnt, epos, ci, colrange, maxext, cmap;
     gap.reserve(sizeof(int));
     maxext.reserve(sizeof(int));
+    cmap.reserve((size_t)n * sizeof(int));
     // record width: the kernel variant's dependency capacity (4 or 16)
     const int L = maxlen <= 4 ? 4 : MAXD;
     const int cpmax = L == 4 ? CPMAX : CPMAX_LONG;
+    // plane size: the middle of the dependency offsets above half the largest
+    int P = 0;
+    if (c->opt.tri_col_split != 1) {
+        dep_offsets<<<g, 256, 0, c->stream>>>(n, 0, p.dptr.as<int>(), p.ddep.as<int>(), lens.as<int>());
+        HALA_CHECK_LAUNCH();
+        const int dmax = reduce_max_i32(c, lens.as<int>(), n);
+        if (dmax >= 2) {
+            dep_offsets<<<g, 256, 0, c->stream>>>(n, dmax / 2, p.dptr.as<int>(), p.ddep.as<int>(), lens.as<int>());
+            HALA_CHECK_LAUNCH();
+            const int dmin = -reduce_max_i32(c, lens.as<int>(), n);
+            P = (int)(((int64_t)dmax + dmin) / 2);
+        }
+        if (P <= 0 || n % P != 0) P = 0;
+    }
+    bool resorted = false;
     int ncol = ncol_req > 0 ? ncol_req : 4;
     while (ncol_req <= 0 && (int64_t)maxw > (int64_t)ncol * cpmax) ncol *= 2;
     for (; ncol <= 512; ncol *= 2) {
         if (ncol_req > 0 && ncol != ncol_req) break;
+        // pieces per slab: the option, else the largest power of two B with B * B <= ncol
+        int B = c->opt.tri_col_split > 0 ? c->opt.tri_col_split : 1;
+        if (c->opt.tri_col_split == 0)
+            while (4 * B * B <= ncol) B *= 2;
+        if (B > 1 && (P == 0 || ncol % B != 0 || (int64_t)P < (int64_t)B * WAVE || n / P < ncol / B)) B = 1;
+        if (B > 1) {
+            // Contiguous columns only wait on lower-numbered ones.  A split one
+            // may wait on a higher-numbered one (e.g. a 27-point row on the next
+            // piece of the previous plane): two columns then trade values every
+            // level (measured 1.6x slower on a 27-point operator), and every
+            // column needs a CU of its own to be sure to progress.  Automatic
+            // splits are therefore kept only where no dependency points upwards.
+            col_map<<<g, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, ncol / B, B, P, cmap.as<int>());
+            HALA_CHECK_LAUNCH();
+            HALA_HIP(hipMemsetAsync(gap.p, 0, sizeof(int), c->stream));
+            col_upward<<<g, 256, 0, c->stream>>>(n, cmap.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
+                                                gap.as<int>());
+            HALA_CHECK_LAUNCH();
+            const bool upward = fetch_i32(c, gap.as<int>()) != 0;
+            if (upward && (c->opt.tri_col_split == 0 || ncol > c->num_cus)) B = 1;
+        }
+        const int A = ncol / B;
+        if (B == 1) {
+            col_map<<<g, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, A, 1, 1, cmap.as<int>());
+            HALA_CHECK_LAUNCH();
+        }
         const int64_t ncell = (int64_t)nlev * ncol;
         cellcnt.reserve((size_t)ncell * sizeof(int));
         HALA_HIP(hipMemsetAsync(cellcnt.p, 0, (size_t)ncell * sizeof(int), c->stream));
         HALA_HIP(hipMemsetAsync(gap.p, 0, sizeof(int), c->stream));
-        col_count<<<g, 256, 0, c->stream>>>(n, ncol, p.backward ? 1 : 0, p.lev.as<int>(), p.dptr.as<int>(),
+        col_count<<<g, 256, 0, c->stream>>>(n, ncol, cmap.as<int>(), p.lev.as<int>(), p.dptr.as<int>(),
                                            p.ddep.as<int>(), cellcnt.as<int>(), gap.as<int>());
         HALA_CHECK_LAUNCH();
         const int maxcell = reduce_max_i32(c, cellcnt.as<int>(), (int)ncell);
@@ -350,20 +440,28 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         p.col.why = 3;
         if (CP > cpmax) continue;
         // automatic choice: narrower cells until a level's records fit the per-step budget
-        if (ncol_req <= 0 && ncol < 512 && (int64_t)CP * rec_bytes(L, sizeof(double)) > STEP_BYTES) continue;
+        if (ncol_req <= 0 && ncol < 512 &&
+            (int64_t)CP * rec_bytes(L, sizeof(double)) > (B > 1 ? STEP_BYTES_SPLIT : STEP_BYTES))
+            continue;
         const int K = std::min(fetch_i32(c, gap.as<int>()) + 1, KMAX);
+        // cells must be contiguous in every level's row list: the (level, row)
+        // layout has them for the contiguous map; otherwise order rows by column
+        if (B > 1 || resorted) {
+            reorder_levels(c, p, cmap.as<int>(), ncol);
+            resorted = true;
+        }
+        const int desc = (p.backward && !resorted) ? 1 : 0;
         // external counts, cell starts, cell records
         ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
         epos.reserve((size_t)(nslots + 1) * sizeof(int));
-        col_extcnt<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, n, ncol, p.backward ? 1 : 0, K,
-                                                               p.chunk_row.as<int>(), p.lev.as<int>(),
-                                                               p.dptr.as<int>(), p.ddep.as<int>(), ecnt.as<int>());
+        col_extcnt<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, cmap.as<int>(), K, p.chunk_row.as<int>(),
+                                                               p.lev.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
+                                                               ecnt.as<int>());
         HALA_CHECK_LAUNCH();
         exclusive_scan_i32(c, ecnt.as<int>(), epos.as<int>(), nslots);
         const int next = fetch_i32(c, epos.as<int>() + nslots);
         cstart.reserve((size_t)ncell * sizeof(int));
-        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, p.backward ? 1 : 0, cellcnt.as<int>(),
-                                                              cstart.as<int>());
+        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, desc, cellcnt.as<int>(), cstart.as<int>());
         HALA_CHECK_LAUNCH();
         ci.reserve((size_t)ncell * CI * sizeof(int));
         colrange.reserve((size_t)2 * ncol * sizeof(int));
@@ -384,6 +482,8 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         // dependency records, level by level in k-major order
         tri_colplan &q = p.col;
         q.ncol = ncol;
+        q.split = B;
+        q.plane = B > 1 ? P : 0;
         q.K = K;
         q.CP = CP;
         q.nlevcol = nlevcol;
@@ -399,7 +499,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         q.ext_slot.reserve((size_t)(next + BTMAX) * sizeof(int));
         HALA_HIP(hipMemsetAsync(q.ext_slot.p, 0, q.ext_slot.bytes, c->stream));
         col_records<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
-            nslots, n, ncol, p.backward ? 1 : 0, K, CP, q.L, nlev, p.chunk_row.as<int>(), p.lev.as<int>(),
+            nslots, cmap.as<int>(), ncol, K, CP, q.L, nlev, p.chunk_row.as<int>(), p.lev.as<int>(),
             p.lvl_chunk.as<int>(), cstart.as<int>(), p.slot_of_row.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
             p.dsrc.as<int>(), epos.as<int>(), q.code.as<int>(), q.src.as<int>(), q.ext_slot.as<int>());
         HALA_CHECK_LAUNCH();

@@ -223,6 +223,61 @@ __global__ void fill_entries(int nslots, const int *__restrict__ chunk_row, cons
     }
 }
 
+namespace {
+
+// Slot layout from a level-major row order: chunk rows and slot of every row,
+// entry slabs (k-th dependency of the 64 lanes contiguous) and diagonal sources.
+void lay_out_slots(hala_ctx c, tri_plan &p, const int *order) {
+    const int n = p.n, nslots = p.nchunk * WAVE, g = (n + 255) / 256;
+    p.chunk_row.reserve((size_t)(nslots + SLOT_PAD) * sizeof(int));
+    p.slot_of_row.reserve((size_t)n * sizeof(int));
+    p.chunk_len.reserve((size_t)(p.nchunk + 1) * sizeof(int));
+    p.chunk_off.reserve((size_t)(p.nchunk + 1) * sizeof(int));
+    fill_i32_k<<<(nslots + SLOT_PAD + 255) / 256, 256, 0, c->stream>>>(p.chunk_row.as<int>(), nslots + SLOT_PAD, -1);
+    fill_i32_k<<<(p.nchunk + 256) / 256, 256, 0, c->stream>>>(p.chunk_len.as<int>(), p.nchunk + 1, 0);
+    HALA_CHECK_LAUNCH();
+    place_rows<<<g, 256, 0, c->stream>>>(n, order, p.lev.as<int>(), p.level_ptr.as<int>(), p.lvl_chunk.as<int>(),
+                                         p.chunk_row.as<int>(), p.slot_of_row.as<int>());
+    HALA_CHECK_LAUNCH();
+    chunk_lengths<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), p.dptr.as<int>(),
+                                                              p.chunk_len.as<int>());
+    HALA_CHECK_LAUNCH();
+    exclusive_scan_i32(c, p.chunk_len.as<int>(), p.chunk_off.as<int>(), p.nchunk);
+    int slabs = fetch_i32(c, p.chunk_off.as<int>() + p.nchunk);
+    p.nent = (int64_t)slabs * WAVE;
+    size_t ne = (size_t)(p.nent > 0 ? p.nent : 1);
+    p.ent_dep.reserve(ne * sizeof(int));
+    p.ent_src.reserve(ne * sizeof(int));
+    p.dg_src.reserve((size_t)nslots * sizeof(int));
+    fill_entries<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
+        nslots, p.chunk_row.as<int>(), p.chunk_off.as<int>(), p.chunk_len.as<int>(), p.dptr.as<int>(),
+        p.ddep.as<int>(), p.dsrc.as<int>(), p.slot_of_row.as<int>(), p.diag_src.as<int>(), p.ent_dep.as<int>(),
+        p.ent_src.as<int>(), p.dg_src.as<int>());
+    HALA_CHECK_LAUNCH();
+}
+
+__global__ void level_key(int n, int nkey, const int *__restrict__ lev, const int *__restrict__ key,
+                          int *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = lev[r] * nkey + key[r];
+}
+
+}  // namespace
+
+void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey) {
+    const int n = p.n;
+    HALA_REQUIRE((int64_t)p.nlev * nkey < INT_MAX, HALA_ERR_ARG, "triangular plan: too many level keys");
+    dbuf bk, bptr, order;
+    bk.reserve((size_t)n * sizeof(int));
+    bptr.reserve((size_t)((int64_t)p.nlev * nkey + 1) * sizeof(int));
+    order.reserve((size_t)n * sizeof(int));
+    level_key<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nkey, p.lev.as<int>(), key, bk.as<int>());
+    HALA_CHECK_LAUNCH();
+    bucket_sort(c, n, bk.as<int>(), p.nlev * nkey, bptr.as<int>(), order.as<int>());
+    lay_out_slots(c, p, order.as<int>());
+    HALA_HIP(hipStreamSynchronize(c->stream));
+}
+
 void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     const int n = p.n;
     HALA_REQUIRE(n > 0, HALA_ERR_ARG, "triangular plan: empty matrix");
@@ -256,31 +311,7 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     exclusive_scan_i32(c, nch.as<int>(), chunk_ptr.as<int>(), p.nlev);
     p.nchunk = fetch_i32(c, chunk_ptr.as<int>() + p.nlev);
     const int nslots = p.nchunk * WAVE;
-    p.chunk_row.reserve((size_t)(nslots + SLOT_PAD) * sizeof(int));
-    p.slot_of_row.reserve((size_t)n * sizeof(int));
-    p.chunk_len.reserve((size_t)(p.nchunk + 1) * sizeof(int));
-    p.chunk_off.reserve((size_t)(p.nchunk + 1) * sizeof(int));
-    fill_i32_k<<<(nslots + SLOT_PAD + 255) / 256, 256, 0, c->stream>>>(p.chunk_row.as<int>(), nslots + SLOT_PAD, -1);
-    fill_i32_k<<<(p.nchunk + 256) / 256, 256, 0, c->stream>>>(p.chunk_len.as<int>(), p.nchunk + 1, 0);
-    HALA_CHECK_LAUNCH();
-    place_rows<<<g, 256, 0, c->stream>>>(n, p.perm.as<int>(), p.lev.as<int>(), p.level_ptr.as<int>(),
-                                         chunk_ptr.as<int>(), p.chunk_row.as<int>(), p.slot_of_row.as<int>());
-    HALA_CHECK_LAUNCH();
-    chunk_lengths<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), p.dptr.as<int>(),
-                                                              p.chunk_len.as<int>());
-    HALA_CHECK_LAUNCH();
-    exclusive_scan_i32(c, p.chunk_len.as<int>(), p.chunk_off.as<int>(), p.nchunk);
-    int slabs = fetch_i32(c, p.chunk_off.as<int>() + p.nchunk);
-    p.nent = (int64_t)slabs * WAVE;
-    size_t ne = (size_t)(p.nent > 0 ? p.nent : 1);
-    p.ent_dep.reserve(ne * sizeof(int));
-    p.ent_src.reserve(ne * sizeof(int));
-    p.dg_src.reserve((size_t)nslots * sizeof(int));
-    fill_entries<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
-        nslots, p.chunk_row.as<int>(), p.chunk_off.as<int>(), p.chunk_len.as<int>(), p.dptr.as<int>(),
-        p.ddep.as<int>(), p.dsrc.as<int>(), p.slot_of_row.as<int>(), p.diag_src.as<int>(), p.ent_dep.as<int>(),
-        p.ent_src.as<int>(), p.dg_src.as<int>());
-    HALA_CHECK_LAUNCH();
+    lay_out_slots(c, p, p.perm.as<int>());
     p.out64.reserve((size_t)nslots * sizeof(double));
     if (want32) p.out32.reserve((size_t)nslots * sizeof(float));
     tri_rearm(c, p);

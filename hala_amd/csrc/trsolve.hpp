@@ -28,10 +28,13 @@ enum tri_mode : int {
 };
 
 // Column-blocked layout of a plan (see trcol.hip): rows cut into ncol
-// contiguous blocks, each swept level by level by one workgroup.
+// blocks (contiguous, or pieces of slabs of planes), each swept level by
+// level by one workgroup.
 struct tri_colplan {
     bool ok = false;
     int ncol = 0;     // columns (one workgroup each)
+    int split = 1;    // pieces per slab of planes (1: contiguous row blocks)
+    int plane = 0;    // rows per plane of the split (0 when not split)
     int K = 0;        // levels kept in the LDS window
     int CP = 0;       // rows of the largest cell, rounded up to 64 (= compute threads per workgroup)
     int L = 0;        // dependency record width (longest row)
@@ -91,6 +94,11 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
 
 // Computes levels, permutation, chunks and entry layout; fills out64 with sentinels.
 void build_schedule(hala_ctx c, tri_plan &p, bool want32);
+
+// Re-orders the rows inside every level by (key[r], r), key in [0, nkey), and
+// rebuilds the slot layout; levels, level_ptr and perm ((level, row) order)
+// are unchanged.  Called before any values are loaded.
+void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey);
 
 // Gathers plan values from a factor value array (positions ent_src / diag_src).
 void load_values64(hala_ctx c, tri_plan &p, const double *factor);

@@ -149,9 +149,10 @@ int hala_ilu_create(hala_ctx ctx, int n, int64_t nnz, const int *pntr, const int
 int hala_ilu_destroy(hala_ilu ilu);
 int hala_ilu_info(hala_ilu ilu, hala_ilu_info_t *info);
 /* Layout of the triangular sweeps (no reference counterpart: rocSPARSE's
- * analysis is opaque).  out[6] = {column-blocked sweep in use (0/1), columns,
+ * analysis is opaque).  out[8] = {column-blocked sweep in use (0/1), columns,
  * rows per cell (threads per column workgroup), LDS window levels, most
- * external values one cell stages, reason code when not in use}. */
+ * external values one cell stages, reason code when not in use, columns per
+ * slab of planes (1: contiguous row blocks), rows per plane (0: not split)}. */
 int hala_ilu_sweep_info(hala_ilu ilu, int upper, int *out);
 int hala_ilu_factor_f64(hala_ilu ilu, const double *vals, int want_f32);
 int hala_ilu_apply_f64(hala_ilu ilu, const double *x, double *r);

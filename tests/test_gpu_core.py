@@ -257,11 +257,12 @@ COL_CASES = {  # name: (matrix, columns, lowest and highest distinct externals o
 def test_column_sweep_capacities_bitexact(engine, name):
     """Column-blocked sweeps run (not the level-pool fallback) at every staging capacity, bit-exact."""
     gen, ncol, lo, hi = COL_CASES[name]
-    keys = ("tri_col", "tri_col_cols")
+    keys = ("tri_col", "tri_col_cols", "tri_col_split")
     saved = [engine.get_option(k) for k in keys]
     try:
         engine.set_option("tri_col", 1)
         engine.set_option("tri_col_cols", ncol)
+        engine.set_option("tri_col_split", 1)  # contiguous row blocks: the capacities above are theirs
         p, i, v = gen()
         n = len(p) - 1
         M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
@@ -279,6 +280,53 @@ def test_column_sweep_capacities_bitexact(engine, name):
         M.apply_f32x64(dev(engine, x), gr)
         ref = oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x.astype(np.float32)).astype(np.float64)
         assert np.array_equal(bits(host(gr)), bits(ref))
+    finally:
+        for k, val in zip(keys, saved):
+            engine.set_option(k, val)
+
+
+SPLIT_CASES = {  # name: (matrix, columns, columns per slab of planes (0: automatic), expected split)
+    "lap3d_48_c16_auto": (lambda: st.laplace3d(48, 48, 48), 16, 0, 4),
+    "lap3d_40x24x32_c64_auto": (lambda: st.laplace3d(40, 24, 32), 64, 0, 8),
+    "lap3d_48_c32_s16": (lambda: st.laplace3d(48, 48, 48), 32, 16, 16),
+    # 27-point rows reach the next piece of the previous plane: no automatic split
+    "cd27_32_c16_auto": (lambda: st.convdiff27(32, seed=5), 16, 0, 1),
+    "cd27_32_c16_s4": (lambda: st.convdiff27(32, seed=5), 16, 4, 4),
+    "cd27_40_c64_s2": (lambda: st.convdiff27(40, seed=6), 64, 2, 2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SPLIT_CASES))
+def test_column_sweep_plane_split_bitexact(engine, name):
+    """Columns cut as pieces of slabs of planes (rows re-ordered inside every level): bit-exact, both precisions."""
+    gen, ncol, split, want = SPLIT_CASES[name]
+    keys = ("tri_col", "tri_col_cols", "tri_col_split")
+    saved = [engine.get_option(k) for k in keys]
+    try:
+        engine.set_option("tri_col", 1)
+        engine.set_option("tri_col_cols", ncol)
+        engine.set_option("tri_col_split", split)
+        p, i, v = gen()
+        n = len(p) - 1
+        M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
+        for upper in (False, True):
+            info = M.sweep_info(upper)
+            assert info["column"] and info["ncol"] == ncol and info["split"] == want, info
+        dg, _ = oracle.diag(p, i)
+        f = oracle.ilu0(p, i, v, dg)
+        x = st.uniform(29, n)
+        gr = engine.vector(n)
+        for _ in range(2):
+            M.apply(dev(engine, x), gr)
+            assert np.array_equal(bits(host(gr)), bits(oracle.ilu_apply(p, i, dg, f, x)))
+        M.apply_f32x64(dev(engine, x), gr)
+        ref = oracle.ilu_apply_f32(p, i, dg, f.astype(np.float32), x.astype(np.float32)).astype(np.float64)
+        assert np.array_equal(bits(host(gr)), bits(ref))
+        # the symbolic results the API reports keep the (level, row) order
+        for upper in (False, True):
+            _, lev, _, perm = M.symbolic(upper)
+            lev = host(lev)
+            assert np.array_equal(host(perm), np.lexsort((np.arange(n), lev)))
     finally:
         for k, val in zip(keys, saved):
             engine.set_option(k, val)
