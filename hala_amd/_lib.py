@@ -110,6 +110,10 @@ SIGNATURES = {
     "hala_gen_batch_laplace": [i32, u64, i32, i32, pi32, pi64, vp, vp, vp, vp, vp, vp, vp],
     "hala_gen_diffusion3d": [i32, u64, pi32, pi64, pi32, pi64, vp, vp, vp, vp, vp, vp, vp],
     "hala_gen_gaussian_source": [i32, f64, vp],
+    "hala_mm_info": [C.c_char_p, pi32, pi32, pi64, pi64, pi32, pi32],
+    "hala_mm_read_coo": [C.c_char_p, i32, i64, vp, vp, vp, pi64],
+    "hala_mm_write_csr": [C.c_char_p, i32, i32, vp, vp, vp],
+    "hala_coo_to_csr": [vp, i32, i32, i64, vp, vp, vp, i32, vp, vp, vp, pi64],
 }
 _RESTYPE = {"hala_last_error": C.c_char_p}
 

@@ -422,6 +422,13 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
             if (upward && (c->opt.tri_col_split == 0 || ncol > c->num_cus)) B = 1;
         }
         const int A = ncol / B;
+        // the widest level spreads over ncol cells, so one holds at least maxw / ncol
+        // of its rows: skip column counts whose cells cannot fit the budget
+        if (ncol_req <= 0 && ncol < 512) {
+            const int64_t cap = std::min<int64_t>(
+                cpmax, (B > 1 ? STEP_BYTES_SPLIT : STEP_BYTES) / rec_bytes(L, sizeof(double)) / WAVE * WAVE);
+            if ((int64_t)ncol * cap < maxw) continue;
+        }
         if (B == 1) {
             col_map<<<g, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, A, 1, 1, cmap.as<int>());
             HALA_CHECK_LAUNCH();

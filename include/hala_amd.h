@@ -238,6 +238,32 @@ int hala_grad_coef_f64(hala_ctx ctx, int ncoef, const int *cptr, const int *cnnz
 int hala_assemble_f64(hala_ctx ctx, int64_t nnz, const int *aptr, const int *acoef_idx, const double *aw,
                       const double *coef, double *vals);
 
+/* ---- ingestion (no reference counterpart: HALA takes CSR arrays that already
+ * have ascending unique columns and, for ILU(0), the diagonal,
+ * sparse/hala_sparse_core.hpp:42, :56-59) ----------------------------------- */
+enum { HALA_MM_REAL = 0, HALA_MM_INTEGER = 1, HALA_MM_PATTERN = 2 };
+enum { HALA_MM_GENERAL = 0, HALA_MM_SYMMETRIC = 1, HALA_MM_SKEW = 2 };
+/* Matrix Market header (host): sizes, entries stored in the file and an upper
+ * bound of the entries after symmetric expansion; field and symmetry codes.
+ * Coordinate (real, integer, pattern) and array (real, integer) files;
+ * general, symmetric and skew-symmetric. */
+int hala_mm_info(const char *path, int *nrows, int *ncols, int64_t *stored, int64_t *expanded, int *field,
+                 int *symmetry);
+/* Entries into host COO arrays (0-based), in file order; with expand != 0 the
+ * mirror (j,i) of an off-diagonal symmetric entry follows it (negated when
+ * skew-symmetric).  Array files give every stored position.  *count = entries. */
+int hala_mm_read_coo(const char *path, int expand, int64_t capacity, int *rows, int *cols, double *vals,
+                     int64_t *count);
+/* Host CSR to a "coordinate real general" file, %.17g values (round-trips bits). */
+int hala_mm_write_csr(const char *path, int nrows, int ncols, const int *pntr, const int *indx, const double *vals);
+/* Device COO to CSR with ascending unique columns: entries ordered by (row,
+ * column, position), duplicates summed left to right in input order,
+ * add_diag != 0 inserts a zero for every missing diagonal entry.  Call with
+ * indx = NULL for pntr (nrows + 1) and *nnz only, then with indx / vals_out of
+ * *nnz entries.  Entries out of range give HALA_ERR_PATTERN. */
+int hala_coo_to_csr(hala_ctx ctx, int nrows, int ncols, int64_t nent, const int *rows, const int *cols,
+                    const double *vals, int add_diag, int *pntr, int *indx, double *vals_out, int64_t *nnz);
+
 /* ---- synthetic problems (host generators, deterministic, splitmix64) ----- */
 /* Each fills caller-provided host arrays; call with NULL arrays first to get
  * sizes in *n / *nnz.  Stencils are sorted CSR with Dirichlet rows trimmed. */

@@ -52,6 +52,7 @@ _SIG = {
     "hala_or_grad_vals": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_grad_coef": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_transpose": (None, [i32, i32, vp, vp, vp, vp, vp, vp]),
+    "hala_or_coo_to_csr": (i64, [i32, i32, i64, vp, vp, vp, i32, vp, vp, vp]),
 }
 _REF_SIG = {
     "hala_ref_diag": [i32, vp, vp, vp],
@@ -137,6 +138,22 @@ def spmv_f32(pntr, indx, vals32, x32, alpha=1.0, beta=0.0, y=None):
     lib().hala_or_spmv_f32(n, alpha, P(ci32(pntr)), P(ci32(indx)), P(np.ascontiguousarray(vals32, np.float32)),
                            P(np.ascontiguousarray(x32, np.float32)), beta, P(y))
     return y
+
+
+def coo_to_csr(nrows, ncols, rows, cols, vals, add_diag=False):
+    """Host CSR (pntr, indx, vals) of COO triplets: (row, column, position) order, duplicates summed
+    left to right, optional zero diagonal inserted.  Raises ValueError for an entry out of range."""
+    rows, cols, vals = ci32(rows), ci32(cols), c64(vals)
+    pntr = np.zeros(nrows + 1, np.int32)
+    nnz = lib().hala_or_coo_to_csr(nrows, ncols, rows.size, P(rows), P(cols), P(vals), int(bool(add_diag)), P(pntr),
+                                   None, None)
+    if nnz < 0:
+        raise ValueError(f"entry {-1 - nnz} out of range")
+    indx = np.zeros(nnz, np.int32)
+    out = np.zeros(nnz, np.float64)
+    lib().hala_or_coo_to_csr(nrows, ncols, rows.size, P(rows), P(cols), P(vals), int(bool(add_diag)), P(pntr),
+                             P(indx), P(out))
+    return pntr, indx, out
 
 
 def diag(pntr, indx):

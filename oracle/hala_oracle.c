@@ -797,3 +797,68 @@ void hala_or_transpose(int nrows, int ncols, const int *pntr, const int *indx, c
         }
     free(cur);
 }
+
+/* ---- ingestion ----------------------------------------------------------- */
+static void coo_merge_sort(int64_t *a, int64_t *tmp, int64_t n, const int *rows, const int *cols) {
+    /* bottom-up stable merge sort of entry positions by (row, column) */
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) {
+                const int64_t ea = a[i], eb = a[j];
+                const int later = rows[eb] < rows[ea] || (rows[eb] == rows[ea] && cols[eb] < cols[ea]);
+                tmp[k++] = later ? a[j++] : a[i++];
+            }
+            while (i < mid) tmp[k++] = a[i++];
+            while (j < hi) tmp[k++] = a[j++];
+        }
+        memcpy(a, tmp, (size_t)n * sizeof(int64_t));
+    }
+}
+
+int64_t hala_or_coo_to_csr(int nrows, int ncols, int64_t nent, const int *rows, const int *cols, const double *vals,
+                           int add_diag, int *pntr, int *indx, double *vals_out) {
+    for (int64_t e = 0; e < nent; e++)
+        if (rows[e] < 0 || rows[e] >= nrows || cols[e] < 0 || cols[e] >= ncols) return -1 - e;
+    int64_t *a = (int64_t *)malloc((size_t)(nent > 0 ? nent : 1) * sizeof(int64_t));
+    int64_t *tmp = (int64_t *)malloc((size_t)(nent > 0 ? nent : 1) * sizeof(int64_t));
+    for (int64_t e = 0; e < nent; e++) a[e] = e;
+    coo_merge_sort(a, tmp, nent, rows, cols);
+    int64_t q = 0, o = 0;
+    pntr[0] = 0;
+    for (int r = 0; r < nrows; r++) {
+        int need = add_diag && r < ncols;
+        while (q < nent && rows[a[q]] == r) {
+            const int c = cols[a[q]];
+            if (need && c > r) {
+                if (indx) {
+                    indx[o] = r;
+                    vals_out[o] = 0.0;
+                }
+                o++;
+                need = 0;
+            }
+            double s = indx ? vals[a[q]] : 0.0;
+            for (q++; q < nent && rows[a[q]] == r && cols[a[q]] == c; q++)
+                if (indx) s = s + vals[a[q]];
+            if (c == r) need = 0;
+            if (indx) {
+                indx[o] = c;
+                vals_out[o] = s;
+            }
+            o++;
+        }
+        if (need) {
+            if (indx) {
+                indx[o] = r;
+                vals_out[o] = 0.0;
+            }
+            o++;
+        }
+        pntr[r + 1] = (int)o;
+    }
+    free(a);
+    free(tmp);
+    return o;
+}

@@ -36,6 +36,15 @@ void hala_or_spmv(char trans, int nrows, int ncols, double alpha, const int *pnt
 void hala_or_spmv_f32(int nrows, float alpha, const int *pntr, const int *indx, const float *vals,
                       const float *x, float beta, float *y);
 
+/* ---- ingestion ----------------------------------------------------------- */
+/* COO to CSR (no reference counterpart; the reference takes canonical CSR,
+ * sparse/hala_sparse_core.hpp:42): entries ordered by (row, column, input
+ * position) with a stable merge sort, duplicates summed left to right in input
+ * order, add_diag inserts 0 for a missing diagonal.  indx == NULL: pntr only.
+ * Returns nnz, or -1 - e for the first entry e out of range. */
+int64_t hala_or_coo_to_csr(int nrows, int ncols, int64_t nent, const int *rows, const int *cols, const double *vals,
+                           int add_diag, int *pntr, int *indx, double *vals_out);
+
 /* ---- symbolic analysis --------------------------------------------------- */
 int hala_or_diag(int n, const int *pntr, const int *indx, int *diag);
 int hala_or_levels(int n, const int *pntr, const int *indx, const int *diag, int upper,
