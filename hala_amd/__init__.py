@@ -446,3 +446,103 @@ def assemble(engine, aptr, aidx, aw, coef, vals):
     call("hala_assemble_f64", engine.handle, int(aptr.numel()) - 1, _ptr(aptr), _ptr(aidx), _ptr(aw), _ptr(coef),
          _ptr(vals))
     return vals
+
+
+# ---- mixed engine (wax/hala_lib_extensions.hpp:68-376, wax/hala_mixed_overloads.hpp) ----
+class mixed_engine:
+    """Host (numpy) inputs and outputs, the work on the GPU: every call loads its numpy arguments
+    into device tensors, runs the gpu_engine path and copies its outputs back in place."""
+
+    def __init__(self, gpu):
+        self._gpu = gpu
+
+    def gpu(self):
+        return self._gpu
+
+    def device(self):
+        return self._gpu.device()
+
+
+class mixed_ilu:
+    """ILU(0) of host arrays (wax/hala_lib_extensions.hpp:317-376): the device factor keeps its own
+    copies of the pattern; apply() takes and fills numpy arrays."""
+
+    def __init__(self, engine, gilu):
+        self.engine, self._gilu = engine, gilu
+
+    def cilu(self):
+        return self._gilu
+
+    def apply(self, x, r, num_rhs=1):
+        gx = _to_dev(self.engine, x)
+        gr = self.engine.gpu().vector(gx.numel())
+        n = self._gilu.n
+        for k in range(num_rhs):  # column k of n entries
+            self._gilu.apply(gx[k * n:(k + 1) * n], gr[k * n:(k + 1) * n])
+        r[...] = gr.cpu().numpy().reshape(r.shape)
+        return r
+
+
+def _to_dev(engine, a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(f"cuda:{engine.device()}")
+
+
+def _mixable(fn, outs=()):
+    import functools
+    import inspect
+
+    sig = inspect.signature(fn)
+
+    @functools.wraps(fn)
+    def run(engine, *args, **kw):
+        if not isinstance(engine, mixed_engine):
+            return fn(engine, *args, **kw)
+        ba = sig.bind(engine, *args, **kw)
+        moved = {}
+        for name, val in list(ba.arguments.items()):
+            if isinstance(val, np.ndarray):
+                t = _to_dev(engine, val)
+                ba.arguments[name] = t
+                moved[name] = (val, t)
+            elif isinstance(val, mixed_ilu):
+                ba.arguments[name] = val.cilu()
+        ba.arguments["engine"] = engine.gpu()
+        out = fn(*ba.args, **ba.kwargs)
+        for name in outs:
+            if name in moved:
+                host, t = moved[name]
+                host[...] = t.cpu().numpy().reshape(host.shape)
+        return out
+
+    return run
+
+
+_make_ilu_gpu = make_ilu
+
+
+def make_ilu(engine, pntr, indx, vals, policy="N", want_f32=False):  # noqa: F811
+    """make_ilu(engine, pntr, indx, vals, policy): gpu/hala_gpu_ilu.hpp:191-195 (gpu_engine) and
+    wax/hala_lib_extensions.hpp:368-376 (mixed_engine: host arrays, returns a mixed_ilu)."""
+    if isinstance(engine, mixed_engine):
+        g = engine.gpu()
+        return mixed_ilu(engine, _make_ilu_gpu(g, _to_dev(engine, pntr), _to_dev(engine, indx),
+                                               _to_dev(engine, vals), policy, want_f32))
+    return _make_ilu_gpu(engine, pntr, indx, vals, policy, want_f32)
+
+
+dot = _mixable(dot)
+norm2 = _mixable(norm2)
+axpy = _mixable(axpy, ("y",))
+scal = _mixable(scal, ("x",))
+vcopy = _mixable(vcopy, ("y",))
+sparse_gemv = _mixable(sparse_gemv, ("y",))
+make_sparse_matrix = _mixable(make_sparse_matrix)
+make_triangular_matrix = _mixable(make_triangular_matrix)
+solve_cg = _mixable(solve_cg, ("x", "hist"))
+solve_cg_ilu = _mixable(solve_cg_ilu, ("x", "hist"))
+solve_batch_cg = _mixable(solve_batch_cg, ("X", "hist"))
+solve_batch_cg_ilu = _mixable(solve_batch_cg_ilu, ("X", "hist"))
+solve_gmres = _mixable(solve_gmres, ("x", "hist"))
+solve_gmres_ilu = _mixable(solve_gmres_ilu, ("x", "hist"))
+__all__ += ["mixed_engine", "mixed_ilu"]

@@ -8,10 +8,12 @@
 // C++14 compiler and the library (no HIP headers).  Library errors become
 // std::runtime_error, as in the reference (gpu/hala_rocm_common.hpp:88-109).
 //
-// Scope: real scalars (float, double), unit strides, the gpu_engine.  Vectors
-// on the device are hala::gpu_vector<T> or hala::gpu_wrapped_array<T>.
+// Scope: real scalars (float, double), unit strides, the gpu_engine and the
+// mixed_engine (host-resident inputs, hala_mixed.hpp).  Vectors on the device
+// are hala::gpu_vector<T> or hala::gpu_wrapped_array<T>.
 #pragma once
 
+#include <algorithm>
 #include <cassert>
 #include <cstring>
 #include <functional>
@@ -580,3 +582,5 @@ size_t ilu_buffer_size(ILU const &ilu, VX const &x, VR &&r, int num_rhs) {
 }
 
 }  // namespace hala
+
+#include "hala_mixed.hpp"

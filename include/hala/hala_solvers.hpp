@@ -354,4 +354,109 @@ int solve_gmres_ir(gpu_engine const &engine, double rtol, int restart, VP const 
     return total;
 }
 
+// ---- mixed engine: host-resident inputs loaded once, the device solve, x copied back
+// (hex/solvers/hala_solvers_cg.hpp:249-264, :329-340; hala_solvers_gmres.hpp and
+// hala_solvers_cg_batch.hpp, their mixed_engine overloads).  A user preconditioner
+// receives device vectors, as with the gpu_engine.
+namespace detail {
+template <class VP, class VI, class VV, class F>
+int mixed_solve(mixed_engine const &engine, VP const &pntr, VI const &indx, VV const &vals, F &&run) {
+    auto gp = engine.gpu().load(pntr);
+    auto gi = engine.gpu().load(indx);
+    auto gv = engine.gpu().load(vals);
+    return run(gp, gi, gv);
+}
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+             VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_cg(engine.gpu(), stop, gp, gi, gv, precon, gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, mixed_ilu<double> const &ilu, VB const &b, VX &x) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_cg_ilu(engine.gpu(), stop, gp, gi, gv, ilu.cilu(), gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, VB const &b, VX &x, char policy = 'N') {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_cg_ilu(engine.gpu(), stop, gp, gi, gv, gb, gx, policy);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                   VV const &vals, preconditioner<double> precon, VB const &B, VX &X) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(B);
+        auto gx = gpu_bind_vector(engine.gpu(), X);
+        return solve_batch_cg(engine.gpu(), stop, gp, gi, gv, precon, gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr,
+                       VI const &indx, VV const &vals, mixed_ilu<double> const &ilu, VB const &B, VX &X) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(B);
+        auto gx = gpu_bind_vector(engine.gpu(), X);
+        return solve_batch_cg_ilu(engine.gpu(), stop, gp, gi, gv, ilu.cilu(), gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr,
+                       VI const &indx, VV const &vals, VB const &B, VX &X, char policy = 'N') {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(B);
+        auto gx = gpu_bind_vector(engine.gpu(), X);
+        return solve_batch_cg_ilu(engine.gpu(), stop, gp, gi, gv, gb, gx, policy);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                VI const &indx, VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_gmres(engine.gpu(), stop, restart, gp, gi, gv, precon, gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                    VI const &indx, VV const &vals, mixed_ilu<double> const &ilu, VB const &b, VX &x) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_gmres_ilu(engine.gpu(), stop, restart, gp, gi, gv, ilu.cilu(), gb, gx);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                    VI const &indx, VV const &vals, VB const &b, VX &x, char policy = 'N') {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_gmres_ilu(engine.gpu(), stop, restart, gp, gi, gv, gb, gx, policy);
+    });
+}
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ir(mixed_engine const &engine, double rtol, int restart, VP const &pntr, VI const &indx,
+                   VV const &vals, VB const &b, VX &x, double inner_rtol = 1e-4, int max_refine = 50) {
+    return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
+        auto gb = engine.gpu().load(b);
+        auto gx = gpu_bind_vector(engine.gpu(), x);
+        return solve_gmres_ir(engine.gpu(), rtol, restart, gp, gi, gv, gb, gx, inner_rtol, max_refine);
+    });
+}
+
 }  // namespace hala

@@ -212,6 +212,80 @@ int main() {
         CHECK(relerr(x3) < 1e-8);
         std::printf("PASS solve_batch_cg\n");
     }
+    // the same cases through the mixed engine: std::vector in and out
+    // (tests/sparse_tests.cpp:52-63, tests/solvers_tests.cpp:59-67 run them so)
+    {
+        hala::mixed_engine mengine(engine);
+        std::vector<int> pntr = {0, 2, 5, 8, 11, 13};
+        std::vector<int> indx = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4};
+        std::vector<double> vals = {2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0};
+        std::vector<double> xref = {1.0, 2.0, 3.0, 4.0, 5.0}, y, x;
+        // BLAS-1 and the product
+        std::vector<double> a = {1.0, -2.0, 3.0}, b2 = {4.0, 5.0, -6.0};
+        CHECK(hala::dot(mengine, a, b2) == 4.0 - 10.0 - 18.0);
+        CHECK(std::fabs(hala::norm2(mengine, a) - std::sqrt(14.0)) < 1e-15);
+        hala::axpy(mengine, 2.0, a, b2);
+        CHECK(diffsum(b2, {6.0, 1.0, 0.0}) == 0.0);
+        hala::scal(mengine, -1.0, b2);
+        CHECK(diffsum(b2, {-6.0, -1.0, 0.0}) == 0.0);
+        std::vector<double> c2;
+        hala::vcopy(mengine, a, c2);
+        CHECK(diffsum(c2, a) == 0.0);
+        hala::sparse_gemv(mengine, 'N', 5, 5, 2.0, pntr, indx, vals, xref, 0.0, y);
+        auto ref = densemul(pntr, indx, vals, xref);
+        for (auto &r : ref) r *= 2.0;
+        CHECK(diffsum(y, ref) == 0.0);
+        auto mat = hala::make_sparse_matrix(mengine, 5, pntr, indx, vals);
+        mat.gemv('T', 1.0, xref, 0.0, y);
+        CHECK(diffsum(y, densemul(pntr, indx, vals, xref, true)) == 0.0);
+        // triangular solve and the ILU known answer (tests/sparse_tests.hpp:259-311, :423-440)
+        {
+            std::vector<int> tp = {0, 1, 3, 5}, ti = {0, 0, 1, 0, 2};
+            std::vector<double> tv = {1.0, 4.0, 2.0, 5.0, 3.0}, bN = {-6.0, -32.0, -36.0}, tx;
+            auto tri = hala::make_triangular_matrix(mengine, 'L', 'N', tp, ti, tv);
+            hala::sparse_trsv('N', tri, 0.5, bN, tx);
+            CHECK(diffsum(tx, {-3.0, -2.0, -1.0}) < 1e-14);
+            std::vector<int> ip = {0, 3, 6, 9}, ii = {0, 1, 2, 0, 1, 2, 0, 1, 2};
+            std::vector<double> iv = {3.0, 2.0, 1.0, 2.0, 4.0, 3.0, 2.0, 1.0, 6.0}, r;
+            auto milu = hala::make_ilu(mengine, ip, ii, iv);
+            hala::ilu_apply(milu, std::vector<double>{1.0, 2.0, 3.0}, r);
+            CHECK(diffsum(r, {1.0 / 9.0, 1.0 / 9.0, 4.0 / 9.0}) < 1e-14);
+        }
+        // solvers: the device result copied back into the std::vector
+        auto b = densemul(pntr, indx, vals, xref);
+        auto ilu = hala::make_ilu(mengine, pntr, indx, vals);
+        hala::solve_cg(mengine, 2, pntr, indx, vals,
+                       [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                           hala::ilu_apply(ilu.cilu(), inx, outr);
+                       },
+                       b, x);
+        CHECK(x.size() == 5 && diffsum(x, xref) < 1e-14);
+        x.clear();
+        hala::solve_cg_ilu(mengine, 1.E-9, pntr, indx, vals, ilu, b, x);
+        CHECK(diffsum(x, xref) < 1e-12);
+        std::vector<double> xg;
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        hala::gpu_vector<double> gx;
+        hala::solve_gmres_ilu(engine, 1.E-10, 5, gp, gi, gv, hala::make_gpu_vector(b), gx);
+        gx.unload(xg);
+        x.clear();
+        hala::solve_gmres_ilu(mengine, 1.E-10, 5, pntr, indx, vals, b, x);  // factorisation behind the scene
+        CHECK(diffsum(x, xg) == 0.0);
+        x.clear();
+        hala::solve_gmres(mengine, 1.E-10, 5, pntr, indx, vals,
+                          [&](hala::gpu_vector<double> const &inx, hala::gpu_vector<double> &outr) {
+                              hala::vcopy(engine, inx, outr);
+                          },
+                          b, x);
+        CHECK(diffsum(x, xref) < 1e-9);
+        std::vector<double> B, X;
+        for (int k = 0; k < 3; k++) B.insert(B.end(), b.begin(), b.end());
+        hala::solve_batch_cg_ilu(mengine, {1.E-9, 6}, pntr, indx, vals, ilu, B, X);
+        CHECK(X.size() == 15 && diffsum(std::vector<double>(X.begin() + 10, X.end()), xref) < 1e-12);
+        std::printf("PASS mixed_engine\n");
+    }
     // tests/solvers_tests.hpp:105-137 (GMRES)
     {
         std::vector<int> pntr = {0, 2, 5, 8, 11, 12};
