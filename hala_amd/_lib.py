@@ -114,6 +114,22 @@ SIGNATURES = {
     "hala_mm_read_coo": [C.c_char_p, i32, i64, vp, vp, vp, pi64],
     "hala_mm_write_csr": [C.c_char_p, i32, i32, vp, vp, vp],
     "hala_coo_to_csr": [vp, i32, i32, i64, vp, vp, vp, i32, vp, vp, vp, pi64],
+    "hala_dotc_z": [vp, i64, vp, vp, vp],
+    "hala_dotu_z": [vp, i64, vp, vp, vp],
+    "hala_nrm2_z": [vp, i64, vp, vp],
+    "hala_axpy_z": [vp, i64, f64, f64, vp, vp],
+    "hala_scal_z": [vp, i64, f64, f64, vp],
+    "hala_spmv_z": [vp, ch, f64, f64, vp, vp, f64, f64, vp],
+    "hala_ilu_factor_z": [vp, vp],
+    "hala_ilu_apply_z": [vp, vp, vp],
+    "hala_dotc_c": [vp, i64, vp, vp, vp],
+    "hala_dotu_c": [vp, i64, vp, vp, vp],
+    "hala_nrm2_c": [vp, i64, vp, vp],
+    "hala_axpy_c": [vp, i64, f32, f32, vp, vp],
+    "hala_scal_c": [vp, i64, f32, f32, vp],
+    "hala_spmv_c": [vp, ch, f32, f32, vp, vp, f32, f32, vp],
+    "hala_ilu_factor_c": [vp, vp],
+    "hala_ilu_apply_c": [vp, vp, vp],
 }
 _RESTYPE = {"hala_last_error": C.c_char_p}
 

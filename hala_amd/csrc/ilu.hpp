@@ -20,6 +20,9 @@ struct hala_ilu_s {
     // with FP32 values, FP32 diagonal, four FP32 iterates (built on first use)
     hala_amd::dbuf jl_ptr, jl_idx, jl_val, ju_ptr, ju_idx, ju_val, jd_val, jwork;
     bool jac_pattern = false, jac_values = false;
+    // complex factors (cplx.hip): interleaved real/imaginary, CSR order
+    hala_amd::dbuf zfactor, cfactor;
+    bool zfactored = false, cfactored = false;
 };
 
 struct hala_tri_s {

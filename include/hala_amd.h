@@ -238,6 +238,32 @@ int hala_grad_coef_f64(hala_ctx ctx, int ncoef, const int *cptr, const int *cnnz
 int hala_assemble_f64(hala_ctx ctx, int64_t nnz, const int *aptr, const int *acoef_idx, const double *aw,
                       const double *coef, double *vals);
 
+/* ---- complex scalars: std::complex<double> ("z") and <float> ("c") arrays,
+ * real and imaginary parts interleaved.  The reference instantiates its
+ * BLAS-1, CSR product and ILU(0) for both (tests/solvers_tests.cpp:14-40,
+ * gpu/hala_gpu_blas1.hpp, gpu/hala_rocm_sparse_general.hpp,
+ * gpu/hala_gpu_ilu.hpp).  res: two reals (real, imaginary) for the dots, one
+ * for nrm2.  dotc conjugates x (HALA's dot on complex vectors), dotu does not.
+ * trans 'N', 'T' or 'C'.  The ILU factor reuses the handle's symbolic analysis. */
+int hala_dotc_z(hala_ctx ctx, int64_t n, const double *x, const double *y, double *res);
+int hala_dotu_z(hala_ctx ctx, int64_t n, const double *x, const double *y, double *res);
+int hala_nrm2_z(hala_ctx ctx, int64_t n, const double *x, double *res);
+int hala_axpy_z(hala_ctx ctx, int64_t n, double alpha_re, double alpha_im, const double *x, double *y);
+int hala_scal_z(hala_ctx ctx, int64_t n, double alpha_re, double alpha_im, double *x);
+int hala_spmv_z(hala_csr csr, char trans, double alpha_re, double alpha_im, const double *vals, const double *x,
+                double beta_re, double beta_im, double *y);
+int hala_ilu_factor_z(hala_ilu ilu, const double *vals);
+int hala_ilu_apply_z(hala_ilu ilu, const double *x, double *r);
+int hala_dotc_c(hala_ctx ctx, int64_t n, const float *x, const float *y, float *res);
+int hala_dotu_c(hala_ctx ctx, int64_t n, const float *x, const float *y, float *res);
+int hala_nrm2_c(hala_ctx ctx, int64_t n, const float *x, float *res);
+int hala_axpy_c(hala_ctx ctx, int64_t n, float alpha_re, float alpha_im, const float *x, float *y);
+int hala_scal_c(hala_ctx ctx, int64_t n, float alpha_re, float alpha_im, float *x);
+int hala_spmv_c(hala_csr csr, char trans, float alpha_re, float alpha_im, const float *vals, const float *x,
+                float beta_re, float beta_im, float *y);
+int hala_ilu_factor_c(hala_ilu ilu, const float *vals);
+int hala_ilu_apply_c(hala_ilu ilu, const float *x, float *r);
+
 /* ---- ingestion (no reference counterpart: HALA takes CSR arrays that already
  * have ascending unique columns and, for ILU(0), the diagonal,
  * sparse/hala_sparse_core.hpp:42, :56-59) ----------------------------------- */

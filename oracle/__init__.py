@@ -53,6 +53,16 @@ _SIG = {
     "hala_or_grad_coef": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_transpose": (None, [i32, i32, vp, vp, vp, vp, vp, vp]),
     "hala_or_coo_to_csr": (i64, [i32, i32, i64, vp, vp, vp, i32, vp, vp, vp]),
+    "hala_or_zdot_z": (None, [i64, vp, vp, i32, vp]),
+    "hala_or_zdot_c": (None, [i64, vp, vp, i32, vp]),
+    "hala_or_znrm2_z": (f64, [i64, vp]),
+    "hala_or_znrm2_c": (f32, [i64, vp]),
+    "hala_or_zspmv_z": (None, [ch, i32, i32, f64, f64, vp, vp, vp, vp, f64, f64, vp]),
+    "hala_or_zspmv_c": (None, [ch, i32, i32, f32, f32, vp, vp, vp, vp, f32, f32, vp]),
+    "hala_or_zilu0_z": (None, [i32, vp, vp, vp, vp]),
+    "hala_or_zilu0_c": (None, [i32, vp, vp, vp, vp]),
+    "hala_or_zilu_apply_z": (None, [i32, vp, vp, vp, vp, vp, vp]),
+    "hala_or_zilu_apply_c": (None, [i32, vp, vp, vp, vp, vp, vp]),
 }
 _REF_SIG = {
     "hala_ref_diag": [i32, vp, vp, vp],
@@ -138,6 +148,56 @@ def spmv_f32(pntr, indx, vals32, x32, alpha=1.0, beta=0.0, y=None):
     lib().hala_or_spmv_f32(n, alpha, P(ci32(pntr)), P(ci32(indx)), P(np.ascontiguousarray(vals32, np.float32)),
                            P(np.ascontiguousarray(x32, np.float32)), beta, P(y))
     return y
+
+
+# ---- complex scalars (numpy complex128 / complex64 arrays) -------------------
+def _zs(a):
+    """(array, suffix): complex128 -> 'z', complex64 -> 'c'."""
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.complex128:
+        return a, "z"
+    if a.dtype == np.complex64:
+        return a, "c"
+    raise TypeError("complex128 or complex64 expected")
+
+
+def zdot(x, y, conj=True):
+    """sum conj(x_i) y_i (conj) or sum x_i y_i, canonical order per part."""
+    x, sf = _zs(x)
+    y = np.ascontiguousarray(y, dtype=x.dtype)
+    res = np.zeros(2, np.float64 if sf == "z" else np.float32)
+    getattr(lib(), "hala_or_zdot_" + sf)(x.size, P(x), P(y), int(bool(conj)), P(res))
+    return complex(res[0], res[1])
+
+
+def znrm2(x):
+    x, sf = _zs(x)
+    return getattr(lib(), "hala_or_znrm2_" + sf)(x.size, P(x))
+
+
+def zspmv(trans, nrows, ncols, alpha, pntr, indx, vals, x, beta, y):
+    vals, sf = _zs(vals)
+    x = np.ascontiguousarray(x, dtype=vals.dtype)
+    getattr(lib(), "hala_or_zspmv_" + sf)(trans.encode() if isinstance(trans, str) else trans, nrows, ncols,
+                                           alpha.real, alpha.imag, P(ci32(pntr)), P(ci32(indx)), P(vals), P(x),
+                                           beta.real, beta.imag, P(y))
+    return y
+
+
+def zilu0(pntr, indx, vals, dg):
+    vals, sf = _zs(vals)
+    f = vals.copy()
+    getattr(lib(), "hala_or_zilu0_" + sf)(pntr.size - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)), P(f))
+    return f
+
+
+def zilu_apply(pntr, indx, dg, f, x):
+    f, sf = _zs(f)
+    x = np.ascontiguousarray(x, dtype=f.dtype)
+    r = np.zeros_like(x)
+    getattr(lib(), "hala_or_zilu_apply_" + sf)(pntr.size - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)), P(f), P(x),
+                                                P(r))
+    return r
 
 
 def coo_to_csr(nrows, ncols, rows, cols, vals, add_diag=False):

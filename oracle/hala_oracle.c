@@ -862,3 +862,156 @@ int64_t hala_or_coo_to_csr(int nrows, int ncols, int64_t nent, const int *rows, 
     free(tmp);
     return o;
 }
+
+/* ---- complex scalars ------------------------------------------------------ */
+/* Canonical sum (hala_or_cdot's order) of a term array, in the term type. */
+#define HALA_OR_CSUM(R, NAME)                                                 \
+    static R NAME(int64_t n, const R *t) {                                    \
+        if (n <= 0) return (R)0;                                              \
+        int64_t nb = (n + CB - 1) / CB;                                       \
+        R *part = (R *)malloc((size_t)nb * sizeof(R));                        \
+        R a[CB];                                                              \
+        for (int64_t b = 0; b < nb; b++) {                                    \
+            for (int q = 0; q < CB; q++) {                                    \
+                int64_t i = b * CB + q;                                       \
+                a[q] = (i < n) ? t[i] : (R)0;                                 \
+            }                                                                 \
+            for (int off = CB / 2; off >= 1; off >>= 1)                       \
+                for (int q = 0; q < off; q++) a[q] = a[q] + a[q + off];       \
+            part[b] = a[0];                                                   \
+        }                                                                     \
+        for (int q = 0; q < CB; q++) {                                        \
+            R s = (R)0;                                                       \
+            for (int64_t k = q; k < nb; k += CB) s = s + part[k];             \
+            a[q] = s;                                                         \
+        }                                                                     \
+        free(part);                                                           \
+        for (int off = CB / 2; off >= 1; off >>= 1)                           \
+            for (int q = 0; q < off; q++) a[q] = a[q] + a[q + off];           \
+        return a[0];                                                          \
+    }
+HALA_OR_CSUM(double, csum_z)
+HALA_OR_CSUM(float, csum_c)
+
+#define HALA_OR_COMPLEX(SUF, R, CSUM, SQRT)                                                                          \
+    void hala_or_zdot_##SUF(int64_t n, const R *x, const R *y, int conj, R *res) {                                   \
+        R *tr = (R *)malloc((size_t)(n > 0 ? n : 1) * sizeof(R)), *ti = (R *)malloc((size_t)(n > 0 ? n : 1) * sizeof(R)); \
+        for (int64_t i = 0; i < n; i++) {                                                                            \
+            const R xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];                              \
+            tr[i] = conj ? xr * yr + xi * yi : xr * yr - xi * yi;                                                    \
+            ti[i] = conj ? xr * yi - xi * yr : xr * yi + xi * yr;                                                    \
+        }                                                                                                            \
+        res[0] = CSUM(n, tr);                                                                                        \
+        res[1] = CSUM(n, ti);                                                                                        \
+        free(tr);                                                                                                    \
+        free(ti);                                                                                                    \
+    }                                                                                                                \
+    R hala_or_znrm2_##SUF(int64_t n, const R *x) {                                                                   \
+        R *t = (R *)malloc((size_t)(n > 0 ? n : 1) * sizeof(R));                                                     \
+        for (int64_t i = 0; i < n; i++) t[i] = x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1];                    \
+        R r = SQRT(CSUM(n, t));                                                                                      \
+        free(t);                                                                                                     \
+        return r;                                                                                                    \
+    }                                                                                                                \
+    /* sparse/hala_sparse_utils.hpp:104-118 element order, complex arithmetic */                                     \
+    void hala_or_zspmv_##SUF(char trans, int nrows, int ncols, R are, R aim, const int *pntr, const int *indx,       \
+                             const R *vals, const R *x, R bre, R bim, R *y) {                                        \
+        const int beta0 = bre == (R)0 && bim == (R)0;                                                                \
+        if (trans == 'N' || trans == 'n') {                                                                          \
+            for (int i = 0; i < nrows; i++) {                                                                        \
+                R sr = (R)0, si = (R)0;                                                                              \
+                for (int j = pntr[i]; j < pntr[i + 1]; j++) {                                                        \
+                    const R vr = vals[2 * j], vi = vals[2 * j + 1], xr = x[2 * indx[j]], xi = x[2 * indx[j] + 1];     \
+                    sr = sr + (vr * xr - vi * xi);                                                                   \
+                    si = si + (vr * xi + vi * xr);                                                                   \
+                }                                                                                                    \
+                R orr = are * sr - aim * si, oi = are * si + aim * sr;                                               \
+                if (!beta0) {                                                                                        \
+                    const R yr = y[2 * i], yi = y[2 * i + 1];                                                        \
+                    orr = orr + (bre * yr - bim * yi);                                                               \
+                    oi = oi + (bre * yi + bim * yr);                                                                 \
+                }                                                                                                    \
+                y[2 * i] = orr;                                                                                      \
+                y[2 * i + 1] = oi;                                                                                   \
+            }                                                                                                        \
+            return;                                                                                                  \
+        }                                                                                                            \
+        const int cj = trans == 'C' || trans == 'c';                                                                 \
+        for (int k = 0; k < ncols; k++) {                                                                            \
+            if (beta0) {                                                                                             \
+                y[2 * k] = (R)0;                                                                                     \
+                y[2 * k + 1] = (R)0;                                                                                 \
+            } else {                                                                                                 \
+                const R yr = y[2 * k], yi = y[2 * k + 1];                                                            \
+                y[2 * k] = bre * yr - bim * yi;                                                                      \
+                y[2 * k + 1] = bre * yi + bim * yr;                                                                  \
+            }                                                                                                        \
+        }                                                                                                            \
+        for (int i = 0; i < nrows; i++) {                                                                            \
+            const R axr = are * x[2 * i] - aim * x[2 * i + 1], axi = are * x[2 * i + 1] + aim * x[2 * i];             \
+            for (int j = pntr[i]; j < pntr[i + 1]; j++) {                                                            \
+                const R vr = vals[2 * j], vi = cj ? -vals[2 * j + 1] : vals[2 * j + 1];                              \
+                const int k = indx[j];                                                                               \
+                y[2 * k] = y[2 * k] + (axr * vr - axi * vi);                                                         \
+                y[2 * k + 1] = y[2 * k + 1] + (axr * vi + axi * vr);                                                 \
+            }                                                                                                        \
+        }                                                                                                            \
+    }                                                                                                                \
+    /* IKJ ILU(0) (sparse/hala_sparse_utils.hpp:229-254 operation order per entry), in place */                      \
+    void hala_or_zilu0_##SUF(int n, const int *pntr, const int *indx, const int *diag, R *f) {                       \
+        for (int j = 0; j < n; j++) {                                                                                \
+            const int je = pntr[j + 1];                                                                              \
+            for (int jc = pntr[j]; jc < diag[j]; jc++) {                                                             \
+                const int i = indx[jc];                                                                              \
+                const R ar = f[2 * jc], ai = f[2 * jc + 1], br = f[2 * diag[i]], bi = f[2 * diag[i] + 1];             \
+                const R d = br * br + bi * bi;                                                                       \
+                const R lr = (ar * br + ai * bi) / d, li = (ai * br - ar * bi) / d;                                  \
+                f[2 * jc] = lr;                                                                                      \
+                f[2 * jc + 1] = li;                                                                                  \
+                int ik = diag[i] + 1, jk = jc + 1;                                                                   \
+                const int ie = pntr[i + 1];                                                                          \
+                while (ik < ie && jk < je) {                                                                         \
+                    if (indx[ik] == indx[jk]) {                                                                      \
+                        const R ur = f[2 * ik], ui = f[2 * ik + 1];                                                  \
+                        f[2 * jk] = f[2 * jk] - (lr * ur - li * ui);                                                 \
+                        f[2 * jk + 1] = f[2 * jk + 1] - (lr * ui + li * ur);                                         \
+                        ik++;                                                                                        \
+                        jk++;                                                                                        \
+                    } else if (indx[ik] < indx[jk]) {                                                                \
+                        ik++;                                                                                        \
+                    } else {                                                                                         \
+                        jk++;                                                                                        \
+                    }                                                                                                \
+                }                                                                                                    \
+            }                                                                                                        \
+        }                                                                                                            \
+    }                                                                                                                \
+    /* (LU)^-1 x: unit-L forward sweep then U backward sweep (sparse/hala_sparse_utils.hpp:263-275) */              \
+    void hala_or_zilu_apply_##SUF(int n, const int *pntr, const int *indx, const int *diag, const R *f, const R *x,  \
+                                  R *r) {                                                                            \
+        R *y = (R *)malloc((size_t)(2 * (int64_t)n > 0 ? 2 * (int64_t)n : 1) * sizeof(R));                           \
+        for (int i = 0; i < n; i++) {                                                                                \
+            R sr = x[2 * i], si = x[2 * i + 1];                                                                      \
+            for (int j = pntr[i]; j < diag[i]; j++) {                                                                \
+                const R vr = f[2 * j], vi = f[2 * j + 1], yr = y[2 * indx[j]], yi = y[2 * indx[j] + 1];              \
+                sr = sr - (vr * yr - vi * yi);                                                                       \
+                si = si - (vr * yi + vi * yr);                                                                       \
+            }                                                                                                        \
+            y[2 * i] = sr;                                                                                           \
+            y[2 * i + 1] = si;                                                                                       \
+        }                                                                                                            \
+        for (int i = n - 1; i >= 0; i--) {                                                                           \
+            R sr = y[2 * i], si = y[2 * i + 1];                                                                      \
+            for (int j = diag[i] + 1; j < pntr[i + 1]; j++) {                                                        \
+                const R vr = f[2 * j], vi = f[2 * j + 1], zr = r[2 * indx[j]], zi = r[2 * indx[j] + 1];              \
+                sr = sr - (vr * zr - vi * zi);                                                                       \
+                si = si - (vr * zi + vi * zr);                                                                       \
+            }                                                                                                        \
+            const R br = f[2 * diag[i]], bi = f[2 * diag[i] + 1], d = br * br + bi * bi;                             \
+            r[2 * i] = (sr * br + si * bi) / d;                                                                      \
+            r[2 * i + 1] = (si * br - sr * bi) / d;                                                                  \
+        }                                                                                                            \
+        free(y);                                                                                                     \
+    }
+HALA_OR_COMPLEX(z, double, csum_z, sqrt)
+HALA_OR_COMPLEX(c, float, csum_c, sqrtf)

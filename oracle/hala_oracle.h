@@ -36,6 +36,27 @@ void hala_or_spmv(char trans, int nrows, int ncols, double alpha, const int *pnt
 void hala_or_spmv_f32(int nrows, float alpha, const int *pntr, const int *indx, const float *vals,
                       const float *x, float beta, float *y);
 
+/* ---- complex scalars (interleaved real, imaginary) ------------------------- */
+/* The reference instantiates its routines for complex<float/double>
+ * (tests/solvers_tests.cpp:14-40).  Products and quotients:
+ *   a*b = (ar br - ai bi, ar bi + ai br);  a/b = ((ar br + ai bi)/d, (ai br - ar bi)/d),
+ *   d = br br + bi bi.  dot = conjugated (blas/hala_blas_1.hpp dot vs dotu);
+ * dotc/dotu/nrm2 sum per-element terms in the canonical order (hala_or_cdot). */
+void hala_or_zdot_z(int64_t n, const double *x, const double *y, int conj, double *res);
+void hala_or_zdot_c(int64_t n, const float *x, const float *y, int conj, float *res);
+double hala_or_znrm2_z(int64_t n, const double *x);
+float hala_or_znrm2_c(int64_t n, const float *x);
+void hala_or_zspmv_z(char trans, int nrows, int ncols, double are, double aim, const int *pntr, const int *indx,
+                     const double *vals, const double *x, double bre, double bim, double *y);
+void hala_or_zspmv_c(char trans, int nrows, int ncols, float are, float aim, const int *pntr, const int *indx,
+                     const float *vals, const float *x, float bre, float bim, float *y);
+void hala_or_zilu0_z(int n, const int *pntr, const int *indx, const int *diag, double *f);
+void hala_or_zilu0_c(int n, const int *pntr, const int *indx, const int *diag, float *f);
+void hala_or_zilu_apply_z(int n, const int *pntr, const int *indx, const int *diag, const double *f, const double *x,
+                          double *r);
+void hala_or_zilu_apply_c(int n, const int *pntr, const int *indx, const int *diag, const float *f, const float *x,
+                          float *r);
+
 /* ---- ingestion ----------------------------------------------------------- */
 /* COO to CSR (no reference counterpart; the reference takes canonical CSR,
  * sparse/hala_sparse_core.hpp:42): entries ordered by (row, column, input
