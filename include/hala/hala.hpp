@@ -8,13 +8,16 @@
 // C++14 compiler and the library (no HIP headers).  Library errors become
 // std::runtime_error, as in the reference (gpu/hala_rocm_common.hpp:88-109).
 //
-// Scope: real scalars (float, double), unit strides, the gpu_engine and the
+// Scope: float, double, std::complex<float>, std::complex<double>, unit
+// strides, the gpu_engine and the
 // mixed_engine (host-resident inputs, hala_mixed.hpp).  Vectors on the device
 // are hala::gpu_vector<T> or hala::gpu_wrapped_array<T>.
 #pragma once
 
 #include <algorithm>
 #include <cassert>
+#include <cmath>
+#include <complex>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -119,6 +122,23 @@ struct host_traits<std::valarray<T>> {
 
 // ---------------------------------------------------------------------------
 // device vector
+// Per-type entry points: real FP64/FP32 and complex<double>/<float> (stored as
+// interleaved real, imaginary pairs, as std::complex is).
+namespace detail {
+inline void fill_(hala_ctx c, double *x, int64_t n, double v) { check_hala(hala_fill_f64(c, x, n, v), "fill()"); }
+inline void fill_(hala_ctx c, float *x, int64_t n, float v) { check_hala(hala_fill_f32(c, x, n, v), "fill()"); }
+inline void fill_(hala_ctx c, int *x, int64_t n, int v) { check_hala(hala_fill_i32(c, x, n, v), "fill()"); }
+template <typename R>
+void fill_(hala_ctx c, std::complex<R> *x, int64_t n, std::complex<R> v) {
+    if (v.imag() == R(0) && v.real() == R(0)) {
+        fill_(c, reinterpret_cast<R *>(x), 2 * n, R(0));
+    } else {
+        std::vector<std::complex<R>> h((size_t)n, v);
+        check_hala(hala_memcpy(c, x, h.data(), (size_t)n * sizeof(std::complex<R>), 0), "fill()");
+    }
+}
+}  // namespace detail
+
 // ---------------------------------------------------------------------------
 // gpu/hala_gpu_vector.hpp:49-171: owning device buffer.
 template <typename T>
@@ -180,13 +200,7 @@ class gpu_vector {
     }
     void fill(T value) {
         if (!num) return;
-        hala_ctx c = default_ctx(gpu);
-        if (std::is_same<T, double>::value)
-            check_hala(hala_fill_f64(c, reinterpret_cast<double *>(ptr), (int64_t)num, (double)value), "fill()");
-        else if (std::is_same<T, float>::value)
-            check_hala(hala_fill_f32(c, reinterpret_cast<float *>(ptr), (int64_t)num, (float)value), "fill()");
-        else
-            check_hala(hala_fill_i32(c, reinterpret_cast<int *>(ptr), (int64_t)num, (int)value), "fill()");
+        detail::fill_(default_ctx(gpu), ptr, (int64_t)num, value);
     }
     T *data() { return ptr; }
     T const *data() const { return ptr; }
@@ -292,12 +306,7 @@ void set_zero(gpu_engine const &engine, size_t num_entries, VectorLike &&x) {
     if (num_entries == 0) return;
     if (x.size() < num_entries) x.resize(num_entries);
     using T = std::remove_cv_t<typename std::remove_reference_t<VectorLike>::value_type>;
-    if (std::is_same<T, double>::value)
-        check_hala(hala_fill_f64(engine.handle(), reinterpret_cast<double *>(x.data()), (int64_t)num_entries, 0.0),
-                   "set_zero()");
-    else
-        check_hala(hala_fill_f32(engine.handle(), reinterpret_cast<float *>(x.data()), (int64_t)num_entries, 0.0f),
-                   "set_zero()");
+    detail::fill_(engine.handle(), reinterpret_cast<T *>(x.data()), (int64_t)num_entries, T(0));
 }
 
 template <class V>
@@ -306,66 +315,136 @@ using scalar_of = std::remove_cv_t<typename std::remove_reference_t<V>::value_ty
 // ---------------------------------------------------------------------------
 // BLAS-1 on the engine (gpu/hala_gpu_blas1.hpp, gpu/hala_gpu_overloads.hpp)
 // ---------------------------------------------------------------------------
+namespace detail {
+using zd = std::complex<double>;
+using zf = std::complex<float>;
+inline void copy_(hala_ctx c, int n, double const *x, double *y) { check_hala(hala_copy_f64(c, n, x, y), "vcopy()"); }
+inline void copy_(hala_ctx c, int n, float const *x, float *y) { check_hala(hala_copy_f32(c, n, x, y), "vcopy()"); }
+inline void copy_(hala_ctx c, int n, zd const *x, zd *y) {
+    check_hala(hala_copy_f64(c, 2 * n, reinterpret_cast<double const *>(x), reinterpret_cast<double *>(y)), "vcopy()");
+}
+inline void copy_(hala_ctx c, int n, zf const *x, zf *y) {
+    check_hala(hala_copy_f32(c, 2 * n, reinterpret_cast<float const *>(x), reinterpret_cast<float *>(y)), "vcopy()");
+}
+inline double nrm2_(hala_ctx c, int64_t n, double const *x) {
+    double r;
+    check_hala(hala_nrm2_f64(c, n, x, &r), "norm2()");
+    return r;
+}
+inline float nrm2_(hala_ctx c, int64_t n, float const *x) {
+    float r;
+    check_hala(hala_nrm2_f32(c, n, x, &r), "norm2()");
+    return r;
+}
+inline double nrm2_(hala_ctx c, int64_t n, zd const *x) {
+    double r;
+    check_hala(hala_nrm2_z(c, n, reinterpret_cast<double const *>(x), &r), "norm2()");
+    return r;
+}
+inline float nrm2_(hala_ctx c, int64_t n, zf const *x) {
+    float r;
+    check_hala(hala_nrm2_c(c, n, reinterpret_cast<float const *>(x), &r), "norm2()");
+    return r;
+}
+// dot: conjugates x for complex vectors (the reference's dot vs dotu)
+inline double dot_(hala_ctx c, int64_t n, double const *x, double const *y, bool) {
+    double r;
+    check_hala(hala_dot_f64(c, n, x, y, &r), "dot()");
+    return r;
+}
+inline float dot_(hala_ctx c, int64_t n, float const *x, float const *y, bool) {
+    float r;
+    check_hala(hala_dot_f32(c, n, x, y, &r), "dot()");
+    return r;
+}
+inline zd dot_(hala_ctx c, int64_t n, zd const *x, zd const *y, bool conj) {
+    double r[2];
+    auto fn = conj ? hala_dotc_z : hala_dotu_z;
+    check_hala(fn(c, n, reinterpret_cast<double const *>(x), reinterpret_cast<double const *>(y), r), "dot()");
+    return {r[0], r[1]};
+}
+inline zf dot_(hala_ctx c, int64_t n, zf const *x, zf const *y, bool conj) {
+    float r[2];
+    auto fn = conj ? hala_dotc_c : hala_dotu_c;
+    check_hala(fn(c, n, reinterpret_cast<float const *>(x), reinterpret_cast<float const *>(y), r), "dot()");
+    return {r[0], r[1]};
+}
+inline void axpy_(hala_ctx c, int64_t n, double a, double const *x, double *y) {
+    check_hala(hala_axpy_f64(c, n, a, x, y), "axpy()");
+}
+inline void axpy_(hala_ctx c, int64_t n, float a, float const *x, float *y) {
+    check_hala(hala_axpy_f32(c, n, a, x, y), "axpy()");
+}
+inline void axpy_(hala_ctx c, int64_t n, zd a, zd const *x, zd *y) {
+    check_hala(hala_axpy_z(c, n, a.real(), a.imag(), reinterpret_cast<double const *>(x), reinterpret_cast<double *>(y)),
+               "axpy()");
+}
+inline void axpy_(hala_ctx c, int64_t n, zf a, zf const *x, zf *y) {
+    check_hala(hala_axpy_c(c, n, a.real(), a.imag(), reinterpret_cast<float const *>(x), reinterpret_cast<float *>(y)),
+               "axpy()");
+}
+inline void scal_(hala_ctx c, int64_t n, double a, double *x) { check_hala(hala_scal_f64(c, n, a, x), "scal()"); }
+inline void scal_(hala_ctx c, int64_t n, float a, float *x) { check_hala(hala_scal_f32(c, n, a, x), "scal()"); }
+inline void scal_(hala_ctx c, int64_t n, zd a, zd *x) {
+    check_hala(hala_scal_z(c, n, a.real(), a.imag(), reinterpret_cast<double *>(x)), "scal()");
+}
+inline void scal_(hala_ctx c, int64_t n, zf a, zf *x) {
+    check_hala(hala_scal_c(c, n, a.real(), a.imag(), reinterpret_cast<float *>(x)), "scal()");
+}
+inline void spmv_(hala_csr A, char t, double a, double const *v, double const *x, double b, double *y) {
+    check_hala(hala_spmv_f64(A, t, a, v, x, b, y), "gpu_sparse_matrix::gemv()");
+}
+inline void spmv_(hala_csr A, char t, float a, float const *v, float const *x, float b, float *y) {
+    check_hala(hala_spmv_f32(A, t, a, v, x, b, y), "gpu_sparse_matrix::gemv()");
+}
+inline void spmv_(hala_csr A, char t, zd a, zd const *v, zd const *x, zd b, zd *y) {
+    check_hala(hala_spmv_z(A, t, a.real(), a.imag(), reinterpret_cast<double const *>(v),
+                           reinterpret_cast<double const *>(x), b.real(), b.imag(), reinterpret_cast<double *>(y)),
+               "gpu_sparse_matrix::gemv()");
+}
+inline void spmv_(hala_csr A, char t, zf a, zf const *v, zf const *x, zf b, zf *y) {
+    check_hala(hala_spmv_c(A, t, a.real(), a.imag(), reinterpret_cast<float const *>(v),
+                           reinterpret_cast<float const *>(x), b.real(), b.imag(), reinterpret_cast<float *>(y)),
+               "gpu_sparse_matrix::gemv()");
+}
+}  // namespace detail
+
 template <class VX, class VY>
 void vcopy(gpu_engine const &engine, int N, VX const &x, int incx, VY &&y, int incy) {
     runtime_assert(incx == 1 && incy == 1, "hala_amd: unit strides only");
     if ((int)y.size() < N) y.resize(N);
     using T = scalar_of<VX>;
-    if (std::is_same<T, double>::value)
-        check_hala(hala_copy_f64(engine.handle(), N, (double const *)x.data(), (double *)y.data()), "vcopy()");
-    else
-        check_hala(hala_copy_f32(engine.handle(), N, (float const *)x.data(), (float *)y.data()), "vcopy()");
+    detail::copy_(engine.handle(), N, (T const *)x.data(), (T *)y.data());
 }
 template <class VX, class VY>
 void vcopy(gpu_engine const &engine, VX const &x, VY &&y) {
     vcopy(engine, (int)x.size(), x, 1, y, 1);
 }
+// norm2: the precision type (double for complex<double>)
 template <class VX>
 auto norm2(gpu_engine const &engine, VX const &x) {
     using T = scalar_of<VX>;
-    if (std::is_same<T, double>::value) {
-        double r;
-        check_hala(hala_nrm2_f64(engine.handle(), (int64_t)x.size(), (double const *)x.data(), &r), "norm2()");
-        return (T)r;
-    }
-    float r;
-    check_hala(hala_nrm2_f32(engine.handle(), (int64_t)x.size(), (float const *)x.data(), &r), "norm2()");
-    return (T)r;
+    return detail::nrm2_(engine.handle(), (int64_t)x.size(), (T const *)x.data());
 }
 template <class VX, class VY>
 auto dot(gpu_engine const &engine, VX const &x, VY const &y) {
     using T = scalar_of<VX>;
-    if (std::is_same<T, double>::value) {
-        double r;
-        check_hala(hala_dot_f64(engine.handle(), (int64_t)x.size(), (double const *)x.data(), (double const *)y.data(),
-                                &r),
-                   "dot()");
-        return (T)r;
-    }
-    float r;
-    check_hala(hala_dot_f32(engine.handle(), (int64_t)x.size(), (float const *)x.data(), (float const *)y.data(), &r),
-               "dot()");
-    return (T)r;
+    return detail::dot_(engine.handle(), (int64_t)x.size(), (T const *)x.data(), (T const *)y.data(), true);
+}
+template <class VX, class VY>
+auto dotu(gpu_engine const &engine, VX const &x, VY const &y) {
+    using T = scalar_of<VX>;
+    return detail::dot_(engine.handle(), (int64_t)x.size(), (T const *)x.data(), (T const *)y.data(), false);
 }
 template <typename FP, class VX, class VY>
 void axpy(gpu_engine const &engine, FP alpha, VX const &x, VY &&y) {
     using T = scalar_of<VX>;
-    if (std::is_same<T, double>::value)
-        check_hala(hala_axpy_f64(engine.handle(), (int64_t)x.size(), (double)alpha, (double const *)x.data(),
-                                 (double *)y.data()),
-                   "axpy()");
-    else
-        check_hala(hala_axpy_f32(engine.handle(), (int64_t)x.size(), (float)alpha, (float const *)x.data(),
-                                 (float *)y.data()),
-                   "axpy()");
+    detail::axpy_(engine.handle(), (int64_t)x.size(), (T)alpha, (T const *)x.data(), (T *)y.data());
 }
 template <typename FP, class VX>
 void scal(gpu_engine const &engine, FP alpha, VX &&x) {
     using T = scalar_of<VX>;
-    if (std::is_same<T, double>::value)
-        check_hala(hala_scal_f64(engine.handle(), (int64_t)x.size(), (double)alpha, (double *)x.data()), "scal()");
-    else
-        check_hala(hala_scal_f32(engine.handle(), (int64_t)x.size(), (float)alpha, (float *)x.data()), "scal()");
+    detail::scal_(engine.handle(), (int64_t)x.size(), (T)alpha, (T *)x.data());
 }
 // dense column-major gemv (gpu/hala_gpu_blas2.hpp:40-60), FP64
 template <class VA, class VX, class VY>
@@ -415,17 +494,11 @@ class gpu_sparse_matrix {
     void gemv(char trans, FPa alpha, VX const &x, FPb beta, VY &&y) const {
         size_t out = (trans == 'N' || trans == 'n') ? rows : cols;
         if (y.size() < out) {
-            runtime_assert(beta == 0, "gemv: y too small for beta != 0");
+            runtime_assert(beta == FPb(0), "gemv: y too small for beta != 0");
             y.resize(out);
         }
-        if (std::is_same<T, double>::value)
-            check_hala(hala_spmv_f64(csr.get(), trans, (double)alpha, (double const *)rvals,
-                                     (double const *)x.data(), (double)beta, (double *)y.data()),
-                       "gpu_sparse_matrix::gemv()");
-        else
-            check_hala(hala_spmv_f32(csr.get(), trans, (float)alpha, (float const *)rvals, (float const *)x.data(),
-                                     (float)beta, (float *)y.data()),
-                       "gpu_sparse_matrix::gemv()");
+        runtime_assert(beta == FPb(0) || y.size() >= out, "gemv: y too small");
+        detail::spmv_(csr.get(), trans, (T)alpha, rvals, (T const *)x.data(), (T)beta, (T *)y.data());
     }
     T const *values() const { return rvals; }
 
@@ -521,12 +594,11 @@ class gpu_ilu {
     gpu_ilu(gpu_engine const &engine, VP const &pntr, VI const &indx, VV const &vals, char policy = 'N',
             bool want_f32 = false)
         : rengine(engine), n((int)pntr.size() - 1), rpolicy(policy) {
-        static_assert(std::is_same<T, double>::value, "hala_amd: ILU factors are computed in FP64");
         hala_ilu h = nullptr;
         check_hala(hala_ilu_create(engine.handle(), n, (int64_t)indx.size(), pntr.data(), indx.data(), &h),
                    "make_ilu()");
         ilu = std::shared_ptr<hala_ilu_s>(h, [](hala_ilu p) { hala_ilu_destroy(p); });
-        check_hala(hala_ilu_factor_f64(h, (double const *)vals.data(), want_f32 ? 1 : 0), "make_ilu()");
+        factor(engine, (T const *)vals.data(), (int64_t)indx.size(), want_f32);
     }
     template <class VX, class VR>
     size_t buffer_size(VX const &, VR &&, int) const {
@@ -546,14 +618,42 @@ class gpu_ilu {
         runtime_assert(num_rhs >= 1 && x.size() >= (size_t)n * num_rhs, "gpu_ilu::apply(): x too small");
         if (r.size() < (size_t)n * num_rhs) r.resize((size_t)n * num_rhs);
         for (int k = 0; k < num_rhs; k++)
-            check_hala(hala_ilu_apply_f64(ilu.get(), (double const *)x.data() + (size_t)k * n,
-                                          (double *)r.data() + (size_t)k * n),
-                       "gpu_ilu::apply()");
+            apply_(ilu.get(), (T const *)x.data() + (size_t)k * n, (T *)r.data() + (size_t)k * n);
     }
     gpu_engine const &engine() const { return rengine; }
     hala_ilu handle() const { return ilu.get(); }
 
   private:
+    // FP64 factors (FP32 copies on request); FP32 values are factored in FP64 and applied in FP32;
+    // complex values in their own precision
+    void factor(gpu_engine const &, double const *v, int64_t, bool want_f32) {
+        check_hala(hala_ilu_factor_f64(ilu.get(), v, want_f32 ? 1 : 0), "make_ilu()");
+    }
+    void factor(gpu_engine const &engine, float const *v, int64_t nnz, bool) {
+        gpu_vector<double> v64((size_t)nnz, engine.device());
+        check_hala(hala_convert_f32_f64(engine.handle(), nnz, v, v64.data()), "make_ilu()");
+        check_hala(hala_ilu_factor_f64(ilu.get(), v64.data(), 1), "make_ilu()");
+    }
+    void factor(gpu_engine const &, std::complex<double> const *v, int64_t, bool) {
+        check_hala(hala_ilu_factor_z(ilu.get(), reinterpret_cast<double const *>(v)), "make_ilu()");
+    }
+    void factor(gpu_engine const &, std::complex<float> const *v, int64_t, bool) {
+        check_hala(hala_ilu_factor_c(ilu.get(), reinterpret_cast<float const *>(v)), "make_ilu()");
+    }
+    static void apply_(hala_ilu h, double const *x, double *r) {
+        check_hala(hala_ilu_apply_f64(h, x, r), "gpu_ilu::apply()");
+    }
+    static void apply_(hala_ilu h, float const *x, float *r) {
+        check_hala(hala_ilu_apply_f32(h, x, r), "gpu_ilu::apply()");
+    }
+    static void apply_(hala_ilu h, std::complex<double> const *x, std::complex<double> *r) {
+        check_hala(hala_ilu_apply_z(h, reinterpret_cast<double const *>(x), reinterpret_cast<double *>(r)),
+                   "gpu_ilu::apply()");
+    }
+    static void apply_(hala_ilu h, std::complex<float> const *x, std::complex<float> *r) {
+        check_hala(hala_ilu_apply_c(h, reinterpret_cast<float const *>(x), reinterpret_cast<float *>(r)),
+                   "gpu_ilu::apply()");
+    }
     gpu_engine rengine;
     int n;
     char rpolicy;

@@ -44,24 +44,25 @@ void prepare_x(gpu_engine const &engine, int n, VX &x) {
 // hex/solvers/hala_solvers_cg.hpp:181-244 with solve_cg_core :92-156.
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-             VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+             VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+    using T = scalar_of<VV>;
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_cg: size mismatch");
     prepare_x(engine, n, x);
     auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
-    gpu_vector<double> r(engine.device()), p(engine.device()), ap(engine.device()), z(engine.device());
+    gpu_vector<T> r(engine.device()), p(engine.device()), ap(engine.device()), z(engine.device());
     vcopy(engine, b, r);
     matrix.gemv('N', 1.0, x, 0.0, p);
     axpy(engine, -1.0, p, r);
     precon(r, z);
     vcopy(engine, z, p);
-    double zr = dot(engine, r, z);
+    T zr = dot(engine, r, z);  // conjugated for complex vectors
     int iterations = 1;
     for (;;) {
         matrix.gemv('N', 1.0, p, 0.0, ap);
         iterations++;
-        double nzr = dot(engine, p, ap);
-        double a = zr / nzr;
+        T nzr = dot(engine, p, ap);
+        T a = zr / nzr;
         axpy(engine, a, p, x);
         axpy(engine, -a, ap, r);
         if (iterations == stop.max_iter || norm2(engine, r) < stop.tol) break;
@@ -78,9 +79,11 @@ int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP con
 // solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x): hex/solvers/hala_solvers_cg.hpp:312-326.
 // Runs device-resident (hala_cg_f64); identical arithmetic to solve_cg with
 // the ILU apply as preconditioner.
+// Other scalar types run solve_cg's host loop with the ILU apply as preconditioner.
+namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                 VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x) {
+int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+           VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_cg_ilu: size mismatch");
     prepare_x(engine, n, x);
@@ -90,6 +93,19 @@ int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP
                            (double const *)b.data(), (double *)x.data(), stop.tol, stop.max_iter, &it, nullptr, 0),
                "solve_cg_ilu()");
     return it;
+}
+template <class VP, class VI, class VV, class T, class VB, class VX>
+int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+           VV const &vals, gpu_ilu<T> const &ilu, VB const &b, VX &x, std::false_type) {
+    return solve_cg(engine, stop, pntr, indx, vals,
+                    [&](gpu_vector<T> const &in, gpu_vector<T> &out) { ilu.apply(in, out); }, b, x);
+}
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
+    return detail::cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x, std::is_same<scalar_of<VV>, double>());
 }
 
 // solve_cg_ilu(engine, stop, pntr, indx, vals, b, x, policy): factor on the fly (:354-361).
@@ -104,8 +120,8 @@ int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP
 // B, X hold nrhs right-hand sides / solutions of n entries each, column-major
 // (column k at offset k n), as the reference's sparse_gemm.
 template <class V>
-gpu_wrapped_array<double> batch_column(gpu_engine const &engine, V &v, int n, int k) {
-    return wrap_gpu_array((double *)v.data() + (size_t)k * n, (size_t)n, engine.device());
+gpu_wrapped_array<scalar_of<V>> batch_column(gpu_engine const &engine, V &v, int n, int k) {
+    return wrap_gpu_array((scalar_of<V> *)v.data() + (size_t)k * n, (size_t)n, engine.device());
 }
 
 // solve_batch_cg(engine, stop, pntr, indx, vals, precon, B, X):
@@ -115,13 +131,14 @@ gpu_wrapped_array<double> batch_column(gpu_engine const &engine, V &v, int n, in
 // Host loop over the engine's per-column BLAS, the reference's step order.
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                   VV const &vals, preconditioner<double> precon, VB const &B, VX &X) {
+                   VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &B, VX &X) {
+    using T = scalar_of<VV>;
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg: size mismatch");
     const int nrhs = (int)(B.size() / (size_t)n);
     prepare_x(engine, (int)B.size(), X);
     auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
-    gpu_vector<double> r(B.size(), engine.device()), p(B.size(), engine.device()), ap(B.size(), engine.device()),
+    gpu_vector<T> r(B.size(), engine.device()), p(B.size(), engine.device()), ap(B.size(), engine.device()),
         z(B.size(), engine.device());
     auto col = [&](auto &v, int k) { return batch_column(engine, v, n, k); };
     vcopy(engine, B, r);
@@ -129,24 +146,24 @@ int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, 
     axpy(engine, -1.0, p, r);
     precon(r, z);
     vcopy(engine, z, p);
-    std::vector<double> zr(nrhs), a(nrhs);
+    std::vector<T> zr(nrhs), a(nrhs);
     for (int k = 0; k < nrhs; k++) zr[k] = dot(engine, col(r, k), col(z, k));
     int iterations = 1;
     for (;;) {
         for (int k = 0; k < nrhs; k++) matrix.gemv('N', 1.0, col(p, k), 0.0, col(ap, k));
         iterations++;
         for (int k = 0; k < nrhs; k++) a[k] = zr[k] / dot(engine, col(p, k), col(ap, k));
-        for (int k = 0; k < nrhs; k++) axpy(engine, 1.0 * a[k], col(p, k), col(X, k));
-        for (int k = 0; k < nrhs; k++) axpy(engine, -1.0 * a[k], col(ap, k), col(r, k));
-        double mx = 0.0;  // batch_max_norm2: the largest <r_k, r_k>, then its square root
+        for (int k = 0; k < nrhs; k++) axpy(engine, T(1) * a[k], col(p, k), col(X, k));
+        for (int k = 0; k < nrhs; k++) axpy(engine, T(-1) * a[k], col(ap, k), col(r, k));
+        auto mx = std::real(T(0));  // batch_max_norm2: the largest <r_k, r_k>, then its square root
         for (int k = 0; k < nrhs; k++) {
-            const double d = dot(engine, col(r, k), col(r, k));
+            const auto d = std::real(dot(engine, col(r, k), col(r, k)));
             if (k == 0 || d > mx) mx = d;
         }
         if (iterations == stop.max_iter || std::sqrt(mx) < stop.tol) break;
         precon(r, z);
         for (int k = 0; k < nrhs; k++) {
-            const double nzr = dot(engine, col(r, k), col(z, k));
+            const T nzr = dot(engine, col(r, k), col(z, k));
             scal(engine, nzr / zr[k], col(p, k));
             zr[k] = nzr;
         }
@@ -159,9 +176,10 @@ int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, 
 // hex/solvers/hala_solvers_cg_batch.hpp:156-200.  Runs device-resident
 // (hala_cg_batch_f64), the same arithmetic as solve_batch_cg with the ILU
 // applied to every column.
+namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                       VV const &vals, gpu_ilu<double> const &ilu, VB const &B, VX &X) {
+int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, gpu_ilu<double> const &ilu, VB const &B, VX &X, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg_ilu: size mismatch");
     prepare_x(engine, (int)B.size(), X);
@@ -172,6 +190,20 @@ int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &st
                                  stop.max_iter, &it, nullptr, 0),
                "solve_batch_cg_ilu()");
     return it;
+}
+template <class VP, class VI, class VV, class T, class VB, class VX>
+int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                 VV const &vals, gpu_ilu<T> const &ilu, VB const &B, VX &X, std::false_type) {
+    const int nrhs = (int)(B.size() / (pntr.size() - 1));
+    return solve_batch_cg(engine, stop, pntr, indx, vals,
+                          [&](gpu_vector<T> const &in, gpu_vector<T> &out) { ilu.apply(in, out, nrhs); }, B, X);
+}
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+                       VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &B, VX &X) {
+    return detail::batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X, std::is_same<scalar_of<VV>, double>());
 }
 
 // solve_batch_cg_ilu(engine, stop, pntr, indx, vals, B, X, policy): factor on the fly (:238-249).
@@ -200,39 +232,65 @@ int solve_cg_device(gpu_engine const &engine, stop_criteria<double> const &stop,
 namespace detail {
 // reference BLAS drotg / drot / dtpsv('U','N','N') on the host (the reference
 // calls them on the CPU even with a GPU engine, hex/solvers/hala_solvers_gmres.hpp:196-222)
-inline void rotg(double &a, double &b, double &c, double &s) {
-    double sa = a, sb = b;
-    double roe = (std::fabs(sa) > std::fabs(sb)) ? sa : sb;
-    double scale = std::fabs(sa) + std::fabs(sb), r, z;
-    if (scale == 0.0) {
-        c = 1.0;
-        s = 0.0;
-        r = 0.0;
-        z = 0.0;
+template <typename R, typename = std::enable_if_t<std::is_floating_point<R>::value>>
+void rotg(R &a, R &b, R &c, R &s) {
+    R sa = a, sb = b;
+    R roe = (std::fabs(sa) > std::fabs(sb)) ? sa : sb;
+    R scale = std::fabs(sa) + std::fabs(sb), r, z;
+    if (scale == R(0)) {
+        c = R(1);
+        s = R(0);
+        r = R(0);
+        z = R(0);
     } else {
-        double as = sa / scale, bs = sb / scale;
+        R as = sa / scale, bs = sb / scale;
         r = scale * std::sqrt(as * as + bs * bs);
-        r = (roe >= 0.0) ? r : -r;
+        r = (roe >= R(0)) ? r : -r;
         c = sa / r;
         s = sb / r;
-        z = 1.0;
+        z = R(1);
         if (std::fabs(sa) > std::fabs(sb)) z = s;
-        if (std::fabs(sb) >= std::fabs(sa) && c != 0.0) z = 1.0 / c;
+        if (std::fabs(sb) >= std::fabs(sa) && c != R(0)) z = R(1) / c;
     }
     a = r;
     b = z;
 }
-inline void rot(double &x, double &y, double c, double s) {
-    double t = c * x + s * y;
+template <typename R, typename = std::enable_if_t<std::is_floating_point<R>::value>>
+void rot(R &x, R &y, R c, R s) {
+    R t = c * x + s * y;
     y = c * y - s * x;
     x = t;
 }
-inline void tpsv_un(int n, std::vector<double> const &ap, std::vector<double> &x) {
+// reference BLAS crotg / zrotg (real cosine, complex sine) and crot / zrot
+template <typename R>
+void rotg(std::complex<R> &a, std::complex<R> b, R &c, std::complex<R> &s) {
+    const R aa = std::abs(a);
+    if (aa == R(0)) {
+        c = R(0);
+        s = std::complex<R>(1);
+        a = b;
+        return;
+    }
+    const R scale = aa + std::abs(b);
+    const R norm = scale * std::sqrt(std::norm(a / scale) + std::norm(b / scale));
+    const std::complex<R> alpha = a / aa;
+    c = aa / norm;
+    s = alpha * std::conj(b) / norm;
+    a = alpha * norm;
+}
+template <typename R>
+void rot(std::complex<R> &x, std::complex<R> &y, R c, std::complex<R> s) {
+    const std::complex<R> t = c * x + s * y;
+    y = c * y - std::conj(s) * x;
+    x = t;
+}
+template <typename T>
+void tpsv_un(int n, std::vector<T> const &ap, std::vector<T> &x) {
     int kk = n * (n + 1) / 2 - 1;
     for (int j = n - 1; j >= 0; j--) {
-        if (x[j] != 0.0) {
+        if (x[j] != T(0)) {
             x[j] = x[j] / ap[kk];
-            double t = x[j];
+            T t = x[j];
             int k = kk - 1;
             for (int i = j - 1; i >= 0; i--) {
                 x[i] = x[i] - t * ap[k];
@@ -242,22 +300,57 @@ inline void tpsv_un(int n, std::vector<double> const &ap, std::vector<double> &x
         kk -= j + 1;
     }
 }
+// Krylov projection: coeffs = W_k^H r, r -= W_k coeffs (classical Gram-Schmidt,
+// hex/solvers/hala_solvers_gmres.hpp:47-77); FP64 as two dense gemv calls
+inline std::vector<double> project(gpu_engine const &engine, int n, int k, gpu_vector<double> &W,
+                                   gpu_vector<double> &r, gpu_vector<double> &cd) {
+    auto Wk = wrap_gpu_array(W.data(), (size_t)n * k, engine.device());
+    cd.resize(k);
+    gemv(engine, 'T', n, k, 1.0, Wk, r, 0.0, cd);
+    gemv(engine, 'N', n, k, -1.0, Wk, cd, 1.0, r);
+    return cd.unload();
+}
+template <typename T>
+std::vector<T> project(gpu_engine const &engine, int n, int k, gpu_vector<T> &W, gpu_vector<T> &r,
+                       gpu_vector<T> &) {
+    std::vector<T> c(k);
+    for (int i = 0; i < k; i++) c[i] = dot(engine, wrap_gpu_array(W.data() + (size_t)i * n, (size_t)n, engine.device()), r);
+    for (int i = 0; i < k; i++)
+        axpy(engine, -c[i], wrap_gpu_array(W.data() + (size_t)i * n, (size_t)n, engine.device()), r);
+    return c;
+}
+// x += W_k z
+template <class VX>
+void combine(gpu_engine const &engine, int n, gpu_vector<double> &W, std::vector<double> const &Z,
+             gpu_vector<double> &zd, VX &x) {
+    zd.load(Z);
+    auto Wk = wrap_gpu_array(W.data(), (size_t)n * Z.size(), engine.device());
+    gemv(engine, 'N', n, (int)Z.size(), 1.0, Wk, zd, 1.0, x);
+}
+template <typename T, class VX>
+void combine(gpu_engine const &engine, int n, gpu_vector<T> &W, std::vector<T> const &Z, gpu_vector<T> &, VX &x) {
+    for (size_t i = 0; i < Z.size(); i++)
+        axpy(engine, Z[i], wrap_gpu_array(W.data() + i * n, (size_t)n, engine.device()), x);
+}
 }  // namespace detail
 
 // solve_gmres(engine, stop, restart, pntr, indx, vals, precon, b, x):
 // hex/solvers/hala_solvers_gmres.hpp:125-230 (host Givens, device vectors).
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                VI const &indx, VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+                VI const &indx, VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+    using T = scalar_of<VV>;
+    using R = decltype(std::abs(T(0)));
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n && restart > 0, "solve_gmres: bad arguments");
     prepare_x(engine, n, x);
     auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
-    std::vector<double> H, S, C, Z;
-    double inner_res = 0.0, outer_res = stop.tol + 1.0;
+    std::vector<T> H, S, Z;
+    std::vector<R> C;
+    R inner_res = 0, outer_res = R(stop.tol + 1.0);
     int total = 0, outer = 0;
-    gpu_vector<double> t(n, engine.device()), r(n, engine.device()), W((size_t)n * restart, engine.device());
-    gpu_vector<double> cd(engine.device()), zd(engine.device());
+    gpu_vector<T> t(n, engine.device()), r(n, engine.device()), W((size_t)n * restart, engine.device());
+    gpu_vector<T> cd(engine.device()), zd(engine.device());
     while (outer_res > stop.tol && outer < stop.max_iter) {
         H.clear();
         S.clear();
@@ -268,40 +361,35 @@ int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int
         precon(t, r);
         total++;
         inner_res = norm2(engine, r);
-        scal(engine, 1.0 / inner_res, r);
-        Z.push_back(inner_res);
-        check_hala(hala_copy_f64(engine.handle(), n, r.data(), W.data()), "solve_gmres()");
+        scal(engine, R(1) / inner_res, r);
+        Z.push_back(T(inner_res));
+        detail::copy_(engine.handle(), n, (T const *)r.data(), (T *)W.data());
         int k = 0;
         while (inner_res > stop.tol && k < restart) {
             matrix.gemv('N', 1.0, r, 0.0, t);
             precon(t, r);
             total++;
-            auto Wk = wrap_gpu_array(W.data(), (size_t)n * (k + 1), engine.device());
-            cd.resize(k + 1);
-            gemv(engine, 'T', n, k + 1, 1.0, Wk, r, 0.0, cd);
-            gemv(engine, 'N', n, k + 1, -1.0, Wk, cd, 1.0, r);
-            std::vector<double> coeffs = cd.unload();
-            double nrm = norm2(engine, r);
-            scal(engine, 1.0 / nrm, r);
+            std::vector<T> coeffs = detail::project(engine, n, k + 1, W, r, cd);
+            R nrm = norm2(engine, r);
+            scal(engine, R(1) / nrm, r);
             for (int i = 0; i < k; i++) detail::rot(coeffs[i], coeffs[i + 1], C[i], S[i]);
-            double c, s, beta = nrm;
+            T s, beta = T(nrm);
+            R c;
             detail::rotg(coeffs[k], beta, c, s);
             H.insert(H.end(), coeffs.begin(), coeffs.end());
             S.push_back(s);
             C.push_back(c);
-            inner_res = std::fabs(S.back() * Z.back());
+            inner_res = std::abs(S.back() * Z.back());
             k++;
             if (inner_res > stop.tol && k < restart) {
-                check_hala(hala_copy_f64(engine.handle(), n, r.data(), W.data() + (size_t)k * n), "solve_gmres()");
-                Z.push_back(0.0);
+                detail::copy_(engine.handle(), n, (T const *)r.data(), (T *)W.data() + (size_t)k * n);
+                Z.push_back(T(0));
                 detail::rot(Z[k - 1], Z[k], C.back(), S.back());
             }
         }
         if (!H.empty()) {
             detail::tpsv_un((int)Z.size(), H, Z);
-            zd.load(Z);
-            auto Wk = wrap_gpu_array(W.data(), (size_t)n * Z.size(), engine.device());
-            gemv(engine, 'N', n, (int)Z.size(), 1.0, Wk, zd, 1.0, x);
+            detail::combine(engine, n, W, Z, zd, x);
         }
         outer++;
         outer_res = inner_res;
@@ -311,9 +399,10 @@ int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int
 
 // solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x):
 // hex/solvers/hala_solvers_gmres.hpp:250-267, device-resident (hala_gmres_f64).
+namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                    VI const &indx, VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x) {
+int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+              VI const &indx, VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_gmres_ilu: size mismatch");
     prepare_x(engine, n, x);
@@ -324,6 +413,20 @@ int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop,
                               nullptr, 0, nullptr),
                "solve_gmres_ilu()");
     return total;
+}
+template <class VP, class VI, class VV, class T, class VB, class VX>
+int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+              VI const &indx, VV const &vals, gpu_ilu<T> const &ilu, VB const &b, VX &x, std::false_type) {
+    return solve_gmres(engine, stop, restart, pntr, indx, vals,
+                       [&](gpu_vector<T> const &in, gpu_vector<T> &out) { ilu.apply(in, out); }, b, x);
+}
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+                    VI const &indx, VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
+    return detail::gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x,
+                             std::is_same<scalar_of<VV>, double>());
 }
 
 template <class VP, class VI, class VV, class VB, class VX>
@@ -370,7 +473,7 @@ int mixed_solve(mixed_engine const &engine, VP const &pntr, VI const &indx, VV c
 
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-             VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+             VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);
@@ -379,7 +482,7 @@ int solve_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP c
 }
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                 VV const &vals, mixed_ilu<double> const &ilu, VB const &b, VX &x) {
+                 VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);
@@ -397,7 +500,7 @@ int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, 
 }
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_batch_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                   VV const &vals, preconditioner<double> precon, VB const &B, VX &X) {
+                   VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &B, VX &X) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(B);
         auto gx = gpu_bind_vector(engine.gpu(), X);
@@ -406,7 +509,7 @@ int solve_batch_cg(mixed_engine const &engine, stop_criteria<double> const &stop
 }
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr,
-                       VI const &indx, VV const &vals, mixed_ilu<double> const &ilu, VB const &B, VX &X) {
+                       VI const &indx, VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &B, VX &X) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(B);
         auto gx = gpu_bind_vector(engine.gpu(), X);
@@ -424,7 +527,7 @@ int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &
 }
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_gmres(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                VI const &indx, VV const &vals, preconditioner<double> precon, VB const &b, VX &x) {
+                VI const &indx, VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);
@@ -433,7 +536,7 @@ int solve_gmres(mixed_engine const &engine, stop_criteria<double> const &stop, i
 }
 template <class VP, class VI, class VV, class VB, class VX>
 int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                    VI const &indx, VV const &vals, mixed_ilu<double> const &ilu, VB const &b, VX &x) {
+                    VI const &indx, VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);

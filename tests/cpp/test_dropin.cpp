@@ -4,6 +4,7 @@
 // tests/solvers_tests.hpp, tests/cuda_sparse_tests.hpp) and the GPU example's
 // PCG table (examples/example_gpu.cpp:90-98, :363-365).
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -38,6 +39,85 @@ static std::vector<double> densemul(std::vector<int> const &p, std::vector<int> 
             else y[r] += v[j] * x[ix[j]];
         }
     return y;
+}
+
+// tests/solvers_tests.hpp:12-137 for one scalar type: CG, batch CG and GMRES
+// with ILU(0) on the reference's 5x5 systems (complex: complex-valued x).
+template <typename T>
+static double errnorm(std::vector<T> const &x, std::vector<T> const &xref) {
+    double e = 0, r = 0;
+    for (size_t i = 0; i < xref.size(); i++) {
+        e += std::norm(std::complex<double>(x[i]) - std::complex<double>(xref[i]));
+        r += std::norm(std::complex<double>(xref[i]));
+    }
+    return x.size() == xref.size() ? std::sqrt(e / r) : 1e300;
+}
+template <typename T>
+static std::vector<T> tmul(std::vector<int> const &p, std::vector<int> const &ix, std::vector<T> const &v,
+                           std::vector<T> const &x, int nrhs = 1) {
+    int n = (int)p.size() - 1;
+    std::vector<T> y((size_t)n * nrhs, T(0));
+    for (int k = 0; k < nrhs; k++)
+        for (int r = 0; r < n; r++)
+            for (int j = p[r]; j < p[r + 1]; j++) y[k * n + r] += v[j] * x[k * n + ix[j]];
+    return y;
+}
+template <typename T>
+struct mkv {
+    static T get(double re, double) { return T(re); }
+};
+template <typename R>
+struct mkv<std::complex<R>> {
+    static std::complex<R> get(double re, double im) { return {R(re), R(im)}; }
+};
+template <typename T>
+static void typed_solvers(hala::gpu_engine const &engine, const char *name, double tol) {
+    const bool single = std::is_same<T, float>::value || std::is_same<T, std::complex<float>>::value;
+    const double stop = single ? 1.E-4 : 1.E-9;
+    std::vector<int> pntr = {0, 2, 5, 8, 11, 13};
+    std::vector<int> indx = {0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4};
+    std::vector<T> vals = {2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0};
+    std::vector<T> xref(15);
+    for (int i = 0; i < 15; i++) xref[i] = mkv<T>::get(1.0 + i, 0.5 * i - 3.0);
+    std::vector<T> x5(xref.begin(), xref.begin() + 5);
+    auto run = [&](std::vector<int> const &p, std::vector<int> const &ix, std::vector<T> const &v, bool exact) {
+        auto gp = hala::make_gpu_vector(p);
+        auto gi = hala::make_gpu_vector(ix);
+        auto gv = hala::make_gpu_vector(v);
+        auto ilu = hala::make_ilu(engine, gp, gi, gv, 'N');
+        auto gb = hala::make_gpu_vector(tmul(p, ix, v, x5));
+        std::vector<T> x;
+        hala::gpu_vector<T> gx;
+        // CG with the ILU as preconditioner lambda
+        if (exact) {
+            hala::solve_cg(engine, 2, gp, gi, gv,
+                           [&](hala::gpu_vector<T> const &in, hala::gpu_vector<T> &out) { hala::ilu_apply(ilu, in, out); },
+                           gb, gx);
+            gx.unload(x);
+            CHECK(errnorm(x, x5) < tol);
+        }
+        // CG and GMRES with the ILU object, then three right-hand sides
+        gx = hala::gpu_vector<T>();
+        hala::solve_cg_ilu(engine, {stop, 50}, gp, gi, gv, ilu, gb, gx);
+        gx.unload(x);
+        CHECK(errnorm(x, x5) < tol * 100);
+        gx = hala::gpu_vector<T>();
+        hala::solve_gmres_ilu(engine, exact ? hala::stop_criteria<double>(2) : hala::stop_criteria<double>(stop), exact ? 2 : 6,
+                              gp, gi, gv, ilu, gb, gx);
+        gx.unload(x);
+        CHECK(errnorm(x, x5) < tol * 100);
+        auto gB = hala::make_gpu_vector(tmul(p, ix, v, xref, 3));
+        hala::gpu_vector<T> gX;
+        hala::solve_batch_cg_ilu(engine, {stop, 50}, gp, gi, gv, ilu, gB, gX);
+        gX.unload(x);
+        CHECK(errnorm(x, xref) < tol * 100);
+    };
+    run(pntr, indx, vals, true);  // tri-diagonal: ILU(0) is exact
+    pntr = {0, 3, 6, 9, 12, 15};
+    indx = {0, 1, 4, 0, 1, 2, 1, 2, 3, 2, 3, 4, 0, 3, 4};
+    vals = {2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 2.0};
+    run(pntr, indx, vals, false);
+    std::printf("PASS solvers<%s>\n", name);
 }
 
 int main() {
@@ -286,6 +366,11 @@ int main() {
         CHECK(X.size() == 15 && diffsum(std::vector<double>(X.begin() + 10, X.end()), xref) < 1e-12);
         std::printf("PASS mixed_engine\n");
     }
+    // the four scalar types of tests/solvers_tests.cpp:14-40
+    typed_solvers<double>(engine, "double", 1e-13);
+    typed_solvers<float>(engine, "float", 1e-5);
+    typed_solvers<std::complex<double>>(engine, "complex<double>", 1e-13);
+    typed_solvers<std::complex<float>>(engine, "complex<float>", 1e-5);
     // tests/solvers_tests.hpp:105-137 (GMRES)
     {
         std::vector<int> pntr = {0, 2, 5, 8, 11, 12};
