@@ -10,7 +10,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/hala_amd.h"
 
@@ -141,7 +143,35 @@ struct hala_ctx_s {
     hala_amd::dbuf vec;      // solver work vectors
     hala_amd::dbuf lbc;      // last-block arrival counters (zeroed at creation, self re-arming)
     double *host_pinned = nullptr;  // pinned host landing zone for scalars
+    // temporaries of the analysis passes (see tmp_scope)
+    std::vector<std::unique_ptr<hala_amd::dbuf>> tmp;
+    size_t tmp_top = 0;
+    // second context on a stream of its own, for analysis work run beside this
+    // context's (created on first use, destroyed with this context)
+    hala_ctx_s *aux = nullptr;
 };
+
+namespace hala_amd {
+// Temporary device buffers of one library call, kept by the context and reused
+// by later calls instead of allocated and freed each time (hipFree waits for
+// the whole device).  Every user runs on the context's stream, so a reuse is
+// ordered after the previous one.  Scopes nest (last in, first out).
+class tmp_scope {
+  public:
+    explicit tmp_scope(hala_ctx_s *c) : c_(c), mark_(c->tmp_top) {}
+    ~tmp_scope() { c_->tmp_top = mark_; }
+    tmp_scope(const tmp_scope &) = delete;
+    tmp_scope &operator=(const tmp_scope &) = delete;
+    dbuf &get() {
+        if (c_->tmp_top == c_->tmp.size()) c_->tmp.emplace_back(new dbuf());
+        return *c_->tmp[c_->tmp_top++];
+    }
+
+  private:
+    hala_ctx_s *c_;
+    size_t mark_;
+};
+}  // namespace hala_amd
 
 namespace hala_amd {
 
@@ -156,6 +186,10 @@ enum : int {
 };
 
 inline void set_device(hala_ctx c) { HALA_HIP(hipSetDevice(c->device)); }
+
+// The context's auxiliary context (same device, options copied, its own
+// non-blocking stream of default priority), created on first use.
+hala_ctx aux_ctx(hala_ctx c);
 
 // Bit patterns never produced by arithmetic: "not yet computed" markers for
 // value-as-flag dependency waits (a signalling NaN with a fixed payload).

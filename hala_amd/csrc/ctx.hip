@@ -66,6 +66,23 @@ float *partials32(hala_ctx c, size_t count) { return partials<float>(c, c->red32
 
 }  // namespace hala_amd
 
+hala_ctx hala_amd::aux_ctx(hala_ctx c) {
+    if (!c->aux) {
+        set_device(c);
+        hipStream_t s = nullptr;
+        HALA_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hala_ctx a = nullptr;
+        if (hala_ctx_create(c->device, s, &a) != HALA_OK) {
+            (void)hipStreamDestroy(s);
+            throw status_error{HALA_ERR_HIP, "auxiliary context: " + std::string(hala_last_error())};
+        }
+        a->own_stream = true;
+        c->aux = a;
+    }
+    c->aux->opt = c->opt;
+    return c->aux;
+}
+
 using namespace hala_amd;
 
 extern "C" {
@@ -124,7 +141,9 @@ int hala_ctx_destroy(hala_ctx c) {
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        if (c->aux) (void)hala_ctx_destroy(c->aux);
         if (c->host_pinned) (void)hipHostFree(c->host_pinned);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
     });
 }

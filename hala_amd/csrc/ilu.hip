@@ -3,6 +3,8 @@
 // and rocsparse_?csrsv of gpu/hala_gpu_ilu.hpp:81-172).
 #include <climits>
 #include <cmath>
+#include <exception>
+#include <thread>
 
 #include "ilu.hpp"
 
@@ -130,10 +132,39 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
             M->diag.reserve((size_t)n * sizeof(int));
             diag_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, pntr, indx, M->diag.as<int>());
             HALA_CHECK_LAUNCH();
-            build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
-            build_schedule(c, M->L, false);
-            build_deps(c, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
-            build_schedule(c, M->U, false);
+            // The two analyses are independent and mostly latency-bound: U's runs
+            // on the auxiliary context's stream, from a second host thread.
+            hala_ctx a = aux_ctx(c);
+            hipEvent_t ready = nullptr;
+            HALA_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            std::exception_ptr uerr;
+            try {
+                HALA_HIP(hipEventRecord(ready, c->stream));
+                HALA_HIP(hipStreamWaitEvent(a->stream, ready, 0));
+                std::thread ut([&] {
+                    try {
+                        set_device(a);
+                        build_deps(a, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
+                        build_schedule(a, M->U, false);
+                        HALA_HIP(hipStreamSynchronize(a->stream));
+                    } catch (...) {
+                        uerr = std::current_exception();
+                    }
+                });
+                try {
+                    build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
+                    build_schedule(c, M->L, false);
+                } catch (...) {
+                    ut.join();
+                    throw;
+                }
+                ut.join();
+            } catch (...) {
+                (void)hipEventDestroy(ready);
+                throw;
+            }
+            (void)hipEventDestroy(ready);
+            if (uerr) std::rethrow_exception(uerr);
             M->factor.reserve((size_t)nnz * sizeof(double));
             M->tmp64.reserve((size_t)n * sizeof(double));
         } catch (...) {

@@ -366,7 +366,8 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     if (n < 2 * WAVE || nlev < 2) return false;
     int maxw = 0;
     for (int l = 0; l < nlev; l++) maxw = std::max(maxw, p.h_level_ptr[l + 1] - p.h_level_ptr[l]);
-    dbuf lens;
+    tmp_scope ts(c);
+    dbuf &lens = ts.get();
     lens.reserve((size_t)n * sizeof(int));
     row_lens<<<(n + 255) / 256, 256, 0, c->stream>>>(n, p.dptr.as<int>(), lens.as<int>());
     HALA_CHECK_LAUNCH();
@@ -374,7 +375,9 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     p.col.why = 2;
     if (maxlen > MAXD || maxlen < 1) return false;
     const int g = (n + 255) / 256;
-    dbuf cellcnt, gap, cstart, ecnt, epos, ci, colrange, maxext, cmap;
+    dbuf &cellcnt = ts.get(), &gap = ts.get(), &cstart = ts.get(), &ecnt = ts.get(), &epos = ts.get();
+    dbuf &maxext = ts.get(), &cmap = ts.get();
+    dbuf ci, colrange;  // handed to the plan
     gap.reserve(sizeof(int));
     maxext.reserve(sizeof(int));
     cmap.reserve((size_t)n * sizeof(int));
@@ -512,7 +515,8 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         HALA_CHECK_LAUNCH();
         // each external value staged once per cell
         {
-            dbuf uidx, fail;
+            tmp_scope ts2(c);
+            dbuf &uidx = ts2.get(), &fail = ts2.get();
             uidx.reserve((size_t)(next > 0 ? next : 1) * sizeof(int));
             fail.reserve(sizeof(int));
             HALA_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), c->stream));

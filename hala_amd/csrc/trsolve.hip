@@ -91,7 +91,8 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
     const int *tpntr = tp ? tp->tpntr.as<int>() : nullptr;
     const int *trow = tp ? tp->trow.as<int>() : nullptr;
     const int *tmap = tp ? tp->tmap.as<int>() : nullptr;
-    dbuf cnt;
+    tmp_scope ts(c);
+    dbuf &cnt = ts.get();
     cnt.reserve((size_t)(n + 1) * sizeof(int));
     p.dptr.reserve((size_t)(n + 1) * sizeof(int));
     int g = (n + 255) / 256;
@@ -267,7 +268,8 @@ __global__ void level_key(int n, int nkey, const int *__restrict__ lev, const in
 void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey) {
     const int n = p.n;
     HALA_REQUIRE((int64_t)p.nlev * nkey < INT_MAX, HALA_ERR_ARG, "triangular plan: too many level keys");
-    dbuf bk, bptr, order;
+    tmp_scope ts(c);
+    dbuf &bk = ts.get(), &bptr = ts.get(), &order = ts.get();
     bk.reserve((size_t)n * sizeof(int));
     bptr.reserve((size_t)((int64_t)p.nlev * nkey + 1) * sizeof(int));
     order.reserve((size_t)n * sizeof(int));
@@ -302,7 +304,8 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     HALA_HIP(hipMemcpyAsync(p.h_level_ptr.data(), p.level_ptr.p, (size_t)(p.nlev + 1) * sizeof(int),
                             hipMemcpyDeviceToHost, c->stream));
     // chunks: ceil(level size / 64) per level
-    dbuf nch;
+    tmp_scope ts(c);
+    dbuf &nch = ts.get();
     dbuf &chunk_ptr = p.lvl_chunk;
     nch.reserve((size_t)(p.nlev + 1) * sizeof(int));
     chunk_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));

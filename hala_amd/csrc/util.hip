@@ -82,7 +82,8 @@ void exclusive_scan_i32(hala_ctx c, const int *in, int *out, int n) {
         return;
     }
     int nb = (n + SCAN_B - 1) / SCAN_B;
-    dbuf sums;
+    tmp_scope ts(c);
+    dbuf &sums = ts.get();
     sums.reserve((size_t)(nb + 1) * sizeof(int));
     scan_block_sums<<<nb, 256, 0, c->stream>>>(in, n, sums.as<int>());
     HALA_CHECK_LAUNCH();
@@ -90,7 +91,6 @@ void exclusive_scan_i32(hala_ctx c, const int *in, int *out, int n) {
     HALA_CHECK_LAUNCH();
     scan_apply<<<nb, 256, 0, c->stream>>>(in, n, sums.as<int>(), out);
     HALA_CHECK_LAUNCH();
-    HALA_HIP(hipStreamSynchronize(c->stream));  // sums is released on return
 }
 
 __global__ void count_buckets(int nitems, const int *__restrict__ bucket, int *__restrict__ cnt) {
@@ -167,7 +167,8 @@ __global__ __launch_bounds__(256) void sort_large_buckets(int nbuckets, const in
 
 void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *bptr, int *items) {
     HALA_REQUIRE(nitems >= 0 && nbuckets >= 1, HALA_ERR_ARG, "bucket_sort: bad sizes");
-    dbuf cnt, cur;
+    tmp_scope ts(c);
+    dbuf &cnt = ts.get(), &cur = ts.get();
     cnt.reserve((size_t)(nbuckets + 1) * sizeof(int));
     cur.reserve((size_t)(nbuckets + 1) * sizeof(int));
     HALA_HIP(hipMemsetAsync(cnt.p, 0, (size_t)(nbuckets + 1) * sizeof(int), c->stream));
@@ -185,13 +186,13 @@ void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *b
     // the largest bucket (sizes are cnt[]) decides whether the block-per-bucket pass is needed
     int maxlen = reduce_max_i32(c, cnt.as<int>(), nbuckets);
     if (maxlen > SMALL) {
-        dbuf scratch;
+        dbuf &scratch = ts.get();
         int p2 = 1;
         while (p2 < maxlen) p2 <<= 1;
         if (p2 > LDS_SORT) scratch.reserve((size_t)2 * nitems * sizeof(int) + (size_t)p2 * sizeof(int));
-        sort_large_buckets<<<nbuckets, 256, 0, c->stream>>>(nbuckets, bptr, items, scratch.as<int>());
+        sort_large_buckets<<<nbuckets, 256, 0, c->stream>>>(nbuckets, bptr, items,
+                                                            p2 > LDS_SORT ? scratch.as<int>() : nullptr);
         HALA_CHECK_LAUNCH();
-        HALA_HIP(hipStreamSynchronize(c->stream));
     }
 }
 
