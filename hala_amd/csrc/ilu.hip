@@ -1,6 +1,7 @@
 // ilu.hip -- ILU(0) on the device: symbolic analysis, numeric factorisation
 // level by level, and the triangular applies (replacing rocsparse_?csrilu0
 // and rocsparse_?csrsv of gpu/hala_gpu_ilu.hpp:81-172).
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <exception>
@@ -19,35 +20,70 @@ __global__ void diag_kernel(int n, const int *__restrict__ pntr, const int *__re
     diag[i] = d;
 }
 
-// IKJ ILU(0) of the rows of one level: for each strictly lower entry (j,i) in
+// IKJ ILU(0), level by level: for each strictly lower entry (j,i) of row j in
 // column order, a_ji /= u_ii, then a_jk -= a_ji * u_ik over the columns k > i
 // both rows hold (merge of the sorted rows).  Rows i of earlier levels are
 // final, so each entry sees exactly the operation sequence of the reference's
 // KIJ loop (sparse/hala_sparse_utils.hpp:229-254).
-__global__ void ilu_level_kernel(const int *__restrict__ rows, int count, const int *__restrict__ pntr,
-                                 const int *__restrict__ indx, const int *__restrict__ diag, double *f) {
+//
+// Symbolic part of the elimination, once per pattern: for every strictly lower
+// entry jc = (j,i) of row j, the position of u_ii and the (ik, jk) position
+// pairs its update touches (columns k > i both rows hold, in merge order).
+__global__ void ilu_pair_count(int n, const int *__restrict__ pntr, const int *__restrict__ indx,
+                               const int *__restrict__ diag, int *__restrict__ cnt) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int je = pntr[j + 1], dj = diag[j];
+    for (int jc = pntr[j]; jc < je; jc++) {
+        int m = 0;
+        if (jc < dj) {
+            const int i = indx[jc], ie = pntr[i + 1];
+            int ik = diag[i] + 1, jk = jc + 1;
+            while (ik < ie && jk < je) {
+                const int ci = indx[ik], cj = indx[jk];
+                m += ci == cj;
+                ik += ci <= cj;
+                jk += cj <= ci;
+            }
+        }
+        cnt[jc] = m;
+    }
+}
+
+__global__ void ilu_pair_fill(int n, const int *__restrict__ pntr, const int *__restrict__ indx,
+                              const int *__restrict__ diag, const int *__restrict__ pptr, int *__restrict__ dpos,
+                              int2 *__restrict__ pairs) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int je = pntr[j + 1], dj = diag[j];
+    for (int jc = pntr[j]; jc < dj; jc++) {
+        const int i = indx[jc], ie = pntr[i + 1];
+        dpos[jc] = diag[i];
+        int ik = diag[i] + 1, jk = jc + 1, o = pptr[jc];
+        while (ik < ie && jk < je) {
+            const int ci = indx[ik], cj = indx[jk];
+            if (ci == cj) pairs[o++] = make_int2(ik, jk);
+            ik += ci <= cj;
+            jk += cj <= ci;
+        }
+    }
+}
+
+// The rows of one level from the symbolic pairs: the operations above in their
+// order, with no index search.
+__global__ void ilu_level_pairs(const int *__restrict__ rows, int count, const int *__restrict__ pntr,
+                                const int *__restrict__ diag, const int *__restrict__ dpos,
+                                const int *__restrict__ pptr, const int2 *__restrict__ pairs, double *f) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int j = rows[t];
-    const int je = pntr[j + 1];
     const int dj = diag[j];
     for (int jc = pntr[j]; jc < dj; jc++) {
-        const int i = indx[jc];
-        const double l = f[jc] / f[diag[i]];
+        const double l = f[jc] / f[dpos[jc]];
         f[jc] = l;
-        int ik = diag[i] + 1, jk = jc + 1;
-        const int ie = pntr[i + 1];
-        while (ik < ie && jk < je) {
-            const int ci = indx[ik], cj = indx[jk];
-            if (ci == cj) {
-                f[jk] = f[jk] - l * f[ik];
-                ik++;
-                jk++;
-            } else if (ci < cj) {
-                ik++;
-            } else {
-                jk++;
-            }
+        for (int p = pptr[jc]; p < pptr[jc + 1]; p++) {
+            const int2 q = pairs[p];
+            f[q.y] = f[q.y] - l * f[q.x];
         }
     }
 }
@@ -112,6 +148,24 @@ using namespace hala_amd;
 
 extern "C" {
 
+static void ilu_pairs(hala_ctx c, hala_ilu M) {
+    const int n = M->n, g = (n + 255) / 256;
+    const int64_t nnz = M->nnz;
+    tmp_scope ts(c);
+    dbuf &cnt = ts.get();
+    cnt.reserve((size_t)nnz * sizeof(int));
+    M->pptr.reserve((size_t)(nnz + 1) * sizeof(int));
+    M->dpos.reserve((size_t)nnz * sizeof(int));
+    ilu_pair_count<<<g, 256, 0, c->stream>>>(n, M->pntr, M->indx, M->diag.as<int>(), cnt.as<int>());
+    HALA_CHECK_LAUNCH();
+    exclusive_scan_i32(c, cnt.as<int>(), M->pptr.as<int>(), (int)nnz);
+    const int np = fetch_i32(c, M->pptr.as<int>() + nnz);
+    M->pairs.reserve((size_t)std::max(np, 1) * sizeof(int2));
+    ilu_pair_fill<<<g, 256, 0, c->stream>>>(n, M->pntr, M->indx, M->diag.as<int>(), M->pptr.as<int>(),
+                                            M->dpos.as<int>(), M->pairs.as<int2>());
+    HALA_CHECK_LAUNCH();
+}
+
 int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *indx, hala_ilu *out) {
     return guard([&] {
         HALA_REQUIRE(c && out, HALA_ERR_ARG, "hala_ilu_create: null argument");
@@ -154,6 +208,7 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                 try {
                     build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
                     build_schedule(c, M->L, false);
+                    ilu_pairs(c, M);
                 } catch (...) {
                     ut.join();
                     throw;
@@ -225,8 +280,9 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         for (int l = 1; l < M->L.nlev; l++) {  // level 0 rows have no strictly lower entries
             int cnt = lp[l + 1] - lp[l];
             if (cnt <= 0) continue;
-            ilu_level_kernel<<<(cnt + 255) / 256, 256, 0, c->stream>>>(perm + lp[l], cnt, M->pntr, M->indx,
-                                                                      M->diag.as<int>(), f);
+            ilu_level_pairs<<<(cnt + 255) / 256, 256, 0, c->stream>>>(perm + lp[l], cnt, M->pntr, M->diag.as<int>(),
+                                                                     M->dpos.as<int>(), M->pptr.as<int>(),
+                                                                     M->pairs.as<int2>(), f);
             HALA_CHECK_LAUNCH();
         }
         int *bad = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 6);
