@@ -10,7 +10,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,46 +66,111 @@ int guard(F &&body) {
 // ---------------------------------------------------------------------------
 // device buffers owned by the library
 // ---------------------------------------------------------------------------
+// Device blocks kept for reuse by one context: an ILU analysis allocates some
+// hundred buffers and its destruction frees them, and each hipMalloc / hipFree
+// costs tens to hundreds of microseconds (hipFree also waits for the whole
+// device).  Inside an alloc_scope, buffers are taken from and returned to the
+// scope's cache.  A cache serves one stream: a block released while kernels
+// on that stream may still read it is handed out again only to later work on
+// the same stream, and owners synchronise the stream before their final
+// release (an ILU destroy does).
+struct buf_cache {
+    std::mutex m;
+    std::multimap<size_t, void *> blocks;  // size -> block
+    size_t held = 0;
+    static constexpr size_t CAP = size_t(16) << 30;  // bytes kept at most
+    // a block of at least b bytes and at most 2b, or nullptr
+    void *take(size_t b, size_t *got) {
+        std::lock_guard<std::mutex> g(m);
+        auto it = blocks.lower_bound(b);
+        if (it == blocks.end() || it->first > 2 * b) return nullptr;
+        void *p = it->second;
+        *got = it->first;
+        held -= it->first;
+        blocks.erase(it);
+        return p;
+    }
+    void put(void *p, size_t b) {
+        {
+            std::lock_guard<std::mutex> g(m);
+            if (held + b <= CAP) {
+                blocks.emplace(b, p);
+                held += b;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+    ~buf_cache() {
+        for (auto &kv : blocks) (void)hipFree(kv.second);
+    }
+};
+
+inline thread_local buf_cache *tl_cache = nullptr;
+
+struct alloc_scope {
+    buf_cache *c0;
+    explicit alloc_scope(buf_cache *c) : c0(tl_cache) { tl_cache = c; }
+    ~alloc_scope() { tl_cache = c0; }
+    alloc_scope(const alloc_scope &) = delete;
+    alloc_scope &operator=(const alloc_scope &) = delete;
+};
+
 struct dbuf {
     void *p = nullptr;
     size_t bytes = 0;
+    buf_cache *cache = nullptr;  // where the block returns (nullptr: hipFree)
     dbuf() = default;
     dbuf(const dbuf &) = delete;
     dbuf &operator=(const dbuf &) = delete;
-    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    dbuf(dbuf &&o) noexcept : p(o.p), bytes(o.bytes), cache(o.cache) {
         o.p = nullptr;
         o.bytes = 0;
+        o.cache = nullptr;
     }
     dbuf &operator=(dbuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            cache = o.cache;
             o.p = nullptr;
             o.bytes = 0;
+            o.cache = nullptr;
         }
         return *this;
     }
     void swap(dbuf &o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(cache, o.cache);
     }
     ~dbuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cache) cache->put(p, bytes);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
+        cache = nullptr;
     }
     // grows (never shrinks); contents are not preserved
     void reserve(size_t b) {
         if (b <= bytes) return;
         release();
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw status_error{HALA_ERR_ALLOC, "hipMalloc of " + std::to_string(b) + " bytes failed"};
+        size_t got = b;
+        if (tl_cache) p = tl_cache->take(b, &got);
+        if (!p) {
+            hipError_t e = hipMalloc(&p, b);
+            if (e != hipSuccess) {
+                p = nullptr;
+                throw status_error{HALA_ERR_ALLOC, "hipMalloc of " + std::to_string(b) + " bytes failed"};
+            }
+            got = b;
         }
-        bytes = b;
+        cache = tl_cache;
+        bytes = got;
     }
     template <class T>
     T *as() const {
@@ -149,6 +216,8 @@ struct hala_ctx_s {
     // second context on a stream of its own, for analysis work run beside this
     // context's (created on first use, destroyed with this context)
     hala_ctx_s *aux = nullptr;
+    // blocks of the ILU objects built on this context, kept for the next ones
+    std::unique_ptr<hala_amd::buf_cache> cache{new hala_amd::buf_cache()};
 };
 
 namespace hala_amd {

@@ -118,6 +118,7 @@ int hala_ctx_create(int device, void *stream, hala_ctx *out) {
         c->stream = (hipStream_t)stream;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+
         try {
             c->scal.reserve(S_TOTAL * sizeof(double));
             HALA_HIP(hipMemsetAsync(c->scal.p, 0, S_TOTAL * sizeof(double), c->stream));
@@ -142,6 +143,8 @@ int hala_ctx_destroy(hala_ctx c) {
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         if (c->aux) (void)hala_ctx_destroy(c->aux);
+        c->tmp.clear();  // temporaries may live in the cache: release them first
+        c->cache.reset();
         if (c->host_pinned) (void)hipHostFree(c->host_pinned);
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;

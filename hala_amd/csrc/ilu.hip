@@ -177,6 +177,9 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                      "hala_ilu_create: row " + std::to_string(bad) +
                          " has unsorted/out-of-range columns or no diagonal entry");
         auto *M = new hala_ilu_s();
+        // the factor's buffers come from the context's cache; hala_ilu_destroy
+        // synchronises the context's stream before returning them
+        alloc_scope as(c->cache.get());
         try {
             M->ctx = c;
             M->n = n;
@@ -198,6 +201,7 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                 std::thread ut([&] {
                     try {
                         set_device(a);
+                        alloc_scope aas(a->cache.get());  // blocks reused on this stream only
                         build_deps(a, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
                         build_schedule(a, M->U, false);
                         HALA_HIP(hipStreamSynchronize(a->stream));
@@ -273,6 +277,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         HALA_REQUIRE(M && vals, HALA_ERR_ARG, "hala_ilu_factor_f64: null argument");
         hala_ctx c = M->ctx;
         set_device(c);
+        alloc_scope as(c->cache.get());
         double *f = M->factor.as<double>();
         HALA_HIP(hipMemcpyAsync(f, vals, (size_t)M->nnz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const std::vector<int> &lp = M->L.h_level_ptr;
