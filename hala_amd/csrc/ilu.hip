@@ -354,6 +354,7 @@ int hala_ilu_get_symbolic(hala_ilu M, int upper, int *diag, int *lev, int *level
         hala_ctx c = M->ctx;
         set_device(c);
         tri_plan &p = upper ? M->U : M->L;
+        if (perm) sort_perm(c, p);
         auto cp = [&](int *dst, const dbuf &src, size_t count) {
             if (dst) HALA_HIP(hipMemcpyAsync(dst, src.p, count * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
         };

@@ -456,7 +456,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         const int K = std::min(fetch_i32(c, gap.as<int>()) + 1, KMAX);
         // cells must be contiguous in every level's row list: the (level, row)
         // layout has them for the contiguous map; otherwise order rows by column
-        if (B > 1 || resorted) {
+        if (B > 1 || resorted || !p.perm_sorted) {
             reorder_levels(c, p, cmap.as<int>(), ncol);
             resorted = true;
         }

@@ -255,6 +255,7 @@ void lay_out_slots(hala_ctx c, tri_plan &p, const int *order) {
         p.ddep.as<int>(), p.dsrc.as<int>(), p.slot_of_row.as<int>(), p.diag_src.as<int>(), p.ent_dep.as<int>(),
         p.ent_src.as<int>(), p.dg_src.as<int>());
     HALA_CHECK_LAUNCH();
+    p.laid_out = true;
 }
 
 __global__ void level_key(int n, int nkey, const int *__restrict__ lev, const int *__restrict__ key,
@@ -276,7 +277,10 @@ void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey) {
     level_key<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nkey, p.lev.as<int>(), key, bk.as<int>());
     HALA_CHECK_LAUNCH();
     bucket_sort(c, n, bk.as<int>(), p.nlev * nkey, bptr.as<int>(), order.as<int>());
-    lay_out_slots(c, p, order.as<int>());
+    // the new order is also the plan's level order (rows of a cell stay together)
+    HALA_HIP(hipMemcpyAsync(p.perm.p, order.p, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    p.perm_sorted = false;
+    lay_out_slots(c, p, p.perm.as<int>());
     HALA_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -296,10 +300,13 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     HALA_CHECK_LAUNCH();
     check_wait_error(c);
     p.nlev = reduce_max_i32(c, p.lev.as<int>(), n) + 1;
-    // permutation sorted by (level, row)
+    // rows grouped by level; sorted by row inside a level unless the column
+    // layout (which orders every level its own way) will be tried
     p.level_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));
     p.perm.reserve((size_t)n * sizeof(int));
-    bucket_sort(c, n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>());
+    p.perm_sorted = !c->opt.tri_col;
+    p.laid_out = false;
+    bucket_sort(c, n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>(), p.perm_sorted);
     p.h_level_ptr.resize(p.nlev + 1);
     HALA_HIP(hipMemcpyAsync(p.h_level_ptr.data(), p.level_ptr.p, (size_t)(p.nlev + 1) * sizeof(int),
                             hipMemcpyDeviceToHost, c->stream));
@@ -314,12 +321,20 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     exclusive_scan_i32(c, nch.as<int>(), chunk_ptr.as<int>(), p.nlev);
     p.nchunk = fetch_i32(c, chunk_ptr.as<int>() + p.nlev);
     const int nslots = p.nchunk * WAVE;
-    lay_out_slots(c, p, p.perm.as<int>());
+    // the column layout lays the slots out in its own order; else (or when it
+    // gave up before) the level order does
+    if (c->opt.tri_col) build_col_plan(c, p, c->opt.tri_col_cols);
+    if (!p.laid_out) lay_out_slots(c, p, p.perm.as<int>());
     p.out64.reserve((size_t)nslots * sizeof(double));
     if (want32) p.out32.reserve((size_t)nslots * sizeof(float));
     tri_rearm(c, p);
     HALA_HIP(hipStreamSynchronize(c->stream));
-    if (c->opt.tri_col) build_col_plan(c, p, c->opt.tri_col_cols);
+}
+
+void sort_perm(hala_ctx c, tri_plan &p) {
+    if (p.perm_sorted) return;
+    bucket_sort(c, p.n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>(), true);
+    p.perm_sorted = true;
 }
 
 void tri_rearm_prec(hala_ctx c, tri_plan &p, bool is64) {

@@ -54,6 +54,8 @@ struct tri_plan {
     int n = 0;
     bool backward = false;
     bool unit = true;
+    bool perm_sorted = true;  // perm in (level, row) order (else rows of a level in any order)
+    bool laid_out = false;    // slot layout built
     int nlev = 0;
     int nchunk = 0;
     int64_t nent = 0;  // padded entries (64 per slab)
@@ -95,9 +97,13 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
 // Computes levels, permutation, chunks and entry layout; fills out64 with sentinels.
 void build_schedule(hala_ctx c, tri_plan &p, bool want32);
 
+// perm in (level, row) order (sorts it if the analysis left a level's rows unsorted).
+void sort_perm(hala_ctx c, tri_plan &p);
+
 // Re-orders the rows inside every level by (key[r], r), key in [0, nkey), and
-// rebuilds the slot layout; levels, level_ptr and perm ((level, row) order)
-// are unchanged.  Called before any values are loaded.
+// rebuilds the slot layout; perm takes the new order (sort_perm restores
+// (level, row)), levels and level_ptr are unchanged.  Called before any values
+// are loaded.
 void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey);
 
 // Gathers plan values from a factor value array (positions ent_src / diag_src).

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void sort_large_buckets(int nbuckets, const in
     for (int i = threadIdx.x; i < len; i += 256) items[s + i] = buf[i];
 }
 
-void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *bptr, int *items) {
+void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *bptr, int *items, bool sorted) {
     HALA_REQUIRE(nitems >= 0 && nbuckets >= 1, HALA_ERR_ARG, "bucket_sort: bad sizes");
     tmp_scope ts(c);
     dbuf &cnt = ts.get(), &cur = ts.get();
@@ -181,6 +181,7 @@ void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *b
     HALA_HIP(hipMemcpyAsync(cur.p, bptr, (size_t)nbuckets * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     scatter_buckets<<<(nitems + 255) / 256, 256, 0, c->stream>>>(nitems, bucket, cur.as<int>(), items);
     HALA_CHECK_LAUNCH();
+    if (!sorted) return;  // items of a bucket in scatter (arbitrary) order
     sort_small_buckets<<<(nbuckets + 255) / 256, 256, 0, c->stream>>>(nbuckets, bptr, items);
     HALA_CHECK_LAUNCH();
     // the largest bucket (sizes are cnt[]) decides whether the block-per-bucket pass is needed
