@@ -112,11 +112,16 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
 
 // x += a p; r += (-a) Ap; <r,r> (and <r, r./d> for Jacobi).  The last block
 // takes the stop decision and, for identity/Jacobi, the new beta.
+// lslot (ILU on the column layout, may be null): r also goes straight into the
+// L sweep's slot-order input (the sweep's gather pass, fused here; L's polled
+// output was re-armed by the previous apply's hand-off to U).
 template <int PREC>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__restrict__ p,
                                                         const double *__restrict__ ap, const double *__restrict__ d,
                                                         double *__restrict__ x, double *__restrict__ r, double *part,
-                                                        double *part2, unsigned *cnt, cg_state *st) {
+                                                        double *part2, unsigned *cnt, cg_state *st,
+                                                        const int *__restrict__ lslot = nullptr,
+                                                        double *__restrict__ lxs = nullptr) {
     if (st->done) return;
     __shared__ double sh[CB * EK * 2];
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
@@ -131,6 +136,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
             x[i] = x[i] + a * pi;
             double ri = r[i] + (-a) * ap[i];
             r[i] = ri;
+            if (lslot) lxs[lslot[i]] = ri;
             rr = ri * ri;
             if (PREC == PREC_JAC) rz = ri * (ri / d[i]);
         }
@@ -190,6 +196,42 @@ __global__ __launch_bounds__(256) void cg_rz_kernel(int n, const double *__restr
     for (int k = 0; k < EK; k++) {
         int64_t i = ((int64_t)blockIdx.x * EK + k) * CB + threadIdx.x;
         v[k] = (i < n) ? r[i] * z[i] : 0.0;
+    }
+    tree256<double, EK>(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < EK; k++) {
+            int64_t b = (int64_t)blockIdx.x * EK + k;
+            if (b < nb) st_agent(part + b, v[k]);
+        }
+    if (last_block(cnt, gridDim.x)) {
+        double t = final_pass<double>(part, nb, sh);
+        if (threadIdx.x == 0) {
+            st->beta = t / st->zr;
+            st->zr = t;
+        }
+    }
+}
+
+// z from the U sweep's slot-order output (its scatter pass, fused here) and
+// nzr = <r, z> in the canonical order; beta = nzr / zr; zr = nzr
+__global__ __launch_bounds__(256) void cg_zscatter_rz_kernel(int n, const int *__restrict__ uslot,
+                                                             const double *__restrict__ uout,
+                                                             const double *__restrict__ r, double *__restrict__ z,
+                                                             double *part, unsigned *cnt, cg_state *st) {
+    if (st->done) return;
+    __shared__ double sh[CB * EK];
+    const int64_t nb = ((int64_t)n + CB - 1) / CB;
+    double v[EK];
+#pragma unroll
+    for (int k = 0; k < EK; k++) {
+        int64_t i = ((int64_t)blockIdx.x * EK + k) * CB + threadIdx.x;
+        double a = 0.0;
+        if (i < n) {
+            const double zi = uout[uslot[i]];
+            z[i] = zi;
+            a = r[i] * zi;
+        }
+        v[k] = a;
     }
     tree256<double, EK>(v, sh);
     if (threadIdx.x == 0)
@@ -276,6 +318,7 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     cg_residual_kernel<false><<<gs, 256, 0, c->stream>>>(n, pntr, indx, vals, x, b, r);
     HALA_CHECK_LAUNCH();
     const int *gate = &st->done;
+    const bool fused = P.prec == PREC_ILU && !P.transposed && c->opt.cg_fuse && ilu_fused64(P.M);
     auto precond = [&](const double *in, double *out, const int *g) {
         if (P.transposed) ilu_apply_t64(P.M, in, out, g);
         else ilu_apply64(P.M, in, out, g);
@@ -306,9 +349,20 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
             cg_pupdate_kernel<PREC_JAC><<<gp, 256, 0, c->stream>>>(n, nullptr, r, P.pdiag, p, st);
             break;
         default:
-            cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
-            precond(r, z, gate);
-            cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, lb_counter(c, LB_CG_RZ), st);
+            if (fused) {  // the sweeps' gather and scatter passes fused into the update and r.z
+                tri_plan &L = P.M->L, &U = P.M->U;
+                cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
+                                                                      lb_counter(c, LB_CG_UPDATE), st,
+                                                                      L.slot_of_row.as<int>(), L.col.xs64.as<double>());
+                ilu_apply64_mid(P.M, gate);
+                cg_zscatter_rz_kernel<<<ge, 256, 0, c->stream>>>(n, U.slot_of_row.as<int>(), U.out64.as<double>(), r, z,
+                                                                 part, lb_counter(c, LB_CG_RZ), st);
+            } else {
+                cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
+                                                                      lb_counter(c, LB_CG_UPDATE), st);
+                precond(r, z, gate);
+                cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, lb_counter(c, LB_CG_RZ), st);
+            }
             cg_pupdate_kernel<PREC_ILU><<<gp, 256, 0, c->stream>>>(n, z, nullptr, nullptr, p, st);
         }
         HALA_CHECK_LAUNCH();

@@ -194,6 +194,7 @@ struct hala_options {
     int tri_col = 1;           // column-blocked triangular sweeps where the pattern allows (0: level pool only)
     int tri_col_cols = 0;      // columns of the blocked sweep (0: smallest that fits)
     int tri_col_split = 0;     // columns per slab of planes (0: automatic, 1: contiguous row blocks)
+    int cg_fuse = 1;           // ILU-PCG: the sweeps' gather/scatter fused into the update and r.z kernels
     int ilu32_jacobi = 0;      // FP32 ILU applies between FP64 vectors: k > 0 Jacobi sweeps per triangle (0: exact)
 };
 

@@ -97,9 +97,27 @@ __global__ void pivot_check(int n, const int *__restrict__ diag, const double *_
 
 __global__ void set_int(int *p, int v) { *p = v; }
 
+bool ilu_fused64(hala_ilu M) {
+    return M->factored && col_usable(M->ctx, M->L, true) && col_usable(M->ctx, M->U, true);
+}
+
+void ilu_apply64_mid(hala_ilu M, const int *gate) {
+    hala_ctx c = M->ctx;
+    col_sweep64(c, M->L, TRI_SUB, 1.0, gate);
+    col_handoff64(c, M->U, M->L, gate);
+    col_sweep64(c, M->U, TRI_SUB, 1.0, gate);
+}
+
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
+    if (ilu_fused64(M)) {  // L's output handed to U in slot order
+        tri_col_exec<double, double>(c, M->L, TRI_SUB, 1.0, x, nullptr, gate);
+        col_handoff64(c, M->U, M->L, gate);
+        col_sweep64(c, M->U, TRI_SUB, 1.0, gate);
+        col_scatter64(c, M->U, r, gate);
+        return;
+    }
     tri_exec<double, double, double>(c, M->L, TRI_SUB, 1.0, x, M->tmp64.as<double>(), gate);
     tri_exec<double, double, double>(c, M->U, TRI_SUB, 1.0, M->tmp64.as<double>(), r, gate);
 }

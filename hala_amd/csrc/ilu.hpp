@@ -44,6 +44,12 @@ namespace hala_amd {
 // r = (LU)^-1 x in the requested precision (see hala_ilu_apply_*)
 // gate (nullable, device flag): skip the apply when *gate != 0 (solver stop flag)
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
+// Both triangles on the column layout: the apply's ends can be fused by the
+// caller.  ilu_apply64_mid runs L from its gathered input (col.xs64 filled and
+// out64 re-armed by the caller), the hand-off and U; the result stays in
+// U.out64 in slot order.
+bool ilu_fused64(hala_ilu M);
+void ilu_apply64_mid(hala_ilu M, const int *gate);
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_prepare_adjoint(hala_ilu M);

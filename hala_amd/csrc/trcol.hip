@@ -597,6 +597,25 @@ __global__ void col_gather_in(int nslots, const int *__restrict__ chunk_row, con
 }
 
 // After a column sweep: the slot-ordered result into natural order.
+// The input of plan `dst` straight from the polled output of plan `src` (the
+// natural-order hand-off vector never materialises), re-arming dst's output
+// and, once read, src's (every row of src is read once here).
+template <class T>
+__global__ void col_gather_slots(int nslots, const int *__restrict__ chunk_row, T *__restrict__ src,
+                                 const int *__restrict__ src_slot, T *__restrict__ xs, T *__restrict__ out,
+                                 const int *gate) {
+    if (gate && *gate) return;
+    const int s = xcd_block((nslots + 255) / 256) * 256 + threadIdx.x;
+    if (s >= nslots) return;
+    out[s] = sentinel<T>();
+    const int r = chunk_row[s];
+    if (r >= 0) {
+        const int q = src_slot[r];
+        xs[s] = src[q];
+        src[q] = sentinel<T>();
+    }
+}
+
 template <class T, class TOUT>
 __global__ void col_scatter_out(int n, const int *__restrict__ slot_of_row, const T *__restrict__ out,
                                 TOUT *__restrict__ out_nat, const int *gate) {
@@ -886,16 +905,13 @@ bool col_usable(hala_ctx c, const tri_plan &p, bool is64) {
     return is64 ? (q.val64.p != nullptr) : (q.val32.p != nullptr);
 }
 
+namespace {
+
+// the column kernel alone: input already in slot order in xs, output left in slot order
 template <class T, class TOUT>
-void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate) {
+void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate) {
     constexpr bool is64 = sizeof(T) == 8;
     tri_colplan &q = p.col;
-    const int nslots = p.nchunk * WAVE;
-    T *xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
-    T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
-    col_gather_in<T, TOUT><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in,
-                                                                                 xs, out, gate);
-    HALA_CHECK_LAUNCH();
     col_args<T, TOUT> a;
     a.nlev = p.nlev;
     a.K = q.K;
@@ -904,9 +920,9 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     a.colrange = q.colrange.as<int>();
     a.rec = is64 ? reinterpret_cast<const int4 *>(q.val64.p) : reinterpret_cast<const int4 *>(q.val32.p);
     a.ext_slot = q.ext_slot.as<int>();
-    a.xs = xs;
+    a.xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
     a.alpha = alpha;
-    a.out = out;
+    a.out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
 
     a.ticket = counters(c) + 104;
     a.err = wait_error_flag(c);
@@ -920,11 +936,44 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
         else launch_col<T, TOUT, TRI_SUM, false>(c, p, a, lds);
     }
     HALA_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+template <class T, class TOUT>
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate) {
+    constexpr bool is64 = sizeof(T) == 8;
+    tri_colplan &q = p.col;
+    const int nslots = p.nchunk * WAVE;
+    T *xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
+    T *out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
+    col_gather_in<T, TOUT><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in,
+                                                                                 xs, out, gate);
+    HALA_CHECK_LAUNCH();
+    col_sweep<T, TOUT>(c, p, mode, alpha, gate);
     if (out_nat) {
         col_scatter_out<T, TOUT><<<xcd_grid((p.n + 255) / 256), 256, 0, c->stream>>>(p.n, p.slot_of_row.as<int>(), out,
                                                                                     out_nat, gate);
         HALA_CHECK_LAUNCH();
     }
+}
+
+void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate) {
+    col_sweep<double, double>(c, p, mode, alpha, gate);
+}
+
+void col_handoff64(hala_ctx c, tri_plan &dst, tri_plan &src, const int *gate) {
+    const int nslots = dst.nchunk * WAVE;
+    col_gather_slots<double><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(
+        nslots, dst.chunk_row.as<int>(), src.out64.as<double>(), src.slot_of_row.as<int>(), dst.col.xs64.as<double>(),
+        dst.out64.as<double>(), gate);
+    HALA_CHECK_LAUNCH();
+}
+
+void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate) {
+    col_scatter_out<double, double><<<xcd_grid((p.n + 255) / 256), 256, 0, c->stream>>>(
+        p.n, p.slot_of_row.as<int>(), p.out64.as<double>(), out_nat, gate);
+    HALA_CHECK_LAUNCH();
 }
 
 template void tri_col_exec<double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, const int *);

@@ -127,6 +127,14 @@ bool col_usable(hala_ctx c, const tri_plan &p, bool is64);
 template <class T, class TOUT>
 void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate);
 
+// Pieces of the FP64 column sweep for callers that fuse its ends: the column
+// kernel alone (input in slot order in col.xs64, output left in slot order in
+// out64); dst's input taken from src's slot-order output (re-arming both
+// outputs); the slot-order output scattered to natural order.
+void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate);
+void col_handoff64(hala_ctx c, tri_plan &dst, tri_plan &src, const int *gate);
+void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate);
+
 // Refills the polled output buffer(s) with sentinels (both precisions, or one).
 void tri_rearm(hala_ctx c, tri_plan &p);
 void tri_rearm_prec(hala_ctx c, tri_plan &p, bool is64);

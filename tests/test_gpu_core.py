@@ -442,7 +442,10 @@ def test_cg_ilu_published_table(engine):
         assert np.array_equal(bits(u), bits(xo))
 
 
-def test_cg_ilu_3d_bitexact(engine):
+@pytest.mark.parametrize("fuse", [1, 0], ids=["fused", "separate"])
+def test_cg_ilu_3d_bitexact(engine, fuse):
+    # fused: the triangular sweeps' gather and scatter passes run inside the
+    # CG update and r.z kernels; separate: the full ILU apply each pass
     import torch
 
     p, i, v = st.laplace3d(32, 32, 32)
@@ -456,7 +459,12 @@ def test_cg_ilu_3d_bitexact(engine):
     ilu = H.make_ilu(engine, gp, gi, gv)
     gx = engine.vector(n)
     gh = torch.zeros(2000, dtype=torch.float64, device=gx.device)
-    git = H.solve_cg_ilu(engine, tol, gp, gi, gv, ilu, dev(engine, b), gx, hist=gh)
+    saved = engine.get_option("cg_fuse")
+    engine.set_option("cg_fuse", fuse)
+    try:
+        git = H.solve_cg_ilu(engine, tol, gp, gi, gv, ilu, dev(engine, b), gx, hist=gh)
+    finally:
+        engine.set_option("cg_fuse", saved)
     assert git == it
     assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
     assert np.array_equal(bits(host(gx)), bits(x))
