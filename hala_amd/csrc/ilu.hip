@@ -69,23 +69,78 @@ __global__ void ilu_pair_fill(int n, const int *__restrict__ pntr, const int *__
     }
 }
 
-// The rows of one level from the symbolic pairs: the operations above in their
-// order, with no index search.
-__global__ void ilu_level_pairs(const int *__restrict__ rows, int count, const int *__restrict__ pntr,
-                                const int *__restrict__ diag, const int *__restrict__ dpos,
-                                const int *__restrict__ pptr, const int2 *__restrict__ pairs, double *f) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int j = rows[t];
-    const int dj = diag[j];
-    for (int jc = pntr[j]; jc < dj; jc++) {
-        const double l = f[jc] / f[dpos[jc]];
-        f[jc] = l;
-        for (int p = pptr[jc]; p < pptr[jc + 1]; p++) {
-            const int2 q = pairs[p];
-            f[q.y] = f[q.y] - l * f[q.x];
+// The rows of one level from the symbolic pairs, one wavefront per row: for
+// each strictly lower entry jc = (j,i) in order, l = f[jc] / u_ii, f[jc] = l,
+// then f[jk] -= l * f[ik] over its pairs.  The row is staged in LDS and every
+// operand from earlier rows (pivots u_ii, entries u_ik: final by now) is
+// fetched up front, so a row costs a few memory round trips instead of one
+// chain of dependent loads per entry.  The elimination stays in order; the
+// updates of one entry touch distinct positions, so its lanes apply them at
+// once.  Rows over the staged limits run the same loop on lane 0.
+constexpr int ILU_RB = 128;  // row entries staged
+constexpr int ILU_PS = 4;    // pairs per lane (256 per row)
+__global__ __launch_bounds__(256) void ilu_level_rows(const int4 *__restrict__ frow, const int *__restrict__ fnp,
+                                                      int count, const int *__restrict__ dpos,
+                                                      const int *__restrict__ pptr, const int2 *__restrict__ pairs,
+                                                      double *f) {
+    __shared__ double rowbuf[4][ILU_RB];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
+    const int t = blockIdx.x * 4 + w;
+    if (t >= count) return;  // a whole wave
+    const int4 ri = frow[t];
+    const int rs = ri.x, re = ri.y, dj = ri.z, p0 = ri.w, np = fnp[t];
+    const int len = re - rs, nl = dj - rs;
+    if (len > ILU_RB || nl >= WAVE || np > ILU_PS * WAVE) {
+        if (lane == 0)
+            for (int jc = rs; jc < dj; jc++) {
+                const double l = f[jc] / f[dpos[jc]];
+                f[jc] = l;
+                for (int p = pptr[jc]; p < pptr[jc + 1]; p++) {
+                    const int2 q = pairs[p];
+                    f[q.y] = f[q.y] - l * f[q.x];
+                }
+            }
+        return;
+    }
+    double *rb = rowbuf[w];
+    for (int e = lane; e < len; e += WAVE) rb[e] = f[rs + e];
+    const int pp = lane <= nl ? pptr[rs + lane] : 0;  // pair range of lower entry `lane`
+    const double piv = lane < nl ? f[dpos[rs + lane]] : 1.0;
+    int y[ILU_PS];
+    double fx[ILU_PS];
+#pragma unroll
+    for (int k = 0; k < ILU_PS; k++) {
+        const int idx = k * WAVE + lane;
+        y[k] = 0;
+        fx[k] = 0.0;
+        if (idx < np) {
+            const int2 q = pairs[p0 + idx];
+            y[k] = q.y - rs;
+            fx[k] = f[q.x];
         }
     }
+    for (int jc = 0; jc < nl; jc++) {
+        const int pb = __shfl(pp, jc, WAVE) - p0, pe = __shfl(pp, jc + 1, WAVE) - p0;
+        const double l = rb[jc] / __shfl(piv, jc, WAVE);
+        if (lane == 0) rb[jc] = l;
+#pragma unroll
+        for (int k = 0; k < ILU_PS; k++) {
+            const int idx = k * WAVE + lane;
+            if (idx >= pb && idx < pe) rb[y[k]] = rb[y[k]] - l * fx[k];
+        }
+    }
+    for (int e = lane; e < len; e += WAVE) f[rs + e] = rb[e];
+}
+
+// the factorisation's row records, in the level order of the L schedule
+__global__ void ilu_row_info(int n, const int *__restrict__ order, const int *__restrict__ pntr,
+                             const int *__restrict__ diag, const int *__restrict__ pptr, int4 *__restrict__ frow,
+                             int *__restrict__ fnp) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int j = order[t], rs = pntr[j], dj = diag[j];
+    frow[t] = make_int4(rs, pntr[j + 1], dj, pptr[rs]);
+    fnp[t] = pptr[dj] - pptr[rs];
 }
 
 __global__ void pivot_check(int n, const int *__restrict__ diag, const double *__restrict__ f, int *bad) {
@@ -181,6 +236,12 @@ static void ilu_pairs(hala_ctx c, hala_ilu M) {
     M->pairs.reserve((size_t)std::max(np, 1) * sizeof(int2));
     ilu_pair_fill<<<g, 256, 0, c->stream>>>(n, M->pntr, M->indx, M->diag.as<int>(), M->pptr.as<int>(),
                                             M->dpos.as<int>(), M->pairs.as<int2>());
+    HALA_CHECK_LAUNCH();
+    // a copy of the level order: the plan's own may be re-sorted later
+    M->frow.reserve((size_t)n * sizeof(int4));
+    M->fnp.reserve((size_t)n * sizeof(int));
+    ilu_row_info<<<g, 256, 0, c->stream>>>(n, M->L.perm.as<int>(), M->pntr, M->diag.as<int>(), M->pptr.as<int>(),
+                                           M->frow.as<int4>(), M->fnp.as<int>());
     HALA_CHECK_LAUNCH();
 }
 
@@ -299,13 +360,12 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         double *f = M->factor.as<double>();
         HALA_HIP(hipMemcpyAsync(f, vals, (size_t)M->nnz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         const std::vector<int> &lp = M->L.h_level_ptr;
-        const int *perm = M->L.perm.as<int>();
         for (int l = 1; l < M->L.nlev; l++) {  // level 0 rows have no strictly lower entries
             int cnt = lp[l + 1] - lp[l];
             if (cnt <= 0) continue;
-            ilu_level_pairs<<<(cnt + 255) / 256, 256, 0, c->stream>>>(perm + lp[l], cnt, M->pntr, M->diag.as<int>(),
-                                                                     M->dpos.as<int>(), M->pptr.as<int>(),
-                                                                     M->pairs.as<int2>(), f);
+            ilu_level_rows<<<(cnt + 3) / 4, 256, 0, c->stream>>>(M->frow.as<int4>() + lp[l], M->fnp.as<int>() + lp[l],
+                                                                cnt, M->dpos.as<int>(), M->pptr.as<int>(),
+                                                                M->pairs.as<int2>(), f);
             HALA_CHECK_LAUNCH();
         }
         int *bad = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 6);

@@ -817,15 +817,17 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     } else {
         // ---------------- compute waves ----------------
         constexpr int NQ = col_rec<T, MAXD>::NQ;
-        auto load_rec = [&](col_rec<T, MAXD> &r, int lam) {
-            const int *ci = cis + lam * CI;
-            const int slot0 = ci[0], size = ci[2] & 0xffff;
+        auto load_rec_at = [&](col_rec<T, MAXD> &r, int slot0, int size) {
             const int s = slot0 + t;
             r.slot = t < size ? s : -1;
             r.xin = a.xs[s];  // slot arrays are padded past their end: idle lanes read in bounds
             const int4 *src = a.rec + (size_t)s * NQ;
 #pragma unroll
             for (int i = 0; i < NQ; i++) r.q[i] = src[i];
+        };
+        auto load_rec = [&](col_rec<T, MAXD> &r, int lam) {
+            const int *ci = cis + lam * CI;
+            load_rec_at(r, ci[0], ci[2] & 0xffff);
         };
         col_rec<T, MAXD> rec[P];
         // in the loop's own issue order, so that the loop is entered with the
@@ -845,11 +847,24 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 const bool valid = r.slot >= 0;
                 const int eoff = K * CP + (lam & 1) * XC;  // the external buffer follows the window
                 T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
+                // all LDS reads of the step (the cell record of level lam + P and
+                // the row's operands) issued before the first is waited for
+                const int *cin = cis + (lam + P) * CI;
+                const int nslot0 = cin[0], nsize = cin[2] & 0xffff;
+                int cds[MAXD];
+                T xs[MAXD];
 #pragma unroll
                 for (int k = 0; k < MAXD; k++) {
                     const int cd = valid ? rec_code(r.q, k) : -1;
+                    cds[k] = cd;
+                    xs[k] = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < MAXD; k++) {
+                    const int cd = cds[k];
                     const T v = rec_val<T>(r.q, 2 * MAXD + (int)sizeof(T) * (k + 1));
-                    const T x = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
+                    const T x = xs[k];
                     if (MODE == TRI_SUB) {
                         const T nacc = acc - v * x;
                         acc = cd != -1 ? nacc : acc;
@@ -862,7 +877,7 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 if (!UNIT) acc = acc / rec_val<T>(r.q, 2 * MAXD);
                 win[(lvl % K) * CP + t] = acc;
                 if (valid) st_agent(a.out + r.slot, acc);
-                load_rec(rec[s], lam + P);
+                load_rec_at(rec[s], nslot0, nsize);
                 lds_barrier();
             }
         }

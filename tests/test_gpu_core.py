@@ -162,11 +162,33 @@ def test_pattern_validation(engine):
 
 
 # ---------------------------------------------------------------- ILU(0)
+def arrow2d(m=16, seed=9):
+    """m x m 5-point Laplacian plus a dense last row/column and a row/column of
+    ~100 entries: rows longer than the numeric kernel stages in LDS (and with
+    more elimination pairs than it holds) next to rows that fit."""
+    import scipy.sparse as sp
+
+    p, i, v = st.laplace2d(m, m)
+    n = len(p) - 1
+    A = sp.csr_matrix((v, i, p), shape=(n, n)).tolil()
+    rng = np.random.default_rng(seed)
+    mid = n // 2
+    for c in range(n - 1):
+        A[n - 1, c] = A[c, n - 1] = rng.uniform(-0.01, 0.01)
+    for c in range(mid - 60, n - 1, 2):
+        if c != mid:
+            A[mid, c] = A[c, mid] = rng.uniform(-0.01, 0.01)
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
 ILU_MATS = {
     "lap2d_64": lambda: st.laplace2d(64, 64),
     "lap3d_24": lambda: st.laplace3d(24, 24, 24),
     "cd27_16": lambda: st.convdiff27(16, seed=3),
     "pde_20": lambda: st.discretize_pde(20)[:3],
+    "arrow_16": arrow2d,
 }
 
 
