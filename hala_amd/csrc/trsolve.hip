@@ -112,6 +112,10 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
 // ---------------------------------------------------------------------------
 // level sets: sync-free, rows in order (forward) or reverse order (backward)
 // ---------------------------------------------------------------------------
+// A lane keeps LDEP dependencies of its row in registers and polls all of
+// those still pending at once, so a row waits one memory round trip per
+// attempt rather than one per dependency.
+constexpr int LDEP = 8;
 __global__ __launch_bounds__(256) void level_kernel(int n, int backward, const int *__restrict__ dptr,
                                                     const int *__restrict__ ddep, int *lev, unsigned *ticket,
                                                     int *err, int nwaves) {
@@ -123,23 +127,43 @@ __global__ __launch_bounds__(256) void level_kernel(int n, int backward, const i
     int idx = w * WAVE + lane;
     int row = backward ? n - 1 - idx : idx;
     bool done = !(idx < n);
-    int k = 0, end = 0, m = 0;
+    int cb = 0, end = 0, m = 0;
+    unsigned pend = 0;  // bit q: dependency cb + q not yet seen
+    int dk[LDEP];
+    auto load_chunk = [&] {
+        pend = 0;
+#pragma unroll
+        for (int q = 0; q < LDEP; q++) {
+            const bool in = cb + q < end;
+            dk[q] = in ? ddep[cb + q] : 0;
+            pend |= in ? (1u << q) : 0u;
+        }
+    };
     if (!done) {
-        k = dptr[row];
+        cb = dptr[row];
         end = dptr[row + 1];
+        load_chunk();
     }
     unsigned spins = 0;
     while (__any(!done)) {
         if (!done) {
-            while (k < end) {
-                int l = ld_agent(lev + ddep[k]);
-                if (l < 0) break;
-                m = max(m, l + 1);
-                k++;
-            }
-            if (k == end) {
-                st_agent(lev + row, m);
-                done = true;
+            int lv[LDEP];
+#pragma unroll
+            for (int q = 0; q < LDEP; q++) lv[q] = (pend >> q & 1u) ? ld_agent(lev + dk[q]) : -1;
+#pragma unroll
+            for (int q = 0; q < LDEP; q++)
+                if ((pend >> q & 1u) && lv[q] >= 0) {
+                    m = max(m, lv[q] + 1);
+                    pend &= ~(1u << q);
+                }
+            if (pend == 0) {
+                cb += LDEP;
+                if (cb < end) {
+                    load_chunk();
+                } else {
+                    st_agent(lev + row, m);
+                    done = true;
+                }
             }
         }
         if (__any(!done)) {
