@@ -9,10 +9,14 @@ numeric factorisation, then device-resident PCG until ||r|| < 1e-10 ||b||.
 value = mean seconds per step (lower is better).
 
 roofline:      what dominates a step's time -- the triangular sweep of the
-               ILU(0) apply (two per PCG pass; each is a gather pre-pass, the
-               column-blocked sweep kernel and a scatter post-pass) -- timed
-               live with HIP events on the engine stream;
-roofline_spmv: the CSR SpMV (the bandwidth kernel of the path);
+               ILU(0) apply (two per PCG pass: the input gathered into L's
+               slot order, the column-blocked L sweep, L's output handed to
+               U's slot order, the U sweep, U's output scattered to natural
+               order), one triangle = half an apply, timed live with HIP
+               events on the engine stream;
+roofline_spmv: the CSR SpMV (the bandwidth kernel of the path), with the
+               rate of a plain 256 MiB device copy measured beside it
+               (`copy_gbs`: what this HBM sustains for a streaming kernel);
 cpu_baseline:  the C restatement of HALA's CPU engine, one thread, on a
                bounded sample of the same solve;
 extra:         the other workloads -- plain CG on the 256^2 5-point Laplacian,
@@ -37,7 +41,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "PCG/GMRES time-to-1e-10 residual & SpMV HBM GB/s; mixed-prec vs FP64 on 1 GPU"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_poisson3d_128.txt")
 
 
 def log(*a):
@@ -131,7 +135,7 @@ def cpu_baseline(p, i, v, b, tol, iters_gpu, sample_iters):
 
 
 # ------------------------------------------------------------------ extras
-def extra_c1(H, eng, st, oracle):
+def extra_cg_poisson2d(H, eng, st, oracle):
     """Plain CG, 256^2 5-point Laplacian, b ~ U[-1,1] (seed 1), x0 = 0, 1e-10 relative."""
     p, i, v = st.laplace2d(256, 256)
     n = len(p) - 1
@@ -157,7 +161,7 @@ def extra_c1(H, eng, st, oracle):
             "iters_per_s": round((it - 1) / t, 1), "final_relres": rel}
 
 
-def extra_c3(H, eng, st, oracle):
+def extra_gmres_ir_convdiff27(H, eng, st, oracle):
     """GMRES(30) refinement on 96^3 27-point convection-diffusion: FP32 inner (exact or Jacobi-applied
     FP32 ILU) vs FP64 inner."""
     import torch
@@ -222,7 +226,7 @@ def extra_c3(H, eng, st, oracle):
     return out
 
 
-def extra_c4(H, eng, st):
+def extra_batched_pcg(H, eng, st):
     """4096 independent 2D Laplacians + shift, Jacobi-PCG to 1e-10, one persistent graph-captured launch."""
     import torch
 
@@ -249,7 +253,7 @@ def extra_c4(H, eng, st):
             "system_iters_per_s": round(float(it.sum()) / (ms * 1e-3), 1)}
 
 
-def extra_c5(H, eng, st):
+def extra_adjoint_grad(H, eng, st):
     """Adjoint gradient of 0.5||x - x_t||^2 w.r.t. face conductivities, 64^3 diffusion."""
     import torch
 
@@ -379,6 +383,9 @@ def main():
     by64 = spmv_bytes(n, nnz)
     gv32 = gv.float()
     ms32 = event_ms_cold(lambda: A.gemv("N", 1.0, gb, 0.0, gy, vals=gv32), 20, flush)
+    # achievable streaming rate: a 256 MiB device copy (reads and writes 512 MiB)
+    ms_copy = event_ms(lambda: H.vcopy(eng, flush[0], flush[1]), 20)
+    copy_gbs = 2 * flush[0].numel() * 8 / (ms_copy * 1e-3) / 1e9
     del flush
     by32 = spmv_bytes(n, nnz, vbytes=4)
     ach64 = by64 / (ms64 * 1e-3) / 1e9
@@ -404,7 +411,8 @@ def main():
         "solve_ms": round(1e3 * float(np.mean(solves)), 3),
         "iters_per_s": round((it - 1) / float(np.mean(solves)), 2),
         "final_relres": relres,
-        "roofline": {"kernel": "ILU(0) triangular sweep, one triangle: col_gather_in + tri_col_kernel<f64> + "
+        "roofline": {"kernel": "ILU(0) triangular sweep, one triangle = half an apply: col_gather_in + "
+                               "tri_col_kernel<f64> (L) + col_gather_slots + tri_col_kernel<f64> (U) + "
                                "col_scatter_out",
                      "bound": "hbm", "achieved": round(ach_sweep, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_sweep_bytes"),
@@ -415,14 +423,17 @@ def main():
         "roofline_spmv": {"kernel": "spmv_rows_kernel<f64> (CSR, LDS-staged 256-row blocks)", "bound": "hbm",
                           "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": traffic.get("spmv_bytes"),
-                          "bytes_per_launch": by64, "avg_launch_ms": round(ms64, 5)},
+                          "bytes_per_launch": by64, "avg_launch_ms": round(ms64, 5),
+                          "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(ach64 / copy_gbs, 4)},
         "spmv_f32_gbs": round(by32 / (ms32 * 1e-3) / 1e9, 1),
         "spmv_f32_avg_ms": round(ms32, 5),
     }
     if not args.no_extra:
         extra = {}
-        for name, fn in (("c1", lambda: extra_c1(H, eng, st, oracle)), ("c3", lambda: extra_c3(H, eng, st, oracle)),
-                         ("c4", lambda: extra_c4(H, eng, st)), ("c5", lambda: extra_c5(H, eng, st))):
+        for name, fn in (("cg_poisson2d", lambda: extra_cg_poisson2d(H, eng, st, oracle)),
+                         ("gmres_ir_convdiff27", lambda: extra_gmres_ir_convdiff27(H, eng, st, oracle)),
+                         ("batched_pcg", lambda: extra_batched_pcg(H, eng, st)),
+                         ("adjoint_grad", lambda: extra_adjoint_grad(H, eng, st))):
             log(f"[bench] extra {name}")
             extra[name] = fn()
             log(f"[bench] extra {name}: {json.dumps(extra[name])}")

@@ -20,7 +20,7 @@ constexpr int SPMV_ROWS = 256;  // rows per block in the LDS-staged kernel (= ca
 
 // Matrices whose values + indices reach this size are streamed with
 // non-temporal loads: inside a solver they are evicted between products
-// anyway (measured on the 217 MB C2 operator from a cold MALL: 59 -> 50 us),
+// anyway (measured on the 217 MB 128^3 7-point operator from a cold MALL: 59 -> 50 us),
 // while small ones stay cached across products.
 constexpr int64_t NT_STREAM_BYTES = 64ll << 20;
 template <class TV>

@@ -42,7 +42,7 @@ constexpr int STEP_BYTES_SPLIT = 8 * 1024;
 constexpr int ELMAX = 8;     // external values per loader lane at most (512 per cell)
 constexpr int DEDUP_MAX = 4096;  // external references of one cell at most
 // Levels an external value is polled ahead of its use: 2 for short-row
-// operators (C2 apply 0.577 vs 0.596 ms), 1 for long rows (whose columns
+// operators (128^3 7-point apply 0.577 vs 0.596 ms), 1 for long rows (whose columns
 // already lag their producers by several levels: 1.33 vs 1.54 ms at 2).
 constexpr int PE_MAX = 2;
 constexpr int poll_ahead(int maxd) { return maxd == 4 ? 2 : 1; }
