@@ -812,6 +812,10 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
 #pragma unroll
                 for (int e = 0; e < EL; e++) esr[s][e] = raw(lam + PE + P + 1, e);
                 lds_barrier();
+                // keep the next level's first look at its polled values below the
+                // barrier: hoisted above it, the loader would wait there for a poll
+                // issued one level earlier instead of two
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     } else {
