@@ -238,9 +238,30 @@ class gpu_triangular_matrix:
             lib().hala_tri_destroy(self.handle)
             self.handle = None
 
+    def _suffix(self):
+        import torch
+
+        return {torch.float64: "f64", torch.float32: "f32", torch.complex128: "z", torch.complex64: "c"}[self.vals.dtype]
+
+    def _alpha(self, sf, alpha):
+        if sf in ("z", "c"):
+            a = complex(alpha)
+            return (a.real, a.imag)
+        return (float(alpha),)
+
     def trsv(self, trans, alpha, b, x):
-        call("hala_trsv_f64", self.handle, trans.encode(), float(alpha), _ptr(self.vals), _ptr(b), _ptr(x))
+        """x = op(T)^-1 (alpha b); the scalar type follows the matrix values."""
+        sf = self._suffix()
+        call("hala_trsv_" + sf, self.handle, trans.encode(), *self._alpha(sf, alpha), _ptr(self.vals), _ptr(b),
+             _ptr(x))
         return x
+
+    def trsm(self, transa, transb, nrhs, alpha, B, ldb=-1):
+        """B <- op(T)^-1 (alpha B), column by column (sparse_trsm)."""
+        sf = self._suffix()
+        call("hala_trsm_" + sf, self.handle, transa.encode(), transb.encode(), int(nrhs), *self._alpha(sf, alpha),
+             _ptr(self.vals), _ptr(B), int(ldb))
+        return B
 
 
 def make_triangular_matrix(engine, uplo, diag, pntr, indx, vals):

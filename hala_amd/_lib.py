@@ -88,6 +88,13 @@ SIGNATURES = {
     "hala_tri_create": [vp, ch, ch, i32, i64, vp, vp, pvp],
     "hala_tri_destroy": [vp],
     "hala_trsv_f64": [vp, ch, f64, vp, vp, vp],
+    "hala_trsv_f32": [vp, ch, f32, vp, vp, vp],
+    "hala_trsv_z": [vp, ch, f64, f64, vp, vp, vp],
+    "hala_trsv_c": [vp, ch, f32, f32, vp, vp, vp],
+    "hala_trsm_f64": [vp, ch, ch, i32, f64, vp, vp, i32],
+    "hala_trsm_f32": [vp, ch, ch, i32, f32, vp, vp, i32],
+    "hala_trsm_z": [vp, ch, ch, i32, f64, f64, vp, vp, i32],
+    "hala_trsm_c": [vp, ch, ch, i32, f32, f32, vp, vp, i32],
     "hala_cg_f64": [vp, vp, i32, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_cg_t_f64": [vp, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_cg_batch_f64": [vp, vp, i32, vp, i32, vp, vp, f64, i32, pi32, vp, i32],

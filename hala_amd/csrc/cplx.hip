@@ -227,6 +227,33 @@ __global__ void zupper_level(const int *__restrict__ rows, int count, const int 
     st(z, i, cdiv(acc, ld(f, d)));
 }
 
+// One level of a standalone triangular solve through a plan's dependency
+// lists: x_r = (alpha b_r - sum_k v_k x_dep(k)) [/ d_r], the terms in the
+// plan's order (row order for 'N', the reference's source-row order for 'T'
+// and 'C').  CONJ ('C'): conjugated values and diagonal.
+template <class R, bool CONJ>
+__global__ void ztri_level(const int *__restrict__ rows, int count, const int *__restrict__ dptr,
+                           const int *__restrict__ ddep, const int *__restrict__ dsrc, const int *__restrict__ dgs,
+                           const R *__restrict__ vals, cx<R> alpha, const R *__restrict__ b, R *__restrict__ x) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int r = rows[t];
+    cx<R> s = {R(0), R(0)};
+    for (int k = dptr[r]; k < dptr[r + 1]; k++) {
+        cx<R> v = ld(vals, dsrc[k]);
+        if (CONJ) v = cconj(v);
+        s = cadd(s, cmul(v, ld(x, ddep[k])));
+    }
+    cx<R> y = csub(cmul(alpha, ld(b, r)), s);
+    const int dp = dgs[r];
+    if (dp >= 0) {
+        cx<R> d = ld(vals, dp);
+        if (CONJ) d = cconj(d);
+        y = cdiv(y, d);
+    }
+    st(x, r, y);
+}
+
 template <class R>
 __global__ void zpivot_check(int n, const int *__restrict__ diag, const R *__restrict__ f, int *bad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,6 +341,31 @@ void zilu_apply(hala_ilu M, const R *x, R *r) {
 }
 
 template <class R>
+void ztrsv(hala_tri T, char trans, cx<R> alpha, const R *vals, const R *b, R *x) {
+    HALA_REQUIRE(T && vals && b && x, HALA_ERR_ARG, "complex trsv: null argument");
+    const bool tr = !(trans == 'N' || trans == 'n');
+    const bool cj = trans == 'C' || trans == 'c';
+    HALA_REQUIRE(!tr || cj || trans == 'T' || trans == 't', HALA_ERR_ARG, "complex trsv: trans must be N, T or C");
+    hala_ctx c = T->ctx;
+    set_device(c);
+    tri_plan &p = tr ? T->T : T->N;
+    const std::vector<int> &lp = p.h_level_ptr;
+    const int *perm = p.perm.as<int>();
+    for (int l = 0; l < p.nlev; l++) {
+        const int cnt = lp[l + 1] - lp[l];
+        if (cnt <= 0) continue;
+        const int g = (cnt + 255) / 256;
+        if (cj)
+            ztri_level<R, true><<<g, 256, 0, c->stream>>>(perm + lp[l], cnt, p.dptr.as<int>(), p.ddep.as<int>(),
+                                                          p.dsrc.as<int>(), p.diag_src.as<int>(), vals, alpha, b, x);
+        else
+            ztri_level<R, false><<<g, 256, 0, c->stream>>>(perm + lp[l], cnt, p.dptr.as<int>(), p.ddep.as<int>(),
+                                                           p.dsrc.as<int>(), p.diag_src.as<int>(), vals, alpha, b, x);
+    }
+    HALA_CHECK_LAUNCH();
+}
+
+template <class R>
 void zspmv(hala_csr A, char trans, cx<R> alpha, const R *vals, const R *x, cx<R> beta, R *y) {
     HALA_REQUIRE(A && (A->nnz == 0 || vals) && x && y, HALA_ERR_ARG, "complex spmv: null argument");
     HALA_REQUIRE(trans == 'N' || trans == 'n' || trans == 'T' || trans == 't' || trans == 'C' || trans == 'c',
@@ -379,6 +431,19 @@ using namespace hala_amd;
         return guard([&] {                                                                                       \
             HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "complex ILU apply: null argument");                          \
             zilu_apply<R>(M, x, r);                                                                              \
+        });                                                                                                      \
+    }                                                                                                            \
+    int hala_trsv_##SUF(hala_tri T, char trans, R are, R aim, const R *vals, const R *b, R *x) {                 \
+        return guard([&] { ztrsv<R>(T, trans, cx<R>{are, aim}, vals, b, x); });                                  \
+    }                                                                                                            \
+    int hala_trsm_##SUF(hala_tri T, char transa, char transb, int nrhs, R are, R aim, const R *vals, R *B,       \
+                        int ldb) {                                                                               \
+        return guard([&] {                                                                                       \
+            HALA_REQUIRE(T && vals, HALA_ERR_ARG, "complex trsm: null argument");                                 \
+            set_device(T->ctx);                                                                                  \
+            tri_trsm_cols(T, transb, nrhs, 2 * sizeof(R), B, ldb, [&](const void *in, void *out) {                \
+                ztrsv<R>(T, transa, cx<R>{are, aim}, vals, (const R *)in, (R *)out);                              \
+            });                                                                                                  \
         });                                                                                                      \
     }
 

@@ -1,5 +1,6 @@
 // ilu.hpp -- ILU(0) handle (the gpu_ilu of gpu/hala_gpu_ilu.hpp:45-195).
 #pragma once
+#include <functional>
 #include "spmv.hpp"
 #include "trsolve.hpp"
 
@@ -42,6 +43,11 @@ struct hala_tri_s {
 };
 
 namespace hala_amd {
+// Multi-right-hand-side triangular solve (hala_trsm_*): B <- op(T)^-1 (alpha B)
+// column by column through `col`, which solves one column from a contiguous
+// input into a contiguous output.  esize: bytes per element.
+void tri_trsm_cols(hala_tri T, char transb, int nrhs, size_t esize, void *B, int ldb,
+                   const std::function<void(const void *, void *)> &col);
 // r = (LU)^-1 x in the requested precision (see hala_ilu_apply_*)
 // gate (nullable, device flag): skip the apply when *gate != 0 (solver stop flag)
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);

@@ -28,6 +28,60 @@ __global__ void tri_check(int n, int lower, const int *__restrict__ pntr, const 
 
 __global__ void set_i32_tri(int *p, int v) { *p = v; }
 
+__global__ void f32_to_f64(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+// n elements of E from src (stride ss elements) to dst (stride ds)
+template <class E>
+__global__ void strided_copy_k(int n, const E *__restrict__ src, int64_t ss, E *__restrict__ dst, int64_t ds) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[(int64_t)i * ds] = src[(int64_t)i * ss];
+}
+
+static void strided_copy(hala_ctx c, int n, size_t esize, const void *src, int64_t ss, void *dst, int64_t ds) {
+    const int g = (n + 255) / 256;
+    if (esize == 4)
+        strided_copy_k<int><<<g, 256, 0, c->stream>>>(n, (const int *)src, ss, (int *)dst, ds);
+    else if (esize == 8)
+        strided_copy_k<int2><<<g, 256, 0, c->stream>>>(n, (const int2 *)src, ss, (int2 *)dst, ds);
+    else
+        strided_copy_k<int4><<<g, 256, 0, c->stream>>>(n, (const int4 *)src, ss, (int4 *)dst, ds);
+    HALA_CHECK_LAUNCH();
+}
+
+static bool is_n(char t) { return t == 'N' || t == 'n'; }
+
+void tri_trsm_cols(hala_tri T, char transb, int nrhs, size_t esize, void *B, int ldb,
+                   const std::function<void(const void *, void *)> &col) {
+    HALA_REQUIRE(T && (B || nrhs == 0) && nrhs >= 0, HALA_ERR_ARG, "hala_trsm: bad argument");
+    HALA_REQUIRE(is_n(transb) || transb == 'T' || transb == 't' || transb == 'C' || transb == 'c', HALA_ERR_ARG,
+                 "hala_trsm: transb must be N, T or C");
+    const int n = T->n;
+    const bool bn = is_n(transb);
+    if (ldb < 0) ldb = bn ? n : nrhs;
+    HALA_REQUIRE(ldb >= (bn ? n : nrhs) && ldb >= 1, HALA_ERR_ARG, "hala_trsm: ldb below the minimum");
+    hala_ctx c = T->ctx;
+    tmp_scope ts(c);
+    dbuf &t1 = ts.get();
+    dbuf &t2 = ts.get();
+    t1.reserve((size_t)n * esize);
+    t2.reserve((size_t)n * esize);
+    char *b = static_cast<char *>(B);
+    for (int i = 0; i < nrhs; i++) {
+        if (bn) {  // column i is contiguous: a copy of it is the right-hand side, the solution lands in place
+            char *bc = b + (size_t)i * (size_t)ldb * esize;
+            HALA_HIP(hipMemcpyAsync(t1.p, bc, (size_t)n * esize, hipMemcpyDeviceToDevice, c->stream));
+            col(t1.p, bc);
+        } else {  // column i is row i of the stored B: gather, solve, scatter
+            strided_copy(c, n, esize, b + (size_t)i * esize, ldb, t1.p, 1);
+            col(t1.p, t2.p);
+            strided_copy(c, n, esize, t2.p, 1, b + (size_t)i * esize, ldb);
+        }
+    }
+}
+
 }  // namespace hala_amd
 
 using namespace hala_amd;
@@ -106,6 +160,53 @@ int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, cons
         load_values64(c, p, vals);
         tri_exec<double, double, double>(c, p, TRI_SUM, alpha, b, x);
         check_wait_error(c);
+    });
+}
+
+// FP32 values and vectors, FP32 arithmetic in the same term order
+int hala_trsv_f32(hala_tri T, char trans, float alpha, const float *vals, const float *b, float *x) {
+    return guard([&] {
+        HALA_REQUIRE(T && vals && b && x, HALA_ERR_ARG, "hala_trsv_f32: null argument");
+        hala_ctx c = T->ctx;
+        set_device(c);
+        bool tr = !(trans == 'N' || trans == 'n');
+        HALA_REQUIRE(!tr || trans == 'T' || trans == 't' || trans == 'C' || trans == 'c', HALA_ERR_ARG,
+                     "hala_trsv_f32: trans must be N, T or C");
+        tri_plan &p = tr ? T->T : T->N;
+        {  // the plan's loader takes FP64 values; the widening and its narrowing back are exact
+            tmp_scope ts(c);
+            dbuf &w = ts.get();
+            w.reserve((size_t)T->nnz * sizeof(double));
+            f32_to_f64<<<(int)((T->nnz + 255) / 256), 256, 0, c->stream>>>(T->nnz, vals, w.as<double>());
+            HALA_CHECK_LAUNCH();
+            load_values32(c, p, w.as<double>());
+            HALA_HIP(hipStreamSynchronize(c->stream));
+        }
+        tri_exec<float, float, float>(c, p, TRI_SUM, alpha, b, x);
+        check_wait_error(c);
+    });
+}
+
+int hala_trsm_f64(hala_tri T, char transa, char transb, int nrhs, double alpha, const double *vals, double *B,
+                  int ldb) {
+    return guard([&] {
+        HALA_REQUIRE(T && vals, HALA_ERR_ARG, "hala_trsm_f64: null argument");
+        set_device(T->ctx);
+        tri_trsm_cols(T, transb, nrhs, sizeof(double), B, ldb, [&](const void *in, void *out) {
+            const int rc = hala_trsv_f64(T, transa, alpha, vals, (const double *)in, (double *)out);
+            if (rc != HALA_OK) throw status_error{rc, hala_last_error()};
+        });
+    });
+}
+
+int hala_trsm_f32(hala_tri T, char transa, char transb, int nrhs, float alpha, const float *vals, float *B, int ldb) {
+    return guard([&] {
+        HALA_REQUIRE(T && vals, HALA_ERR_ARG, "hala_trsm_f32: null argument");
+        set_device(T->ctx);
+        tri_trsm_cols(T, transb, nrhs, sizeof(float), B, ldb, [&](const void *in, void *out) {
+            const int rc = hala_trsv_f32(T, transa, alpha, vals, (const float *)in, (float *)out);
+            if (rc != HALA_OK) throw status_error{rc, hala_last_error()};
+        });
     });
 }
 

@@ -544,7 +544,6 @@ class gpu_triangular_matrix {
     gpu_triangular_matrix(gpu_engine const &engine, char uplo, char diag, VP const &pntr, VI const &indx,
                           VV const &vals, char /*policy*/ = 'N')
         : rengine(engine), rvals(vals.data()), n((int)pntr.size() - 1) {
-        static_assert(std::is_same<T, double>::value, "hala_amd: triangular solves are FP64");
         hala_tri h = nullptr;
         check_hala(hala_tri_create(engine.handle(), uplo, diag, n, (int64_t)indx.size(), pntr.data(), indx.data(), &h),
                    "make_triangular_matrix()");
@@ -561,12 +560,47 @@ class gpu_triangular_matrix {
     template <typename FP, class VB, class VX>
     void trsv(char trans, FP alpha, VB const &b, VX &&x) const {
         if ((int)x.size() < n) x.resize(n);
-        check_hala(hala_trsv_f64(tri.get(), trans, (double)alpha, (double const *)rvals, (double const *)b.data(),
-                                 (double *)x.data()),
-                   "gpu_triangular_matrix::trsv()");
+        check_hala(trsv_(trans, T(alpha), b.data(), x.data()), "gpu_triangular_matrix::trsv()");
+    }
+    // B <- op(T)^-1 (alpha B) column by column (sparse/hala_sparse_structs.hpp:278-300)
+    template <typename FP, class VB>
+    size_t trsm_buffer_size(char, char, int, FP, VB &&, int = -1) const {
+        return 0;
+    }
+    template <typename FP, class VB, class VW>
+    void trsm(char transa, char transb, int nrhs, FP alpha, VB &&B, int ldb, VW &&) const {
+        trsm(transa, transb, nrhs, alpha, B, ldb);
+    }
+    template <typename FP, class VB>
+    void trsm(char transa, char transb, int nrhs, FP alpha, VB &&B, int ldb = -1) const {
+        check_hala(trsm_(transa, transb, nrhs, T(alpha), B.data(), ldb), "gpu_triangular_matrix::trsm()");
     }
 
   private:
+    int trsv_(char t, double a, double const *b, double *x) const {
+        return hala_trsv_f64(tri.get(), t, a, (double const *)rvals, b, x);
+    }
+    int trsv_(char t, float a, float const *b, float *x) const {
+        return hala_trsv_f32(tri.get(), t, a, (float const *)rvals, b, x);
+    }
+    int trsv_(char t, std::complex<double> a, std::complex<double> const *b, std::complex<double> *x) const {
+        return hala_trsv_z(tri.get(), t, a.real(), a.imag(), (double const *)rvals, (double const *)b, (double *)x);
+    }
+    int trsv_(char t, std::complex<float> a, std::complex<float> const *b, std::complex<float> *x) const {
+        return hala_trsv_c(tri.get(), t, a.real(), a.imag(), (float const *)rvals, (float const *)b, (float *)x);
+    }
+    int trsm_(char ta, char tb, int k, double a, double *B, int ldb) const {
+        return hala_trsm_f64(tri.get(), ta, tb, k, a, (double const *)rvals, B, ldb);
+    }
+    int trsm_(char ta, char tb, int k, float a, float *B, int ldb) const {
+        return hala_trsm_f32(tri.get(), ta, tb, k, a, (float const *)rvals, B, ldb);
+    }
+    int trsm_(char ta, char tb, int k, std::complex<double> a, std::complex<double> *B, int ldb) const {
+        return hala_trsm_z(tri.get(), ta, tb, k, a.real(), a.imag(), (double const *)rvals, (double *)B, ldb);
+    }
+    int trsm_(char ta, char tb, int k, std::complex<float> a, std::complex<float> *B, int ldb) const {
+        return hala_trsm_c(tri.get(), ta, tb, k, a.real(), a.imag(), (float const *)rvals, (float *)B, ldb);
+    }
     gpu_engine rengine;
     T const *rvals;
     int n;
@@ -580,6 +614,11 @@ auto make_triangular_matrix(gpu_engine const &engine, char uplo, char diag, VP c
 template <class TM, typename FP, class VB, class VX>
 void sparse_trsv(char trans, TM const &tri, FP alpha, VB const &b, VX &&x) {
     tri.trsv(trans, alpha, b, x);
+}
+// sparse/hala_sparse_blas.hpp:203-205
+template <class TM, typename FP, class VB>
+void sparse_trsm(char transa, char transb, int nrhs, TM const &tri, FP alpha, VB &&B, int ldb = -1) {
+    tri.trsm(transa, transb, nrhs, alpha, B, ldb);
 }
 
 // ---------------------------------------------------------------------------

@@ -141,6 +141,15 @@ struct mixed_triangular_matrix {
     void trsv(char trans, FP alpha, VB const &b, VX &&x, VW &&) const {
         trsv(trans, alpha, b, x);
     }
+    template <typename FP, class VB>
+    void trsm(char transa, char transb, int nrhs, FP alpha, VB &&B, int ldb = -1) const {
+        auto gB = gpu_bind_vector(rengine.gpu(), B);
+        tri.trsm(transa, transb, nrhs, alpha, gB, ldb);
+    }
+    template <typename FP, class VB, class VW>
+    void trsm(char transa, char transb, int nrhs, FP alpha, VB &&B, int ldb, VW &&) const {
+        trsm(transa, transb, nrhs, alpha, B, ldb);
+    }
 
     gpu_vector<int> gpu_pntr, gpu_indx;
     gpu_vector<T> gpu_vals;

@@ -173,6 +173,26 @@ int hala_tri_create(hala_ctx ctx, char uplo, char diag, int n, int64_t nnz, cons
                     hala_tri *tri);
 int hala_tri_destroy(hala_tri tri);
 int hala_trsv_f64(hala_tri tri, char trans, double alpha, const double *vals, const double *b, double *x);
+/* The same for float and complex<double> ("z") / complex<float> ("c") values
+ * (interleaved real, imaginary), trans N, T or C ('C' conjugates the values):
+ * the reference instantiates the triangular matrix for the four scalar types
+ * (gpu/hala_rocm_sparse_triangular.hpp:99-115, tests/solvers_tests.cpp:14-40). */
+int hala_trsv_f32(hala_tri tri, char trans, float alpha, const float *vals, const float *b, float *x);
+int hala_trsv_z(hala_tri tri, char trans, double alpha_re, double alpha_im, const double *vals, const double *b,
+                double *x);
+int hala_trsv_c(hala_tri tri, char trans, float alpha_re, float alpha_im, const float *vals, const float *b, float *x);
+/* sparse_trsm / gpu_triangular_matrix::trsm (gpu/hala_rocm_sparse_triangular.hpp:116-156;
+ * semantics of the CPU engine's trsm, sparse/hala_sparse_structs.hpp:278-300):
+ * B <- op(T)^-1 (alpha B) column by column.  transb 'N': column i is B[i*ldb ..
+ * i*ldb + n); 'T' or 'C': column i is B[i + r*ldb], r < n (the conjugation of B
+ * cancels on both sides).  ldb < 0 selects the minimum (n, or nrhs). */
+int hala_trsm_f64(hala_tri tri, char transa, char transb, int nrhs, double alpha, const double *vals, double *B,
+                  int ldb);
+int hala_trsm_f32(hala_tri tri, char transa, char transb, int nrhs, float alpha, const float *vals, float *B, int ldb);
+int hala_trsm_z(hala_tri tri, char transa, char transb, int nrhs, double alpha_re, double alpha_im, const double *vals,
+                double *B, int ldb);
+int hala_trsm_c(hala_tri tri, char transa, char transb, int nrhs, float alpha_re, float alpha_im, const float *vals,
+                float *B, int ldb);
 
 /* ---- solvers --------------------------------------------------------------- */
 /* solve_cg / solve_cg_ilu on gpu_engine: hex/solvers/hala_solvers_cg.hpp:181-361

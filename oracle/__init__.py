@@ -41,6 +41,9 @@ _SIG = {
     "hala_or_ilu_apply_f32_jacobi": (None, [i32, vp, vp, vp, vp, i32, vp, vp]),
     "hala_or_ilu_apply_t": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_trsv": (None, [ch, ch, ch, i32, f64, vp, vp, vp, vp, vp]),
+    "hala_or_trsv_f32": (None, [ch, ch, ch, i32, f32, vp, vp, vp, vp, vp]),
+    "hala_or_ztrsv_z": (None, [ch, ch, ch, i32, f64, f64, vp, vp, vp, vp, vp]),
+    "hala_or_ztrsv_c": (None, [ch, ch, ch, i32, f32, f32, vp, vp, vp, vp, vp]),
     "hala_or_cg": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, f64, i32, vp, i32]),
     "hala_or_cg_team": (i32, [i32, vp, vp, vp, vp, vp, vp, f64, i32, i32, vp, i32]),
     "hala_or_cg_batch": (i32, [i32, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, vp, f64, i32, vp, i32]),
@@ -275,6 +278,28 @@ def trsv(uplo, trans, dg, alpha, pntr, indx, vals, b):
     x = np.zeros(n, np.float64)
     lib().hala_or_trsv(uplo.encode(), trans.encode(), dg.encode(), n, alpha, P(ci32(pntr)), P(ci32(indx)),
                        P(c64(vals)), P(c64(b)), P(x))
+    return x
+
+
+def trsv_f32(uplo, trans, dg, alpha, pntr, indx, vals, b):
+    """FP32 values, vectors and arithmetic, the term order of trsv()."""
+    n = len(pntr) - 1
+    x = np.zeros(n, np.float32)
+    lib().hala_or_trsv_f32(uplo.encode(), trans.encode(), dg.encode(), n, float(alpha), P(ci32(pntr)), P(ci32(indx)),
+                           P(np.ascontiguousarray(vals, np.float32)), P(np.ascontiguousarray(b, np.float32)), P(x))
+    return x
+
+
+def ztrsv(uplo, trans, dg, alpha, pntr, indx, vals, b):
+    """Complex triangular solve; complex128 inputs use the double version, complex64 the float one."""
+    vals = np.ascontiguousarray(vals)
+    sf = "z" if vals.dtype == np.complex128 else "c"
+    b = np.ascontiguousarray(b, vals.dtype)
+    n = len(pntr) - 1
+    x = np.zeros(n, vals.dtype)
+    alpha = complex(alpha)
+    getattr(lib(), "hala_or_ztrsv_" + sf)(uplo.encode(), trans.encode(), dg.encode(), n, alpha.real, alpha.imag,
+                                          P(ci32(pntr)), P(ci32(indx)), P(vals), P(b), P(x))
     return x
 
 

@@ -317,41 +317,44 @@ void hala_or_ilu_apply_t(int n, const int *pntr, const int *indx, const int *dia
  * (sparse/hala_sparse_utils.hpp:284-335).  'N' sums the off-diagonal terms
  * first, then subtracts; 'T'/'C' is column oriented.
  */
-void hala_or_trsv(char uplo, char trans, char diag, int n, double alpha, const int *pntr, const int *indx,
-                  const double *vals, const double *b, double *x) {
-    int unit = (diag == 'U' || diag == 'u');
-    int lower = (uplo == 'L' || uplo == 'l');
-    if (trans == 'N' || trans == 'n') {
-        if (lower) {
-            x[0] = unit ? alpha * b[0] : alpha * b[0] / vals[0];
-            for (int i = 1; i < n; i++) {
-                double s = 0.0;
-                for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) s = s + vals[j] * x[indx[j]];
-                x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i + 1] - 1];
-            }
-        } else {
-            x[n - 1] = unit ? alpha * b[n - 1] : alpha * b[n - 1] / vals[pntr[n] - 1];
-            for (int i = n - 2; i >= 0; i--) {
-                double s = 0.0;
-                for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) s = s + vals[j] * x[indx[j]];
-                x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i]];
-            }
-        }
-    } else {
-        for (int i = 0; i < n; i++) x[i] = 0.0 * x[i];
-        if (lower) {
-            for (int i = n - 1; i >= 0; i--) {
-                x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i + 1] - 1];
-                for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j];
-            }
-        } else {
-            for (int i = 0; i < n; i++) {
-                x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i]];
-                for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j];
-            }
-        }
+#define OR_TRSV(NAME, R)                                                                                     \
+    void NAME(char uplo, char trans, char diag, int n, R alpha, const int *pntr, const int *indx, const R *vals,  \
+              const R *b, R *x) {                                                                             \
+        int unit = (diag == 'U' || diag == 'u');                                                              \
+        int lower = (uplo == 'L' || uplo == 'l');                                                             \
+        if (trans == 'N' || trans == 'n') {                                                                   \
+            if (lower) {                                                                                      \
+                x[0] = unit ? alpha * b[0] : alpha * b[0] / vals[0];                                          \
+                for (int i = 1; i < n; i++) {                                                                 \
+                    R s = (R)0;                                                                               \
+                    for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) s = s + vals[j] * x[indx[j]];             \
+                    x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i + 1] - 1];            \
+                }                                                                                             \
+            } else {                                                                                          \
+                x[n - 1] = unit ? alpha * b[n - 1] : alpha * b[n - 1] / vals[pntr[n] - 1];                    \
+                for (int i = n - 2; i >= 0; i--) {                                                            \
+                    R s = (R)0;                                                                               \
+                    for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) s = s + vals[j] * x[indx[j]];             \
+                    x[i] = unit ? (alpha * b[i] - s) : (alpha * b[i] - s) / vals[pntr[i]];                    \
+                }                                                                                             \
+            }                                                                                                 \
+        } else {                                                                                              \
+            for (int i = 0; i < n; i++) x[i] = (R)0 * x[i];                                                   \
+            if (lower) {                                                                                      \
+                for (int i = n - 1; i >= 0; i--) {                                                            \
+                    x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i + 1] - 1];      \
+                    for (int j = pntr[i]; j < pntr[i + 1] - 1; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j]; \
+                }                                                                                             \
+            } else {                                                                                          \
+                for (int i = 0; i < n; i++) {                                                                 \
+                    x[i] = unit ? (alpha * b[i] - x[i]) : (alpha * b[i] - x[i]) / vals[pntr[i]];              \
+                    for (int j = pntr[i] + 1; j < pntr[i + 1]; j++) x[indx[j]] = x[indx[j]] + x[i] * vals[j]; \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
     }
-}
+OR_TRSV(hala_or_trsv, double)
+OR_TRSV(hala_or_trsv_f32, float)
 
 /* ------------------------------------------------------------------------- */
 /* Preconditioners as the solvers apply them                                 */
@@ -954,6 +957,59 @@ HALA_OR_CSUM(float, csum_c)
                 const int k = indx[j];                                                                               \
                 y[2 * k] = y[2 * k] + (axr * vr - axi * vi);                                                         \
                 y[2 * k + 1] = y[2 * k + 1] + (axr * vi + axi * vr);                                                 \
+            }                                                                                                        \
+        }                                                                                                            \
+    }                                                                                                                \
+    /* op(T) x = alpha b for one stored triangle (sparse/hala_sparse_utils.hpp:284-335): 'N' sums the row's      \
+       off-diagonal terms v x first; 'T'/'C' is column oriented, x_i * v added to x_k in the sweep's order;        \
+       'C' conjugates the values and the diagonal.  Complex product and quotient as documented above. */           \
+    void hala_or_ztrsv_##SUF(char uplo, char trans, char diag, int n, R are, R aim, const int *pntr,              \
+                             const int *indx, const R *vals, const R *b, R *x) {                                     \
+        const int unit = diag == 'U' || diag == 'u', lower = uplo == 'L' || uplo == 'l';                             \
+        const int cj = trans == 'C' || trans == 'c';                                                                 \
+        if (trans == 'N' || trans == 'n') {                                                                          \
+            for (int t = 0; t < n; t++) {                                                                            \
+                const int i = lower ? t : n - 1 - t;                                                                 \
+                const int js = lower ? pntr[i] : pntr[i] + 1, je = lower ? pntr[i + 1] - 1 : pntr[i + 1];            \
+                R sr = (R)0, si = (R)0;                                                                              \
+                for (int j = js; j < je; j++) {                                                                      \
+                    const R vr = vals[2 * j], vi = vals[2 * j + 1], xr = x[2 * indx[j]], xi = x[2 * indx[j] + 1];     \
+                    sr = sr + (vr * xr - vi * xi);                                                                   \
+                    si = si + (vr * xi + vi * xr);                                                                   \
+                }                                                                                                    \
+                R yr = (are * b[2 * i] - aim * b[2 * i + 1]) - sr, yi = (are * b[2 * i + 1] + aim * b[2 * i]) - si;  \
+                if (!unit) {                                                                                         \
+                    const int d = lower ? pntr[i + 1] - 1 : pntr[i];                                                 \
+                    const R dr = vals[2 * d], di = vals[2 * d + 1], q = dr * dr + di * di;                           \
+                    const R tr = (yr * dr + yi * di) / q, ti = (yi * dr - yr * di) / q;                              \
+                    yr = tr;                                                                                         \
+                    yi = ti;                                                                                         \
+                }                                                                                                    \
+                x[2 * i] = yr;                                                                                       \
+                x[2 * i + 1] = yi;                                                                                   \
+            }                                                                                                        \
+            return;                                                                                                  \
+        }                                                                                                            \
+        for (int i = 0; i < 2 * n; i++) x[i] = (R)0 * x[i];                                                         \
+        for (int t = 0; t < n; t++) {                                                                                \
+            const int i = lower ? n - 1 - t : t;                                                                     \
+            const int d = lower ? pntr[i + 1] - 1 : pntr[i];                                                         \
+            R yr = (are * b[2 * i] - aim * b[2 * i + 1]) - x[2 * i];                                                 \
+            R yi = (are * b[2 * i + 1] + aim * b[2 * i]) - x[2 * i + 1];                                             \
+            if (!unit) {                                                                                             \
+                const R dr = vals[2 * d], di = cj ? -vals[2 * d + 1] : vals[2 * d + 1], q = dr * dr + di * di;       \
+                const R tr = (yr * dr + yi * di) / q, ti = (yi * dr - yr * di) / q;                                  \
+                yr = tr;                                                                                             \
+                yi = ti;                                                                                             \
+            }                                                                                                        \
+            x[2 * i] = yr;                                                                                           \
+            x[2 * i + 1] = yi;                                                                                       \
+            const int js = lower ? pntr[i] : pntr[i] + 1, je = lower ? pntr[i + 1] - 1 : pntr[i + 1];                \
+            for (int j = js; j < je; j++) {                                                                          \
+                const R vr = vals[2 * j], vi = cj ? -vals[2 * j + 1] : vals[2 * j + 1];                              \
+                const int k = indx[j];                                                                               \
+                x[2 * k] = x[2 * k] + (yr * vr - yi * vi);                                                           \
+                x[2 * k + 1] = x[2 * k + 1] + (yr * vi + yi * vr);                                                   \
             }                                                                                                        \
         }                                                                                                            \
     }                                                                                                                \

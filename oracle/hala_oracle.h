@@ -83,6 +83,12 @@ void hala_or_ilu_apply_t(int n, const int *pntr, const int *indx, const int *dia
 /* ---- standalone triangular solve ---------------------------------------- */
 void hala_or_trsv(char uplo, char trans, char diag, int n, double alpha, const int *pntr, const int *indx,
                   const double *vals, const double *b, double *x);
+void hala_or_trsv_f32(char uplo, char trans, char diag, int n, float alpha, const int *pntr, const int *indx,
+                      const float *vals, const float *b, float *x);
+void hala_or_ztrsv_z(char uplo, char trans, char diag, int n, double are, double aim, const int *pntr,
+                     const int *indx, const double *vals, const double *b, double *x);
+void hala_or_ztrsv_c(char uplo, char trans, char diag, int n, float are, float aim, const int *pntr, const int *indx,
+                     const float *vals, const float *b, float *x);
 
 /* ---- Krylov solvers ------------------------------------------------------ */
 /* preconditioner kinds: z = r, z = (LU)^-1 r, z = r ./ d, z = (LU)^-T r */

@@ -63,6 +63,14 @@ static std::vector<T> tmul(std::vector<int> const &p, std::vector<int> const &ix
     return y;
 }
 template <typename T>
+static T conjv(T v) {
+    return v;
+}
+template <typename R>
+static std::complex<R> conjv(std::complex<R> v) {
+    return std::conj(v);
+}
+template <typename T>
 struct mkv {
     static T get(double re, double) { return T(re); }
 };
@@ -118,6 +126,61 @@ static void typed_solvers(hala::gpu_engine const &engine, const char *name, doub
     vals = {2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 2.0};
     run(pntr, indx, vals, false);
     std::printf("PASS solvers<%s>\n", name);
+}
+
+// tests/sparse_tests.hpp:314-366 (sparse_trsm) for one scalar type: B = 2 op(A) Xref,
+// then sparse_trsm(transa, transb, nrhs, tri, 0.5, B) must give Xref back
+template <typename T>
+static void typed_trsm(hala::gpu_engine const &engine, const char *name, double tol) {
+    const int M = 3, nrhs = 6;
+    const bool cplx = !std::is_same<T, float>::value && !std::is_same<T, double>::value;
+    std::vector<T> Adense = {mkv<T>::get(1, 0), mkv<T>::get(4, 0), mkv<T>::get(5, 0), T(0), mkv<T>::get(2, 0), T(0),
+                             T(0), T(0), mkv<T>::get(3, 0)};  // column-major lower triangle
+    std::vector<T> Xref(M * nrhs);
+    for (int i = 0; i < M * nrhs; i++) Xref[i] = mkv<T>::get(-3.0 + i, cplx ? 0.5 * i : 0.0);
+    auto opA = [&](int r, int c, char t) -> T {
+        T v = (t == 'N') ? Adense[r + c * M] : Adense[c + r * M];
+        return t == 'C' ? conjv(v) : v;
+    };
+    auto run = [&](char uplo, std::vector<int> const &p, std::vector<int> const &ix, std::vector<T> const &v,
+                   char transa, char transb, char opchar) {
+        // B = 2 op(A_uplo) Xref in the layout transb selects
+        std::vector<T> B(M * nrhs);
+        for (int k = 0; k < nrhs; k++)
+            for (int r = 0; r < M; r++) {
+                T s = T(0);
+                for (int c = 0; c < M; c++) {
+                    const T xc = (transb == 'N') ? Xref[c + k * M] : Xref[k + c * nrhs];
+                    s += T(2) * opA(r, c, opchar) * xc;
+                }
+                if (transb == 'N') B[r + k * M] = s;
+                else B[k + r * nrhs] = s;
+            }
+        auto gp = hala::make_gpu_vector(p);
+        auto gi = hala::make_gpu_vector(ix);
+        auto gv = hala::make_gpu_vector(v);
+        auto tri = hala::make_triangular_matrix(engine, uplo, 'N', gp, gi, gv);
+        auto gB = hala::make_gpu_vector(B);
+        hala::sparse_trsm(transa, transb, nrhs, tri, T(0.5), gB);
+        std::vector<T> X;
+        gB.unload(X);
+        double err = 0.0;
+        for (int i = 0; i < M * nrhs; i++) err += std::abs(std::complex<double>(X[i]) - std::complex<double>(Xref[i]));
+        CHECK(err < tol);
+    };
+    std::vector<int> lp = {0, 1, 3, 5}, li = {0, 0, 1, 0, 2};
+    std::vector<T> lv = {mkv<T>::get(1, 0), mkv<T>::get(4, 0), mkv<T>::get(2, 0), mkv<T>::get(5, 0), mkv<T>::get(3, 0)};
+    run('L', lp, li, lv, 'N', 'N', 'N');
+    run('L', lp, li, lv, 'T', 'N', 'T');
+    run('L', lp, li, lv, 'N', 'T', 'N');
+    run('L', lp, li, lv, 'C', 'N', 'C');
+    run('L', lp, li, lv, 'N', 'C', 'N');
+    // the upper triangle A^T: 'N' on it is A^T, 'T' on it is A
+    std::vector<int> up = {0, 3, 4, 5}, ui = {0, 1, 2, 1, 2};
+    std::vector<T> uv = {mkv<T>::get(1, 0), mkv<T>::get(4, 0), mkv<T>::get(5, 0), mkv<T>::get(2, 0), mkv<T>::get(3, 0)};
+    run('U', up, ui, uv, 'N', 'N', 'T');
+    run('U', up, ui, uv, 'T', 'N', 'N');
+    std::printf("PASS sparse_trsm<%s>\n", name);
 }
 
 int main() {
@@ -367,6 +430,10 @@ int main() {
         std::printf("PASS mixed_engine\n");
     }
     // the four scalar types of tests/solvers_tests.cpp:14-40
+    typed_trsm<double>(engine, "double", 1e-12);
+    typed_trsm<float>(engine, "float", 1e-4);
+    typed_trsm<std::complex<double>>(engine, "complex<double>", 1e-12);
+    typed_trsm<std::complex<float>>(engine, "complex<float>", 1e-4);
     typed_solvers<double>(engine, "double", 1e-13);
     typed_solvers<float>(engine, "float", 1e-5);
     typed_solvers<std::complex<double>>(engine, "complex<double>", 1e-13);
