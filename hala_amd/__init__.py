@@ -135,6 +135,21 @@ class gpu_sparse_matrix:
             call("hala_spmv_f32x64", self.handle, float(alpha), _ptr(vals), _ptr(x), float(beta), _ptr(y))
         return y
 
+    def gemm(self, transa, transb, b_rows, b_cols, alpha, B, ldb, beta, C, ldc):
+        """C = alpha op(A) op(B) + beta C (column-major B, C; gpu_sparse_matrix::gemm)."""
+        import torch
+
+        sf = {torch.float64: "f64", torch.float32: "f32", torch.complex128: "z",
+              torch.complex64: "c"}[self.vals.dtype]
+        if sf in ("z", "c"):
+            a, b = complex(alpha), complex(beta)
+            sc = (a.real, a.imag), (b.real, b.imag)
+        else:
+            sc = (float(alpha),), (float(beta),)
+        call("hala_spmm_" + sf, self.handle, transa.encode(), transb.encode(), int(b_rows), int(b_cols), *sc[0],
+             _ptr(self.vals), _ptr(B), int(ldb), *sc[1], _ptr(C), int(ldc))
+        return C
+
 
 def make_sparse_matrix(engine, ncols, pntr, indx, vals):
     """make_sparse_matrix(gpu_engine, ncols, pntr, indx, vals): gpu/hala_rocm_sparse_general.hpp:179-185."""

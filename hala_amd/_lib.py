@@ -89,6 +89,10 @@ SIGNATURES = {
     "hala_tri_destroy": [vp],
     "hala_trsv_f64": [vp, ch, f64, vp, vp, vp],
     "hala_trsv_f32": [vp, ch, f32, vp, vp, vp],
+    "hala_spmm_f64": [vp, ch, ch, i32, i32, f64, vp, vp, i32, f64, vp, i32],
+    "hala_spmm_f32": [vp, ch, ch, i32, i32, f32, vp, vp, i32, f32, vp, i32],
+    "hala_spmm_z": [vp, ch, ch, i32, i32, f64, f64, vp, vp, i32, f64, f64, vp, i32],
+    "hala_spmm_c": [vp, ch, ch, i32, i32, f32, f32, vp, vp, i32, f32, f32, vp, i32],
     "hala_trsv_z": [vp, ch, f64, f64, vp, vp, vp],
     "hala_trsv_c": [vp, ch, f32, f32, vp, vp, vp],
     "hala_trsm_f64": [vp, ch, ch, i32, f64, vp, vp, i32],

@@ -255,6 +255,12 @@ __global__ void ztri_level(const int *__restrict__ rows, int count, const int *_
 }
 
 template <class R>
+__global__ void zconj_kernel(int n, R *__restrict__ x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[2 * (int64_t)i + 1] = -x[2 * (int64_t)i + 1];
+}
+
+template <class R>
 __global__ void zpivot_check(int n, const int *__restrict__ diag, const R *__restrict__ f, int *bad) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -431,6 +437,23 @@ using namespace hala_amd;
         return guard([&] {                                                                                       \
             HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "complex ILU apply: null argument");                          \
             zilu_apply<R>(M, x, r);                                                                              \
+        });                                                                                                      \
+    }                                                                                                            \
+    int hala_spmm_##SUF(hala_csr A, char transa, char transb, int b_rows, int b_cols, R are, R aim, const R *vals, \
+                        const R *B, int ldb, R bre, R bim, R *C, int ldc) {                                        \
+        return guard([&] {                                                                                       \
+            HALA_REQUIRE(A, HALA_ERR_ARG, "complex spmm: null matrix");                                           \
+            set_device(A->ctx);                                                                                  \
+            hala_ctx cx_ = A->ctx;                                                                               \
+            spmm_cols(                                                                                           \
+                A, transa, transb, b_rows, b_cols, 2 * sizeof(R), B, ldb, C, ldc,                                \
+                [&](const void *x, void *y) {                                                                    \
+                    zspmv<R>(A, transa, cx<R>{are, aim}, vals, (const R *)x, cx<R>{bre, bim}, (R *)y);           \
+                },                                                                                               \
+                [&](void *t, int n) {                                                                            \
+                    if (n > 0) zconj_kernel<R><<<(n + 255) / 256, 256, 0, cx_->stream>>>(n, (R *)t);             \
+                    HALA_CHECK_LAUNCH();                                                                         \
+                });                                                                                              \
         });                                                                                                      \
     }                                                                                                            \
     int hala_trsv_##SUF(hala_tri T, char trans, R are, R aim, const R *vals, const R *b, R *x) {                 \

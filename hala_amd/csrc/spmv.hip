@@ -1,5 +1,6 @@
 // spmv.hip -- CSR handle, pattern validation and the SpMV kernels
 // (replacing rocsparse_?csrmv of gpu/hala_rocm_sparse_general.hpp:115-128).
+#include <algorithm>
 #include <climits>
 
 #include "spmv.hpp"
@@ -167,6 +168,37 @@ void spmv_trans(hala_csr A, T alpha, const T *vals, const T *x, T beta, T *y) {
                                                                     x, alpha, beta, y);
     HALA_CHECK_LAUNCH();
 }
+void spmm_cols(hala_csr A, char transa, char transb, int b_rows, int b_cols, size_t esize, const void *B, int ldb,
+               void *C, int ldc, const std::function<void(const void *, void *)> &mv,
+               const std::function<void(void *, int)> &conj) {
+    auto isn = [](char t) { return t == 'N' || t == 'n'; };
+    auto ok = [&](char t) { return isn(t) || t == 'T' || t == 't' || t == 'C' || t == 'c'; };
+    HALA_REQUIRE(A && ok(transa) && ok(transb), HALA_ERR_ARG, "spmm: trans must be N, T or C");
+    HALA_REQUIRE(b_rows >= 0 && b_cols >= 0, HALA_ERR_ARG, "spmm: negative size");
+    const int M = isn(transa) ? A->nrows : A->ncols;  // rows of C
+    const int K = isn(transa) ? A->ncols : A->nrows;  // rows of op(B)
+    const int N = isn(transb) ? b_cols : b_rows;      // columns of C
+    HALA_REQUIRE((isn(transb) ? b_rows : b_cols) == K, HALA_ERR_ARG, "spmm: op(B) must have op(A)'s column count of rows");
+    HALA_REQUIRE(ldb >= std::max(b_rows, 1) && ldc >= std::max(M, 1), HALA_ERR_ARG, "spmm: ldb or ldc below the minimum");
+    HALA_REQUIRE(N == 0 || (B && C), HALA_ERR_ARG, "spmm: null pointer");
+    hala_ctx c = A->ctx;
+    const char *b = static_cast<const char *>(B);
+    char *cc = static_cast<char *>(C);
+    tmp_scope ts(c);
+    dbuf &t = ts.get();
+    if (!isn(transb)) t.reserve((size_t)std::max(K, 1) * esize);
+    for (int i = 0; i < N; i++) {
+        char *y = cc + (size_t)i * (size_t)ldc * esize;
+        if (isn(transb)) {
+            mv(b + (size_t)i * (size_t)ldb * esize, y);
+        } else {
+            strided_copy(c, K, esize, b + (size_t)i * esize, ldb, t.p, 1);
+            if ((transb == 'C' || transb == 'c') && conj) conj(t.p, K);
+            mv(t.p, y);
+        }
+    }
+}
+
 template void spmv_trans<double>(hala_csr, double, const double *, const double *, double, double *);
 template void spmv_trans<float>(hala_csr, float, const float *, const float *, float, float *);
 
@@ -306,6 +338,34 @@ int hala_spmv_f32(hala_csr A, char trans, float alpha, const float *vals, const 
         else if (trans == 'T' || trans == 't' || trans == 'C' || trans == 'c')
             spmv_trans<float>(A, alpha, vals, x, beta, y);
         else throw status_error{HALA_ERR_ARG, "spmv: trans must be N, T or C"};
+    });
+}
+
+int hala_spmm_f64(hala_csr A, char transa, char transb, int b_rows, int b_cols, double alpha, const double *vals,
+                  const double *B, int ldb, double beta, double *C, int ldc) {
+    return guard([&] {
+        HALA_REQUIRE(A, HALA_ERR_ARG, "spmm: null matrix");
+        set_device(A->ctx);
+        spmm_cols(A, transa, transb, b_rows, b_cols, sizeof(double), B, ldb, C, ldc,
+                  [&](const void *x, void *y) {
+                      const int rc = hala_spmv_f64(A, transa, alpha, vals, (const double *)x, beta, (double *)y);
+                      if (rc != HALA_OK) throw status_error{rc, hala_last_error()};
+                  },
+                  nullptr);
+    });
+}
+
+int hala_spmm_f32(hala_csr A, char transa, char transb, int b_rows, int b_cols, float alpha, const float *vals,
+                  const float *B, int ldb, float beta, float *C, int ldc) {
+    return guard([&] {
+        HALA_REQUIRE(A, HALA_ERR_ARG, "spmm: null matrix");
+        set_device(A->ctx);
+        spmm_cols(A, transa, transb, b_rows, b_cols, sizeof(float), B, ldb, C, ldc,
+                  [&](const void *x, void *y) {
+                      const int rc = hala_spmv_f32(A, transa, alpha, vals, (const float *)x, beta, (float *)y);
+                      if (rc != HALA_OK) throw status_error{rc, hala_last_error()};
+                  },
+                  nullptr);
     });
 }
 

@@ -1,5 +1,6 @@
 // spmv.hpp -- CSR matrix handle and the sparse matrix-vector product cores.
 #pragma once
+#include <functional>
 #include "util.hpp"
 
 struct hala_csr_s {
@@ -105,5 +106,17 @@ void spmv_axpby(hala_csr A, const TV *vals, const TX *x, TY alpha, TY beta, TY *
 template <class T>
 void spmv_trans(hala_csr A, T alpha, const T *vals, const T *x, T beta, T *y);
 void ensure_transpose(hala_csr A);
+
+// n elements of esize (4, 8 or 16) bytes from src (stride ss elements) to dst (stride ds), on the context's stream
+void strided_copy(hala_ctx c, int n, size_t esize, const void *src, int64_t ss, void *dst, int64_t ds);
+
+// Sparse matrix times dense matrix, column by column as the CPU engine orders it
+// (sparse/hala_sparse_utils.hpp:127-167): C[:, i] = alpha op(A) op(B)[:, i] + beta C[:, i]
+// through `mv` (one product from a contiguous input x of op(A)'s column count
+// into y).  op(B) column i is B's column i ('N') or row i ('T', 'C'; `conj`
+// conjugates the gathered copy for 'C').
+void spmm_cols(hala_csr A, char transa, char transb, int b_rows, int b_cols, size_t esize, const void *B, int ldb,
+               void *C, int ldc, const std::function<void(const void *, void *)> &mv,
+               const std::function<void(void *, int)> &conj);
 
 }  // namespace hala_amd

@@ -40,7 +40,7 @@ __global__ void strided_copy_k(int n, const E *__restrict__ src, int64_t ss, E *
     if (i < n) dst[(int64_t)i * ds] = src[(int64_t)i * ss];
 }
 
-static void strided_copy(hala_ctx c, int n, size_t esize, const void *src, int64_t ss, void *dst, int64_t ds) {
+void strided_copy(hala_ctx c, int n, size_t esize, const void *src, int64_t ss, void *dst, int64_t ds) {
     const int g = (n + 255) / 256;
     if (esize == 4)
         strided_copy_k<int><<<g, 256, 0, c->stream>>>(n, (const int *)src, ss, (int *)dst, ds);

@@ -407,6 +407,28 @@ inline void spmv_(hala_csr A, char t, zf a, zf const *v, zf const *x, zf b, zf *
                            reinterpret_cast<float const *>(x), b.real(), b.imag(), reinterpret_cast<float *>(y)),
                "gpu_sparse_matrix::gemv()");
 }
+inline void spmm_(hala_csr A, char ta, char tb, int br, int bc, double a, double const *v, double const *B, int ldb,
+                  double b, double *C, int ldc) {
+    check_hala(hala_spmm_f64(A, ta, tb, br, bc, a, v, B, ldb, b, C, ldc), "gpu_sparse_matrix::gemm()");
+}
+inline void spmm_(hala_csr A, char ta, char tb, int br, int bc, float a, float const *v, float const *B, int ldb,
+                  float b, float *C, int ldc) {
+    check_hala(hala_spmm_f32(A, ta, tb, br, bc, a, v, B, ldb, b, C, ldc), "gpu_sparse_matrix::gemm()");
+}
+inline void spmm_(hala_csr A, char ta, char tb, int br, int bc, zd a, zd const *v, zd const *B, int ldb, zd b, zd *C,
+                  int ldc) {
+    check_hala(hala_spmm_z(A, ta, tb, br, bc, a.real(), a.imag(), reinterpret_cast<double const *>(v),
+                           reinterpret_cast<double const *>(B), ldb, b.real(), b.imag(), reinterpret_cast<double *>(C),
+                           ldc),
+               "gpu_sparse_matrix::gemm()");
+}
+inline void spmm_(hala_csr A, char ta, char tb, int br, int bc, zf a, zf const *v, zf const *B, int ldb, zf b, zf *C,
+                  int ldc) {
+    check_hala(hala_spmm_c(A, ta, tb, br, bc, a.real(), a.imag(), reinterpret_cast<float const *>(v),
+                           reinterpret_cast<float const *>(B), ldb, b.real(), b.imag(), reinterpret_cast<float *>(C),
+                           ldc),
+               "gpu_sparse_matrix::gemm()");
+}
 }  // namespace detail
 
 template <class VX, class VY>
@@ -500,6 +522,28 @@ class gpu_sparse_matrix {
         runtime_assert(beta == FPb(0) || y.size() >= out, "gemv: y too small");
         detail::spmv_(csr.get(), trans, (T)alpha, rvals, (T const *)x.data(), (T)beta, (T *)y.data());
     }
+    // C = alpha op(A) op(B) + beta C (gpu/hala_rocm_sparse_general.hpp:131-161); C grows to
+    // ldc x N when beta == 0 and it is smaller
+    template <typename FSA, class VB, typename FSB, class VC>
+    size_t gemm_buffer_size(char, char, int, int, FSA, VB const &, int, FSB, VC &&, int) const {
+        return 0;
+    }
+    template <typename FSA, class VB, typename FSB, class VC, class VT>
+    void gemm(char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb, FSB beta, VC &&C,
+              int ldc, VT &&) const {
+        gemm(transa, transb, b_rows, b_cols, alpha, B, ldb, beta, C, ldc);
+    }
+    template <typename FSA, class VB, typename FSB, class VC>
+    void gemm(char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb, FSB beta, VC &&C,
+              int ldc) const {
+        const int N = (transb == 'N' || transb == 'n') ? b_cols : b_rows;
+        if ((size_t)C.size() < (size_t)ldc * (size_t)N) {
+            runtime_assert(beta == FSB(0), "gemm: C too small for beta != 0");
+            C.resize((size_t)ldc * (size_t)N);
+        }
+        detail::spmm_(csr.get(), transa, transb, b_rows, b_cols, (T)alpha, rvals, (T const *)B.data(), ldb, (T)beta,
+                      (T *)C.data(), ldc);
+    }
     T const *values() const { return rvals; }
 
   private:
@@ -517,6 +561,12 @@ auto make_sparse_matrix(gpu_engine const &engine, int num_rows, int num_cols, in
 template <class VP, class VI, class VV>
 auto make_sparse_matrix(gpu_engine const &engine, int num_cols, VP const &pntr, VI const &indx, VV const &vals) {
     return gpu_sparse_matrix<scalar_of<VV>>(engine, num_cols, pntr, indx, vals);
+}
+// sparse/hala_sparse_blas.hpp:217-224
+template <typename MT, typename FSA, class VB, typename FSB, class VC>
+void sparse_gemm(MT const &matrix, char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb,
+                 FSB beta, VC &&C, int ldc) {
+    matrix.gemm(transa, transb, b_rows, b_cols, alpha, B, ldb, beta, C, ldc);
 }
 template <typename FPa, class VP, class VI, class VV, class VX, typename FPb, class VY>
 void sparse_gemv(gpu_engine const &engine, char trans, int M, int N, FPa alpha, VP const &pntr, VI const &indx,

@@ -91,6 +91,24 @@ struct mixed_sparse_matrix {
     void gemv(char trans, FPa alpha, VX const &x, FPb beta, VY &&y, VB &&) const {
         gemv(trans, alpha, x, beta, y);
     }
+    template <typename FSA, class VB, typename FSB, class VC>
+    size_t gemm_buffer_size(char, char, int, int, FSA, VB const &, int, FSB, VC &&, int) const {
+        return 0;
+    }
+    template <typename FSA, class VB, typename FSB, class VC>
+    void gemm(char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb, FSB beta, VC &&C,
+              int ldc) const {
+        const int N = (transb == 'N' || transb == 'n') ? b_cols : b_rows;
+        if ((size_t)C.size() < (size_t)ldc * (size_t)N) C.resize((size_t)ldc * (size_t)N);
+        auto gB = mat.engine().load(B);
+        auto gC = gpu_bind_vector(mat.engine(), C);
+        mat.gemm(transa, transb, b_rows, b_cols, alpha, gB, ldb, beta, gC, ldc);
+    }
+    template <typename FSA, class VB, typename FSB, class VC, class VT>
+    void gemm(char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb, FSB beta, VC &&C,
+              int ldc, VT &&) const {
+        gemm(transa, transb, b_rows, b_cols, alpha, B, ldb, beta, C, ldc);
+    }
 
     gpu_vector<int> rpntr, rindx;
     gpu_vector<T> rvals;

@@ -128,6 +128,16 @@ int hala_spmv_f32(hala_csr A, char trans, float alpha, const float *vals, const 
 /* FP32 values, FP32 products and accumulation, FP64 vectors:
  * y = alpha * double(sum_f32(vals32 * float(x))) + beta y. */
 int hala_spmv_f32x64(hala_csr A, double alpha, const float *vals, const double *x, double beta, double *y);
+/* sparse_gemm / gpu_sparse_matrix::gemm (gpu/hala_rocm_sparse_general.hpp:131-161):
+ * C = alpha op(A) op(B) + beta C, B b_rows x b_cols and C column-major with
+ * leading dimensions ldb, ldc; op(B) is B ('N'), B^T ('T') or B^H ('C').
+ * Column by column in the CPU engine's order (sparse/hala_sparse_utils.hpp:127-167),
+ * each column one hala_spmv product (bit-identical to it); beta == 0 does not
+ * read C.  Complex variants: hala_spmm_z / hala_spmm_c (alpha, beta as re, im). */
+int hala_spmm_f64(hala_csr A, char transa, char transb, int b_rows, int b_cols, double alpha, const double *vals,
+                  const double *B, int ldb, double beta, double *C, int ldc);
+int hala_spmm_f32(hala_csr A, char transa, char transb, int b_rows, int b_cols, float alpha, const float *vals,
+                  const float *B, int ldb, float beta, float *C, int ldc);
 
 /* ---- ILU(0) ------------------------------------------------------------- */
 /* gpu_ilu<T> and make_ilu(gpu_engine, ...): gpu/hala_gpu_ilu.hpp:45-195.
@@ -281,6 +291,10 @@ int hala_axpy_c(hala_ctx ctx, int64_t n, float alpha_re, float alpha_im, const f
 int hala_scal_c(hala_ctx ctx, int64_t n, float alpha_re, float alpha_im, float *x);
 int hala_spmv_c(hala_csr csr, char trans, float alpha_re, float alpha_im, const float *vals, const float *x,
                 float beta_re, float beta_im, float *y);
+int hala_spmm_z(hala_csr A, char transa, char transb, int b_rows, int b_cols, double alpha_re, double alpha_im,
+                const double *vals, const double *B, int ldb, double beta_re, double beta_im, double *C, int ldc);
+int hala_spmm_c(hala_csr A, char transa, char transb, int b_rows, int b_cols, float alpha_re, float alpha_im,
+                const float *vals, const float *B, int ldb, float beta_re, float beta_im, float *C, int ldc);
 int hala_ilu_factor_c(hala_ilu ilu, const float *vals);
 int hala_ilu_apply_c(hala_ilu ilu, const float *x, float *r);
 

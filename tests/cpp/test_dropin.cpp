@@ -181,6 +181,28 @@ static void typed_trsm(hala::gpu_engine const &engine, const char *name, double 
     run('U', up, ui, uv, 'N', 'N', 'T');
     run('U', up, ui, uv, 'T', 'N', 'N');
     std::printf("PASS sparse_trsm<%s>\n", name);
+    // sparse_gemm on the same lower triangle as a general matrix: C = 2 A Xref,
+    // then 'T' on it: C = 2 A^T Xref (tests/sparse_tests.hpp gemm cases)
+    for (char ta : {'N', 'T'}) {
+        auto gp = hala::make_gpu_vector(lp);
+        auto gi = hala::make_gpu_vector(li);
+        auto gv = hala::make_gpu_vector(lv);
+        auto A = hala::make_sparse_matrix(engine, M, gp, gi, gv);
+        auto gX = hala::make_gpu_vector(Xref);
+        hala::gpu_vector<T> gC;
+        hala::sparse_gemm(A, ta, 'N', M, nrhs, T(2.0), gX, M, T(0.0), gC, M);
+        std::vector<T> Cv;
+        gC.unload(Cv);
+        double err = 0.0;
+        for (int k = 0; k < nrhs; k++)
+            for (int r = 0; r < M; r++) {
+                T s = T(0);
+                for (int c = 0; c < M; c++) s += T(2) * opA(r, c, ta) * Xref[c + k * M];
+                err += std::abs(std::complex<double>(Cv[r + k * M]) - std::complex<double>(s));
+            }
+        CHECK(err < tol);
+    }
+    std::printf("PASS sparse_gemm<%s>\n", name);
 }
 
 int main() {
