@@ -38,8 +38,6 @@ struct hala_tri_s {
     hala_amd::tri_plan N, T;  // plain and transposed solves (T built on demand)
     hala_amd::csr_transpose tp;
     bool t_built = false;
-    const double *last_vals_n = nullptr, *last_vals_t = nullptr;
-    hala_amd::dbuf vals_copy_n, vals_copy_t;  // unused placeholders for future caching
 };
 
 namespace hala_amd {
