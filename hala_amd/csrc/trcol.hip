@@ -296,7 +296,7 @@ __global__ void init_range(int ncol, int nlev, int *__restrict__ colrange) {
 // Packed row records, one per slot, RB bytes each (a multiple of 16, read
 // with 16-byte loads): int16 dependency codes [LR] | diagonal T | values T[LR].
 template <class T>
-__global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, const int *__restrict__ code,
+__global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, int zidx, const int *__restrict__ code,
                          const int *__restrict__ src, const int *__restrict__ dg_src, const double *__restrict__ f,
                          unsigned char *__restrict__ rec) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -307,8 +307,11 @@ __global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, const int *
     T *ov = od + 1;
     const bool in = s < nslots;
     for (int k = 0; k < LR; k++) {
-        const int e = in ? src[(int64_t)s * LR + k] : -1;
-        oc[k] = in ? (short)code[(int64_t)s * LR + k] : (short)-1;
+        // kernel-side codes: >= 0 an LDS element (window entry, or the zero
+        // element zidx for no dependency: value 0); < 0 the external e as e - 32768
+        const int c = in ? code[(int64_t)s * LR + k] : -1;
+        const int e = (in && c != -1) ? src[(int64_t)s * LR + k] : -1;
+        oc[k] = (short)(c == -1 ? zidx : (c >= 0 ? c : (-c - 2) - 32768));
         ov[k] = e >= 0 ? (T)f[e] : T(0);
     }
     const int d = in ? dg_src[s] : -1;
@@ -336,12 +339,15 @@ tri_colplan &tri_colplan_reset(tri_colplan &q) {
     return q;
 }
 
+// LDS element (in values) holding 0: it follows the window and the external double buffer
+int col_zero_index(int K, int CP, int EL) { return K * CP + 2 * WAVE * EL; }
+
 // window (K levels x CP rows) + external double buffer (2 x 64 EL values) +
-// slots of the polls in flight (up to PE_MAX x 64 EL) + cell records
+// one zero element + slots of the polls in flight (up to PE_MAX x 64 EL) + cell records
 int64_t lds_bytes(int K, int CP, int EL, int nlevcol, size_t tsize) {
     const int pad = nlevcol + 3 * 8 + 2;  // read-ahead of the deepest pipeline (P = 8)
     const int64_t xc = (int64_t)WAVE * EL;
-    return (int64_t)K * CP * (int64_t)tsize + 2 * xc * (int64_t)tsize + PE_MAX * xc * 4 + (int64_t)pad * CI * 4;
+    return (int64_t)K * CP * (int64_t)tsize + (2 * xc + 2) * (int64_t)tsize + PE_MAX * xc * 4 + (int64_t)pad * CI * 4;
 }
 
 // bytes of one packed row record: int16 codes [LR] | diagonal | values [LR], 16-byte multiple
@@ -562,15 +568,16 @@ void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
     if (!q.ok) return;
     const int nslots = p.nchunk * WAVE, npad = nslots + BTMAX;
     const int RB = rec_bytes(q.L, f32 ? sizeof(float) : sizeof(double));
+    const int zidx = col_zero_index(q.K, q.CP, q.EL);
     dbuf &dst = f32 ? q.val32 : q.val64;
     dst.reserve((size_t)npad * RB);
     const int g = (npad + 255) / 256;
     if (f32) {
-        col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, q.code.as<int>(), q.src.as<int>(),
+        col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, zidx, q.code.as<int>(), q.src.as<int>(),
                                                  p.dg_src.as<int>(), factor, dst.as<unsigned char>());
         if (!q.xs32.p) q.xs32.reserve((size_t)npad * sizeof(float));
     } else {
-        col_pack<double><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, q.code.as<int>(), q.src.as<int>(),
+        col_pack<double><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, zidx, q.code.as<int>(), q.src.as<int>(),
                                                   p.dg_src.as<int>(), factor, dst.as<unsigned char>());
     }
     HALA_CHECK_LAUNCH();
@@ -726,11 +733,15 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     const int CP = a.CP, K = a.K;
     T *win = reinterpret_cast<T *>(smem);
     T *extb = win + (size_t)K * CP;
-    int *eslot = reinterpret_cast<int *>(extb + 2 * XC);  // slots of the polls in flight
+    const int zidx = K * CP + 2 * XC;                         // win[zidx] = 0: the operand of no dependency
+    int *eslot = reinterpret_cast<int *>(extb + 2 * XC + 2);  // slots of the polls in flight
     int *cis = eslot + PE * XC;
     __shared__ int col_s;
     const int t = threadIdx.x;
-    if (t == 0) col_s = (int)atomicAdd(a.ticket, 1u);
+    if (t == 0) {
+        col_s = (int)atomicAdd(a.ticket, 1u);
+        win[zidx] = T(0);
+    }
     __syncthreads();
     const int col = col_s;
     const int first = a.colrange[2 * col], last = a.colrange[2 * col + 1];
@@ -849,7 +860,7 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 const int lvl = first + lam;
                 const col_rec<T, MAXD> &r = rec[s];
                 const bool valid = r.slot >= 0;
-                const int eoff = K * CP + (lam & 1) * XC;  // the external buffer follows the window
+                const int eoff = 32768 + K * CP + (lam & 1) * XC;  // external code e - 32768 -> extb[parity][e]
                 T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
                 // all LDS reads of the step (the cell record of level lam + P and
                 // the row's operands) issued before the first is waited for
@@ -859,9 +870,9 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 T xs[MAXD];
 #pragma unroll
                 for (int k = 0; k < MAXD; k++) {
-                    const int cd = valid ? rec_code(r.q, k) : -1;
+                    const int cd = rec_code(r.q, k);  // lanes without a row carry zidx codes
                     cds[k] = cd;
-                    xs[k] = win[cd >= 0 ? cd : eoff + (cd < -1 ? -cd - 2 : 0)];
+                    xs[k] = win[cd >= 0 ? cd : cd + eoff];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -870,11 +881,12 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                     const T v = rec_val<T>(r.q, 2 * MAXD + (int)sizeof(T) * (k + 1));
                     const T x = xs[k];
                     if (MODE == TRI_SUB) {
-                        const T nacc = acc - v * x;
-                        acc = cd != -1 ? nacc : acc;
+                        // no dependency: acc - 0 * 0, which is acc exactly (also for -0)
+                        acc = acc - v * x;
                     } else {
+                        // a sum starting at +0 would turn -0 into +0: skip instead
                         const T nsum = sum + v * x;
-                        sum = cd != -1 ? nsum : sum;
+                        sum = cd != zidx ? nsum : sum;
                     }
                 }
                 if (MODE == TRI_SUM) acc = a.alpha * r.xin - sum;
