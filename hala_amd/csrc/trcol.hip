@@ -171,12 +171,14 @@ __global__ void col_records(int nslots, const int *__restrict__ cmap, int ncol, 
                             const int *__restrict__ lchunk, const int *__restrict__ cstart,
                             const int *__restrict__ slot_of_row, const int *__restrict__ dptr,
                             const int *__restrict__ ddep, const int *__restrict__ dsrc, const int *__restrict__ epos,
-                            int *__restrict__ code, int *__restrict__ src, int *__restrict__ ext_slot) {
+                            const int *__restrict__ colrange, int *__restrict__ code, int *__restrict__ src,
+                            int *__restrict__ ext_slot, int *__restrict__ spar) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     const int r = chunk_row[s];
     if (r < 0) return;
     const int l = lev[r];
+    spar[s] = (l - colrange[2 * cmap[r]]) & 1;
     const int S0 = lchunk[l] * WAVE, Wl = (lchunk[l + 1] - lchunk[l]) * WAVE;
     const int q = s - S0;
     const int c = cmap[r];
@@ -296,9 +298,10 @@ __global__ void init_range(int ncol, int nlev, int *__restrict__ colrange) {
 // Packed row records, one per slot, RB bytes each (a multiple of 16, read
 // with 16-byte loads): int16 dependency codes [LR] | diagonal T | values T[LR].
 template <class T>
-__global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, int zidx, const int *__restrict__ code,
-                         const int *__restrict__ src, const int *__restrict__ dg_src, const double *__restrict__ f,
-                         unsigned char *__restrict__ rec) {
+__global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, int kcp, int xc, const int *__restrict__ code,
+                         const int *__restrict__ spar, const int *__restrict__ src, const int *__restrict__ dg_src,
+                         const double *__restrict__ f, unsigned char *__restrict__ rec) {
+    const int zidx = kcp + 2 * xc;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots_pad) return;
     unsigned char *o = rec + (size_t)s * RB;
@@ -307,11 +310,12 @@ __global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, int zidx, c
     T *ov = od + 1;
     const bool in = s < nslots;
     for (int k = 0; k < LR; k++) {
-        // kernel-side codes: >= 0 an LDS element (window entry, or the zero
-        // element zidx for no dependency: value 0); < 0 the external e as e - 32768
+        // kernel-side codes are LDS element indices: a window entry, external e
+        // of the slot's level in the buffer of that level's parity, or the zero
+        // element zidx for no dependency (value 0)
         const int c = in ? code[(int64_t)s * LR + k] : -1;
         const int e = (in && c != -1) ? src[(int64_t)s * LR + k] : -1;
-        oc[k] = (short)(c == -1 ? zidx : (c >= 0 ? c : (-c - 2) - 32768));
+        oc[k] = (short)(c == -1 ? zidx : (c >= 0 ? c : kcp + spar[s] * xc + (-c - 2)));
         ov[k] = e >= 0 ? (T)f[e] : T(0);
     }
     const int d = in ? dg_src[s] : -1;
@@ -336,11 +340,9 @@ tri_colplan &tri_colplan_reset(tri_colplan &q) {
     q.code.release();
     q.src.release();
     q.ext_slot.release();
+    q.spar.release();
     return q;
 }
-
-// LDS element (in values) holding 0: it follows the window and the external double buffer
-int col_zero_index(int K, int CP, int EL) { return K * CP + 2 * WAVE * EL; }
 
 // window (K levels x CP rows) + external double buffer (2 x 64 EL values) +
 // one zero element + slots of the polls in flight (up to PE_MAX x 64 EL) + cell records
@@ -512,12 +514,15 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         fill_i32_k<<<(int)((q.nrec + 255) / 256), 256, 0, c->stream>>>(q.src.as<int>(), q.nrec, -1);
         HALA_CHECK_LAUNCH();
         HALA_CHECK_LAUNCH();
+        q.spar.reserve((size_t)(nslots > 0 ? nslots : 1) * sizeof(int));
+        fill_i32_k<<<(nslots + 255) / 256, 256, 0, c->stream>>>(q.spar.as<int>(), nslots, 0);
         q.ext_slot.reserve((size_t)(next + BTMAX) * sizeof(int));
         HALA_HIP(hipMemsetAsync(q.ext_slot.p, 0, q.ext_slot.bytes, c->stream));
         col_records<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
             nslots, cmap.as<int>(), ncol, K, CP, q.L, nlev, p.chunk_row.as<int>(), p.lev.as<int>(),
             p.lvl_chunk.as<int>(), cstart.as<int>(), p.slot_of_row.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
-            p.dsrc.as<int>(), epos.as<int>(), q.code.as<int>(), q.src.as<int>(), q.ext_slot.as<int>());
+            p.dsrc.as<int>(), epos.as<int>(), colrange.as<int>(), q.code.as<int>(), q.src.as<int>(),
+            q.ext_slot.as<int>(), q.spar.as<int>());
         HALA_CHECK_LAUNCH();
         // each external value staged once per cell
         {
@@ -568,16 +573,18 @@ void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
     if (!q.ok) return;
     const int nslots = p.nchunk * WAVE, npad = nslots + BTMAX;
     const int RB = rec_bytes(q.L, f32 ? sizeof(float) : sizeof(double));
-    const int zidx = col_zero_index(q.K, q.CP, q.EL);
+    const int kcp = q.K * q.CP, xc = WAVE * q.EL;
     dbuf &dst = f32 ? q.val32 : q.val64;
     dst.reserve((size_t)npad * RB);
     const int g = (npad + 255) / 256;
     if (f32) {
-        col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, zidx, q.code.as<int>(), q.src.as<int>(),
+        col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, kcp, xc, q.code.as<int>(), q.spar.as<int>(),
+                                                 q.src.as<int>(),
                                                  p.dg_src.as<int>(), factor, dst.as<unsigned char>());
         if (!q.xs32.p) q.xs32.reserve((size_t)npad * sizeof(float));
     } else {
-        col_pack<double><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, zidx, q.code.as<int>(), q.src.as<int>(),
+        col_pack<double><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, kcp, xc, q.code.as<int>(), q.spar.as<int>(),
+                                                  q.src.as<int>(),
                                                   p.dg_src.as<int>(), factor, dst.as<unsigned char>());
     }
     HALA_CHECK_LAUNCH();
@@ -860,7 +867,6 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 const int lvl = first + lam;
                 const col_rec<T, MAXD> &r = rec[s];
                 const bool valid = r.slot >= 0;
-                const int eoff = 32768 + K * CP + (lam & 1) * XC;  // external code e - 32768 -> extb[parity][e]
                 T acc = (MODE == TRI_SUB) ? r.xin : T(0), sum = T(0);
                 // all LDS reads of the step (the cell record of level lam + P and
                 // the row's operands) issued before the first is waited for
@@ -870,9 +876,9 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 T xs[MAXD];
 #pragma unroll
                 for (int k = 0; k < MAXD; k++) {
-                    const int cd = rec_code(r.q, k);  // lanes without a row carry zidx codes
+                    const int cd = rec_code(r.q, k);  // an LDS element (see col_pack)
                     cds[k] = cd;
-                    xs[k] = win[cd >= 0 ? cd : cd + eoff];
+                    xs[k] = win[cd];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll

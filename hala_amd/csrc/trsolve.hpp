@@ -46,6 +46,7 @@ struct tri_colplan {
     int why = 0;      // when not built: 1 small, 2 long rows, 3 cells too wide, 4 LDS, 5/6 externals
     dbuf ci, colrange;           // cell records (column-major, 5 ints each), level range per column
     dbuf code, src, ext_slot;    // dependency codes and factor positions (k-major per level); external slots
+    dbuf spar;                   // per slot: parity of its level within its column (which external buffer it reads)
     dbuf val64, val32;           // packed row records (see col_pack)
     dbuf xs64, xs32;             // input, gathered into slot order per sweep
 };
