@@ -839,11 +839,13 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     } else {
         // ---------------- compute waves ----------------
         constexpr int NQ = col_rec<T, MAXD>::NQ;
+        // 32-bit byte offsets from uniform bases (the plan keeps them below 2^31)
         auto load_rec_at = [&](col_rec<T, MAXD> &r, int slot0, int size) {
-            const int s = slot0 + t;
-            r.slot = t < size ? s : -1;
-            r.xin = a.xs[s];  // slot arrays are padded past their end: idle lanes read in bounds
-            const int4 *src = a.rec + (size_t)s * NQ;
+            const unsigned s = (unsigned)(slot0 + t);
+            r.slot = t < size ? (int)s : -1;
+            // slot arrays are padded past their end: idle lanes read in bounds
+            r.xin = *reinterpret_cast<const T *>(reinterpret_cast<const char *>(a.xs) + s * (unsigned)sizeof(T));
+            const int4 *src = reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(a.rec) + s * (unsigned)(16 * NQ));
 #pragma unroll
             for (int i = 0; i < NQ; i++) r.q[i] = src[i];
         };
@@ -898,7 +900,9 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 if (MODE == TRI_SUM) acc = a.alpha * r.xin - sum;
                 if (!UNIT) acc = acc / rec_val<T>(r.q, 2 * MAXD);
                 win[(lvl % K) * CP + t] = acc;
-                if (valid) st_agent(a.out + r.slot, acc);
+                if (valid)
+                    st_agent(reinterpret_cast<T *>(reinterpret_cast<char *>(a.out) + (unsigned)r.slot * (unsigned)sizeof(T)),
+                             acc);
                 load_rec_at(rec[s], nslot0, nsize);
                 lds_barrier();
             }
@@ -939,6 +943,9 @@ bool col_usable(hala_ctx c, const tri_plan &p, bool is64) {
     const tri_colplan &q = p.col;
     if (!q.ok || !c->opt.tri_col) return false;
     if (q.CP > (q.L == 4 ? CPMAX : CPMAX_LONG)) return false;
+    // the kernel addresses records, inputs and outputs by 32-bit byte offsets
+    const int64_t npad = (int64_t)p.nchunk * WAVE + BTMAX;
+    if (npad * rec_bytes(q.L, sizeof(double)) > INT_MAX) return false;
     return is64 ? (q.val64.p != nullptr) : (q.val32.p != nullptr);
 }
 
