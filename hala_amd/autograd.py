@@ -8,6 +8,9 @@ backward : the forward solve is recomputed (no Krylov state is stored), then
            the adjoint A^T lambda = dL/dx is solved with (LU)^-T-preconditioned
            CG, dL/dvals_j = -lambda_row x_col, and dL/dkappa_c =
            sum_e cw[e] dL/dvals[cidx[e]] over the coefficient map.
+The ILU(0) symbolic analysis (diagonal, level sets, sweep layouts, transposed
+plans) depends only on the pattern: it is built once per problem, and every
+solve re-runs only the numeric factorisation of the current values.
 """
 import numpy as np
 import torch
@@ -47,6 +50,15 @@ class SparseLinearProblem:
         self.tol = rtol * float(torch.linalg.vector_norm(self.f))
         self.max_iter = max_iter
         self.last_iters = (0, 0)
+        self.ilu = None  # analysis of the pattern, kept; refactored for each kappa
+
+    def factor(self, vals):
+        """ILU(0) of the current values: the pattern's analysis is built on the first call only."""
+        if self.ilu is None:
+            self.ilu = make_ilu(self.engine, self.pntr, self.indx, vals)
+        else:
+            self.ilu.factor(vals)
+        return self.ilu
 
     def assemble(self, kappa):
         assemble(self.engine, self.aptr, self.aidx, self.aw, kappa.detach().contiguous(), self.vals)
@@ -55,7 +67,7 @@ class SparseLinearProblem:
     def forward_solve(self, kappa):
         """Returns (x, ilu) for A(kappa) x = f."""
         vals = self.assemble(kappa)
-        ilu = make_ilu(self.engine, self.pntr, self.indx, vals)
+        ilu = self.factor(vals)
         x = self.engine.vector(self.n)
         it = solve_cg(self.engine, stop_criteria(self.tol, self.max_iter), self.pntr, self.indx, vals, ilu, self.f, x,
                       matrix=self.A)
@@ -64,10 +76,14 @@ class SparseLinearProblem:
 
     def gradient(self, kappa, gx):
         """dL/dkappa for dL/dx = gx (recomputing the forward state)."""
+        g = gx.detach().contiguous().to(torch.float64)
+        gnorm = float(torch.linalg.vector_norm(g))
+        if gnorm == 0.0:  # the loss does not depend on x: lambda = 0, so is the gradient
+            self.last_iters = (self.last_iters[0], 0)
+            return torch.zeros(int(self.cptr.numel()) - 1, dtype=torch.float64, device=g.device)
         x, ilu = self.forward_solve(kappa)
         lam = self.engine.vector(self.n)
-        g = gx.detach().contiguous().to(torch.float64)
-        tol = stop_criteria(self.rtol * float(torch.linalg.vector_norm(g)), self.max_iter)
+        tol = stop_criteria(self.rtol * gnorm, self.max_iter)
         it = solve_cg_t(self.engine, tol, self.A, self.vals, ilu, g, lam)
         self.last_iters = (self.last_iters[0], it)
         gvals = torch.empty(self.nnz, dtype=torch.float64, device=x.device)

@@ -7,7 +7,7 @@ namespace hala_amd {
 
 template <class T, int K>
 __global__ __launch_bounds__(256) void cdot_kernel(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
-                                                   int64_t nb, T *part, unsigned *counter, T *out, int sq) {
+                                                   int64_t nb, T *part, int *err, T *out, int sq) {
     __shared__ T sh[CB * K];
     T v[K];
 #pragma unroll
@@ -25,8 +25,8 @@ __global__ __launch_bounds__(256) void cdot_kernel(int64_t n, const T *__restric
             if (b < nb) st_agent(part + b, v[k]);
         }
     }
-    if (last_block(counter, gridDim.x)) {
-        T r = final_pass<T>(part, nb, sh);
+    if (is_collector()) {
+        T r = final_pass<T>(part, nb, sh, err);
         if (threadIdx.x == 0) out[0] = sq == 2 ? (T)sqrt(r) : r;
     }
 }
@@ -41,7 +41,7 @@ void cdot_launch(hala_ctx c, int64_t n, const T *x, const T *y, T *out_dev, int 
     else part = reinterpret_cast<T *>(partials32(c, nb));
     int64_t g = (nb + K - 1) / K;
     HALA_REQUIRE(g < (1ll << 31), HALA_ERR_ARG, "vector too long");
-    cdot_kernel<T, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, part, lb_counter(c, LB_DOT), out_dev, mode);
+    cdot_kernel<T, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, part, wait_error_flag(c), out_dev, mode);
     HALA_CHECK_LAUNCH();
 }
 template void cdot_launch<double>(hala_ctx, int64_t, const double *, const double *, double *, int);
@@ -88,7 +88,7 @@ template <int K>
 __global__ __launch_bounds__(256) void gemv_t_kernel(int m, const double *__restrict__ A, int lda,
                                                      const double *__restrict__ x, double alpha, double beta,
                                                      double *__restrict__ y, int64_t nb, double *part,
-                                                     unsigned *counters_) {
+                                                     int *err) {
     __shared__ double sh[CB * K];
     const int col = blockIdx.y;
     const double *a = A + (int64_t)col * lda;
@@ -107,8 +107,8 @@ __global__ __launch_bounds__(256) void gemv_t_kernel(int m, const double *__rest
             if (b < nb) st_agent(pc + b, v[k]);
         }
     }
-    if (last_block(counters_ + col * (LB_GROUPS + 1), gridDim.x)) {
-        double r = final_pass<double>(pc, nb, sh);
+    if (is_collector()) {
+        double r = final_pass<double>(pc, nb, sh, err);
         if (threadIdx.x == 0) y[col] = (beta == 0.0) ? alpha * r + 0.0 : alpha * r + beta * y[col];
     }
 }
@@ -121,9 +121,16 @@ static T dot_host(hala_ctx c, int64_t n, const T *x, const T *y, int mode) {
     if (n == 0) return T(0);
     T *out = reinterpret_cast<T *>(c->scal.as<double>() + S_DOT);
     cdot_launch<T>(c, n, x, y, out, mode);
-    HALA_HIP(hipMemcpyAsync(c->host_pinned, out, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    // one copy brings back the result and the wait-error flag (same scalar block)
+    const char *lo = reinterpret_cast<const char *>(out);
+    const size_t span = reinterpret_cast<const char *>(wait_error_flag(c) + 1) - lo;
+    HALA_HIP(hipMemcpyAsync(c->host_pinned, lo, span, hipMemcpyDeviceToHost, c->stream));
     HALA_HIP(hipStreamSynchronize(c->stream));
-    return *reinterpret_cast<T *>(c->host_pinned);
+    int flag;
+    std::memcpy(&flag, reinterpret_cast<const char *>(c->host_pinned) + (span - sizeof(int)), sizeof(int));
+    const T r = *reinterpret_cast<T *>(c->host_pinned);
+    check_wait_error_host(c, flag);
+    return r;
 }
 
 template <class K_, class... A>
@@ -235,7 +242,7 @@ int hala_gemv_f64(hala_ctx c, char trans, int m, int n, double alpha, const doub
             double *part = partials64(c, (size_t)nb * n);
             dim3 g((unsigned)((nb + K - 1) / K), (unsigned)n);
             gemv_t_kernel<K><<<g, CB, 0, c->stream>>>(m, A, lda, x, alpha, beta, y, nb, part,
-                                                     lb_counter(c, LB_GEMV));
+                                                     wait_error_flag(c));
             HALA_CHECK_LAUNCH();
         }
     });

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void cg_residual_kernel(int n, const int *__re
 template <int PREC>
 __global__ __launch_bounds__(256) void cg_start_kernel(int n, const double *__restrict__ r,
                                                        const double *__restrict__ z, const double *__restrict__ d,
-                                                       double *__restrict__ p, double *part, unsigned *cnt,
+                                                       double *__restrict__ p, double *part, int *err,
                                                        cg_state *st) {
     __shared__ double sh[CB * EK];
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void cg_start_kernel(int n, const double *__re
             int64_t b = (int64_t)blockIdx.x * EK + k;
             if (b < nb) st_agent(part + b, v[k]);
         }
-    if (last_block(cnt, gridDim.x)) {
-        double t = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) st->zr = t;
     }
 }
@@ -83,7 +83,7 @@ template <bool NT>
 __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restrict__ pntr,
                                                       const int *__restrict__ indx, const double *__restrict__ vals,
                                                       const double *__restrict__ p, double *__restrict__ ap,
-                                                      double *part, unsigned *cnt, cg_state *st) {
+                                                      double *part, int *err, cg_state *st) {
     if (st->done) return;
     __shared__ double prod[spmv_tile<double>::value];
     double *sh = prod;  // LDS reused for the tree
@@ -101,8 +101,8 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
         v = tree256_1<double>(v, sh);
         if (threadIdx.x == 0) st_agent(part + blk, v);
     }
-    if (last_block(cnt, gridDim.x)) {
-        double t = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) {
             st->alpha = st->zr / t;
             st->it = st->it + 1;
@@ -119,7 +119,7 @@ template <int PREC>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__restrict__ p,
                                                         const double *__restrict__ ap, const double *__restrict__ d,
                                                         double *__restrict__ x, double *__restrict__ r, double *part,
-                                                        double *part2, unsigned *cnt, cg_state *st,
+                                                        double *part2, int *err, cg_state *st,
                                                         const int *__restrict__ lslot = nullptr,
                                                         double *__restrict__ lxs = nullptr) {
     if (st->done) return;
@@ -161,10 +161,10 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
                 if (PREC == PREC_JAC) st_agent(part2 + b, v[EK + k]);
             }
         }
-    if (last_block(cnt, gridDim.x)) {
-        double rr = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double rr = final_pass<double>(part, nb, sh, err);
         double rz = 0.0;
-        if (PREC == PREC_JAC) rz = final_pass<double>(part2, nb, sh);
+        if (PREC == PREC_JAC) rz = final_pass<double>(part2, nb, sh, err);
         if (threadIdx.x == 0) {
             double rn = sqrt(rr);
             st->rn = rn;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
 
 // nzr = <r, z>; beta = nzr / zr; zr = nzr   (ILU preconditioner)
 __global__ __launch_bounds__(256) void cg_rz_kernel(int n, const double *__restrict__ r,
-                                                    const double *__restrict__ z, double *part, unsigned *cnt,
+                                                    const double *__restrict__ z, double *part, int *err,
                                                     cg_state *st) {
     if (st->done) return;
     __shared__ double sh[CB * EK];
@@ -203,8 +203,8 @@ __global__ __launch_bounds__(256) void cg_rz_kernel(int n, const double *__restr
             int64_t b = (int64_t)blockIdx.x * EK + k;
             if (b < nb) st_agent(part + b, v[k]);
         }
-    if (last_block(cnt, gridDim.x)) {
-        double t = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) {
             st->beta = t / st->zr;
             st->zr = t;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void cg_rz_kernel(int n, const double *__restr
 __global__ __launch_bounds__(256) void cg_zscatter_rz_kernel(int n, const int *__restrict__ uslot,
                                                              const double *__restrict__ uout,
                                                              const double *__restrict__ r, double *__restrict__ z,
-                                                             double *part, unsigned *cnt, cg_state *st) {
+                                                             double *part, int *err, cg_state *st) {
     if (st->done) return;
     __shared__ double sh[CB * EK];
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
@@ -239,8 +239,8 @@ __global__ __launch_bounds__(256) void cg_zscatter_rz_kernel(int n, const int *_
             int64_t b = (int64_t)blockIdx.x * EK + k;
             if (b < nb) st_agent(part + b, v[k]);
         }
-    if (last_block(cnt, gridDim.x)) {
-        double t = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) {
             st->beta = t / st->zr;
             st->zr = t;
@@ -324,44 +324,44 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
         else ilu_apply64(P.M, in, out, g);
     };
     switch (P.prec) {
-    case PREC_ID: cg_start_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, r, nullptr, nullptr, p, part, lb_counter(c, LB_CG_START), st); break;
+    case PREC_ID: cg_start_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, r, nullptr, nullptr, p, part, wait_error_flag(c), st); break;
     case PREC_JAC:
-        cg_start_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, r, nullptr, P.pdiag, p, part, lb_counter(c, LB_CG_START), st);
+        cg_start_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, r, nullptr, P.pdiag, p, part, wait_error_flag(c), st);
         break;
     default:
         precond(r, z, nullptr);
-        cg_start_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, r, z, nullptr, p, part, lb_counter(c, LB_CG_START), st);
+        cg_start_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, r, z, nullptr, p, part, wait_error_flag(c), st);
     }
     HALA_CHECK_LAUNCH();
 
     auto one_pass = [&]() {
         if (nt)
-            cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
+            cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, wait_error_flag(c), st);
         else
-            cg_spmv_kernel<false><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, lb_counter(c, LB_CG_SPMV), st);
+            cg_spmv_kernel<false><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, wait_error_flag(c), st);
         switch (P.prec) {
         case PREC_ID:
-            cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
+            cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, wait_error_flag(c), st);
             cg_pupdate_kernel<PREC_ID><<<gp, 256, 0, c->stream>>>(n, nullptr, r, nullptr, p, st);
             break;
         case PREC_JAC:
-            cg_update_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, p, ap, P.pdiag, x, r, part, part2, lb_counter(c, LB_CG_UPDATE), st);
+            cg_update_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, p, ap, P.pdiag, x, r, part, part2, wait_error_flag(c), st);
             cg_pupdate_kernel<PREC_JAC><<<gp, 256, 0, c->stream>>>(n, nullptr, r, P.pdiag, p, st);
             break;
         default:
             if (fused) {  // the sweeps' gather and scatter passes fused into the update and r.z
                 tri_plan &L = P.M->L, &U = P.M->U;
                 cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
-                                                                      lb_counter(c, LB_CG_UPDATE), st,
+                                                                      wait_error_flag(c), st,
                                                                       L.slot_of_row.as<int>(), L.col.xs64.as<double>());
                 ilu_apply64_mid(P.M, gate);
                 cg_zscatter_rz_kernel<<<ge, 256, 0, c->stream>>>(n, U.slot_of_row.as<int>(), U.out64.as<double>(), r, z,
-                                                                 part, lb_counter(c, LB_CG_RZ), st);
+                                                                 part, wait_error_flag(c), st);
             } else {
                 cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
-                                                                      lb_counter(c, LB_CG_UPDATE), st);
+                                                                      wait_error_flag(c), st);
                 precond(r, z, gate);
-                cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, lb_counter(c, LB_CG_RZ), st);
+                cg_rz_kernel<<<ge, 256, 0, c->stream>>>(n, r, z, part, wait_error_flag(c), st);
             }
             cg_pupdate_kernel<PREC_ILU><<<gp, 256, 0, c->stream>>>(n, z, nullptr, nullptr, p, st);
         }

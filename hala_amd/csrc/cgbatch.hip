@@ -67,7 +67,8 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *sh, doubl
 // p_k = z_k (z = r for the identity), zr_k = <r_k, z_k>
 template <bool IDENT>
 __global__ __launch_bounds__(256) void cgb_start(int n, const double *__restrict__ R, const double *__restrict__ Z,
-                                                 double *__restrict__ Pv, double *part, double *zr) {
+                                                 double *__restrict__ Pv, double *part, double *zr,
+                                                 int *err) {
     __shared__ double sh[CB * EKB];
     const int64_t nb = ((int64_t)n + CB - 1) / CB, o = (int64_t)blockIdx.y * n;
     double v[EKB];
@@ -84,8 +85,8 @@ __global__ __launch_bounds__(256) void cgb_start(int n, const double *__restrict
     }
     double *pk = part + blockIdx.y * nb;
     store_partials<EKB>(v, sh, pk, nb);
-    if (last_block(nullptr, gridDim.x)) {
-        double t = final_pass<double>(pk, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(pk, nb, sh, err);
         if (threadIdx.x == 0) zr[blockIdx.y] = t;
     }
 }
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void cgb_start(int n, const double *__restrict
 __global__ __launch_bounds__(256) void cgb_spmm(int n, int c0, int nc, const int *__restrict__ pntr,
                                                 const int *__restrict__ indx, const double *__restrict__ vals,
                                                 const double *__restrict__ Pv, double *__restrict__ AP, double *part,
-                                                const double *zr, double *al, cgb_state *st) {
+                                                const double *zr, double *al, cgb_state *st, int *err) {
     if (st->done) return;
     constexpr int TILE = spmv_tile<double>::value;
     __shared__ double prod[TILE];
@@ -139,9 +140,9 @@ __global__ __launch_bounds__(256) void cgb_spmm(int n, int c0, int nc, const int
         v = tree256_1<double>(v, sh);
         if (threadIdx.x == 0) st_agent(part + (int64_t)(c0 + k) * nb + blockIdx.x, v);
     }
-    if (last_block(nullptr, gridDim.x)) {
+    if (is_collector()) {
         for (int k = 0; k < nc; k++) {
-            double t = final_pass<double>(part + (int64_t)(c0 + k) * nb, nb, sh);
+            double t = final_pass<double>(part + (int64_t)(c0 + k) * nb, nb, sh, err);
             if (threadIdx.x == 0) al[c0 + k] = zr[c0 + k] / t;
         }
         if (threadIdx.x == 0 && c0 == 0) st->it = st->it + 1;
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void cgb_spmm(int n, int c0, int nc, const int
 // x_k += a_k p_k; r_k += (-a_k) Ap_k; rr_k = <r_k, r_k>
 __global__ __launch_bounds__(256) void cgb_update(int n, const double *__restrict__ Pv, const double *__restrict__ AP,
                                                   double *__restrict__ X, double *__restrict__ R, double *part,
-                                                  const double *al, double *rr, const cgb_state *st) {
+                                                  const double *al, double *rr, const cgb_state *st, int *err) {
     if (st->done) return;
     __shared__ double sh[CB * EKB];
     const int64_t nb = ((int64_t)n + CB - 1) / CB, o = (int64_t)blockIdx.y * n;
@@ -171,8 +172,8 @@ __global__ __launch_bounds__(256) void cgb_update(int n, const double *__restric
     }
     double *pk = part + blockIdx.y * nb;
     store_partials<EKB>(v, sh, pk, nb);
-    if (last_block(nullptr, gridDim.x)) {
-        double t = final_pass<double>(pk, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(pk, nb, sh, err);
         if (threadIdx.x == 0) rr[blockIdx.y] = t;
     }
 }
@@ -193,7 +194,8 @@ __global__ void cgb_stop(int nrhs, const double *rr, cgb_state *st) {
 // nzr_k = <r_k, z_k>; b_k = nzr_k / zr_k; zr_k = nzr_k
 template <bool IDENT>
 __global__ __launch_bounds__(256) void cgb_rz(int n, const double *__restrict__ R, const double *__restrict__ Z,
-                                              double *part, double *zr, double *al, const cgb_state *st) {
+                                              double *part, double *zr, double *al, const cgb_state *st,
+                                              int *err) {
     if (st->done) return;
     __shared__ double sh[CB * EKB];
     const int64_t nb = ((int64_t)n + CB - 1) / CB, o = (int64_t)blockIdx.y * n;
@@ -206,8 +208,8 @@ __global__ __launch_bounds__(256) void cgb_rz(int n, const double *__restrict__ 
     }
     double *pk = part + blockIdx.y * nb;
     store_partials<EKB>(v, sh, pk, nb);
-    if (last_block(nullptr, gridDim.x)) {
-        double t = final_pass<double>(pk, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(pk, nb, sh, err);
         if (threadIdx.x == 0) {
             al[blockIdx.y] = t / zr[blockIdx.y];
             zr[blockIdx.y] = t;
@@ -267,24 +269,24 @@ void run_cg_batch(hala_ctx c, hala_csr A, const double *vals, hala_ilu M, int nr
         for (int k = 0; k < nrhs; k++) ilu_apply64(M, R + (size_t)k * n, Z + (size_t)k * n, g);
     };
     if (ident) {
-        cgb_start<true><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, nullptr, Pv, part, zr);
+        cgb_start<true><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, nullptr, Pv, part, zr, wait_error_flag(c));
     } else {
         precond(nullptr);
-        cgb_start<false><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, Z, Pv, part, zr);
+        cgb_start<false><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, Z, Pv, part, zr, wait_error_flag(c));
     }
     HALA_CHECK_LAUNCH();
     auto one_pass = [&]() {
         for (int c0 = 0; c0 < nrhs; c0 += CGB_GROUP)
             cgb_spmm<<<gs, 256, 0, c->stream>>>(n, c0, std::min(CGB_GROUP, nrhs - c0), A->pntr, A->indx, vals, Pv, AP,
-                                                part, zr, al, st);
-        cgb_update<<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, Pv, AP, X, R, part, al, rr, st);
+                                                part, zr, al, st, wait_error_flag(c));
+        cgb_update<<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, Pv, AP, X, R, part, al, rr, st, wait_error_flag(c));
         cgb_stop<<<1, 1, 0, c->stream>>>(nrhs, rr, st);
         if (ident) {
-            cgb_rz<true><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, nullptr, part, zr, al, st);
+            cgb_rz<true><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, nullptr, part, zr, al, st, wait_error_flag(c));
             cgb_pupdate<true><<<dim3(gp, nrhs), 256, 0, c->stream>>>(n, R, nullptr, Pv, al, st);
         } else {
             precond(gate);
-            cgb_rz<false><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, Z, part, zr, al, st);
+            cgb_rz<false><<<dim3(ge, nrhs), 256, 0, c->stream>>>(n, R, Z, part, zr, al, st, wait_error_flag(c));
             cgb_pupdate<false><<<dim3(gp, nrhs), 256, 0, c->stream>>>(n, nullptr, Z, Pv, al, st);
         }
         HALA_CHECK_LAUNCH();

@@ -196,6 +196,9 @@ struct hala_options {
     int tri_col_split = 0;     // columns per slab of planes (0: automatic, 1: contiguous row blocks)
     int cg_fuse = 1;           // ILU-PCG: the sweeps' gather/scatter fused into the update and r.z kernels
     int ilu32_jacobi = 0;      // FP32 ILU applies between FP64 vectors: k > 0 Jacobi sweeps per triangle (0: exact)
+    int sync_checks = 1;       // standalone sweeps (ILU apply, trsv) wait for the device and report a wait that
+                               // gave up before returning; 0: they return at once and such an error is reported
+                               // by the next call that synchronises the context (hala_ctx_sync included)
 };
 
 struct hala_ctx_s {
@@ -209,7 +212,7 @@ struct hala_ctx_s {
     hala_amd::dbuf red32;    // reduction partials, FP32 (sentinel between reductions)
     hala_amd::dbuf scal;     // device scalars (8 KiB), zero-initialised at creation
     hala_amd::dbuf vec;      // solver work vectors
-    hala_amd::dbuf lbc;      // last-block arrival counters (zeroed at creation, self re-arming)
+    hala_amd::dbuf lbc;      // ticket counters of the triangular-solve pool (zeroed at creation, self re-arming)
     double *host_pinned = nullptr;  // pinned host landing zone for scalars
     // temporaries of the analysis passes (see tmp_scope)
     std::vector<std::unique_ptr<hala_amd::dbuf>> tmp;
@@ -333,18 +336,29 @@ __device__ __forceinline__ T tree256_1(T v, T *sh) {
 
 // Partials of a reduction are value-as-flag slots: the buffer holds sentinels
 // between reductions; each block stores its partials with agent-scope stores,
-// and the block that finishes last polls every slot until it is no longer the
+// and the collecting block polls every slot until it is no longer the
 // sentinel, then re-arms it.  No release/acquire fence is needed (on gfx950 an
 // agent-scope fence writes back and invalidates the whole L2, which costs far
 // more than the reduction itself).
+//
+// A wait is bounded: after 2^24 polls (seconds; never expected) it raises the
+// context's wait-error flag `err` and the partial counts as 0.  The host reads
+// the flag before it reports a result (check_wait_error), throws, and re-arms
+// every partial slot once the device is idle, so a partial that lands late
+// cannot be mistaken for a later reduction's.
+constexpr unsigned RED_SPIN_LIMIT = 1u << 24;
+
 template <class T>
-__device__ __forceinline__ T poll_slot(T *p) {
+__device__ __forceinline__ T poll_slot(T *p, int *err) {
     T v = ld_agent(p);
     unsigned spins = 0;
     while (is_sent(v)) {
         __builtin_amdgcn_s_sleep(1);
         v = ld_agent(p);
-        if (++spins > (1u << 24)) return T(0);  // never expected: bounded wait
+        if (++spins > RED_SPIN_LIMIT) {
+            atomicOr(err, 4);
+            return T(0);
+        }
     }
     return v;
 }
@@ -353,7 +367,7 @@ __device__ __forceinline__ T poll_slot(T *p) {
 // s[t] = 0 + p[t] + p[t+256] + ..., then tree256.  Result valid in thread 0.
 // Re-arms every slot it reads.
 template <class T>
-__device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh) {
+__device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh, int *err) {
     constexpr int B = 8;  // slots loaded per thread at once
     T s = T(0);
     for (int64_t k0 = threadIdx.x; k0 < nb; k0 += (int64_t)CB * B) {
@@ -368,7 +382,7 @@ __device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh) {
             int64_t k = k0 + (int64_t)CB * j;
             if (k < nb) {
                 T x = v[j];
-                if (is_sent(x)) x = poll_slot(part + k);
+                if (is_sent(x)) x = poll_slot(part + k, err);
                 st_agent(part + k, sentinel<T>());
                 s = s + x;
             }
@@ -378,19 +392,13 @@ __device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh) {
 }
 
 // Collector election: every block calls this after its thread 0 stored its
-// partials; it returns true in block 0 (of each grid row), which then collects
-// the partials with final_pass().  The partials are value-as-flag slots, so
-// the collector needs no arrival count: final_pass() waits on each slot until
-// its block has stored it.  Block 0 is dispatched first and the other blocks
-// never wait on anything, so its wait always ends; the critical path after the
-// last store is one poll round trip, with no chain of atomics in front of it.
-// (`counter` and `nblocks` are unused: the sites keep their counter words.)
-constexpr unsigned LB_GROUPS = 64;
-__device__ __forceinline__ bool last_block(unsigned *counter, unsigned nblocks) {
-    (void)counter;
-    (void)nblocks;
-    return blockIdx.x == 0;
-}
+// partials; it is true in block 0 (of each grid row), which then collects the
+// partials with final_pass().  The partials are value-as-flag slots, so the
+// collector needs no arrival count: final_pass() waits on each slot until its
+// block has stored it.  Block 0 is dispatched first and the other blocks never
+// wait on anything, so its wait always ends; the critical path after the last
+// store is one poll round trip, with no chain of atomics in front of it.
+__device__ __forceinline__ bool is_collector() { return blockIdx.x == 0; }
 
 // Reduction partial buffers (sentinel-filled when grown), defined in ctx.hip.
 double *partials64(hala_ctx c, size_t count);
@@ -398,20 +406,27 @@ float *partials32(hala_ctx c, size_t count);
 
 inline unsigned *counters(hala_ctx c) { return reinterpret_cast<unsigned *>(c->scal.as<double>() + S_CNT); }
 
-// Arrival counters of reduction site `site` (LB_GROUPS + 1 zeroed words each).
-constexpr int LB_SITES = 160;
-inline unsigned *lb_counter(hala_ctx c, int site) { return c->lbc.as<unsigned>() + site * (LB_GROUPS + 1); }
+// Device error flag raised by a dependency or reduction wait that gave up
+// (bit 1: a triangular-solve wait, 2: a pool that no wave ran, 4: a reduction
+// partial).  Read and cleared by check_wait_error (trsolve.hip).
+inline int *wait_error_flag(hala_ctx c) { return reinterpret_cast<int *>(counters(c) + 120); }
+
+// Throws HALA_ERR_NOCONV if a wait on the context's stream gave up since the
+// last check, after clearing the flag and, once the stream is idle, re-arming
+// every reduction partial slot (a partial stored after its collector gave up
+// would otherwise be read by the next reduction).  Synchronises the stream.
+void check_wait_error(hala_ctx c);
+// The flag word as the host last copied it with the stream synchronised
+// (host_pinned must hold the scalar block prefix up to the flag).
+void check_wait_error_host(hala_ctx c, int flag);
+
 // Ticket counters of the triangular-solve pool: TK_MAX counters plus one
 // "waves finished" counter, each on its own 4 KiB page so that their atomics
 // land in different memory channels instead of queueing on one address.
 constexpr int TK_STRIDE = 1024;  // in unsigned words
 constexpr int TK_MAX = 4;
-constexpr int TK_BASE = (LB_SITES * (LB_GROUPS + 1) + TK_STRIDE - 1) / TK_STRIDE * TK_STRIDE;
-constexpr int LBC_WORDS = TK_BASE + (TK_MAX + 1) * TK_STRIDE;
-inline unsigned *tri_tickets(hala_ctx c) { return c->lbc.as<unsigned>() + TK_BASE; }
-enum lb_site : int { LB_DOT = 0, LB_CG_START = 1, LB_CG_SPMV = 2, LB_CG_UPDATE = 3, LB_CG_RZ = 4, LB_GM_NRM = 5,
-                     LB_GM_ORTH = 6, LB_IR_NRM = 7, LB_GEMV = 16 /* .. 16 + 32 columns */,
-                     LB_GM_COL = 64 /* .. 64 + 64 Krylov columns */ };
+constexpr int LBC_WORDS = (TK_MAX + 1) * TK_STRIDE;
+inline unsigned *tri_tickets(hala_ctx c) { return c->lbc.as<unsigned>(); }
 constexpr int GM_MAX_RESTART = 64;
 
 // XCD-aware block order.  Workgroups are dispatched round-robin over the

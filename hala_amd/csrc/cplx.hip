@@ -75,7 +75,7 @@ __device__ __forceinline__ R term(int mode, const R *x, const R *y, int64_t i) {
 // one real canonical reduction of the per-element terms (the structure of cdot_kernel)
 template <class R, int K>
 __global__ __launch_bounds__(256) void zdot_kernel(int64_t n, const R *__restrict__ x, const R *__restrict__ y,
-                                                   int64_t nb, R *part, R *out, int mode) {
+                                                   int64_t nb, R *part, R *out, int mode, int *err) {
     __shared__ R sh[CB * K];
     R v[K];
 #pragma unroll
@@ -91,8 +91,8 @@ __global__ __launch_bounds__(256) void zdot_kernel(int64_t n, const R *__restric
             if (b < nb) st_agent(part + b, v[k]);
         }
     }
-    if (last_block(nullptr, gridDim.x)) {
-        R r = final_pass<R>(part, nb, sh);
+    if (is_collector()) {
+        R r = final_pass<R>(part, nb, sh, err);
         if (threadIdx.x == 0) out[0] = mode == Z_NRM2 ? (R)sqrt(r) : r;
     }
 }
@@ -106,7 +106,7 @@ void zreduce(hala_ctx c, int64_t n, const R *x, const R *y, R *out_dev, int mode
     else part = reinterpret_cast<R *>(partials32(c, nb));
     int64_t g = (nb + K - 1) / K;
     HALA_REQUIRE(g < (1ll << 31), HALA_ERR_ARG, "vector too long");
-    zdot_kernel<R, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, part, out_dev, mode);
+    zdot_kernel<R, K><<<(unsigned)g, CB, 0, c->stream>>>(n, x, y, nb, part, out_dev, mode, wait_error_flag(c));
     HALA_CHECK_LAUNCH();
 }
 

@@ -64,6 +64,32 @@ static T *partials(hala_ctx c, dbuf &b, size_t count) {
 double *partials64(hala_ctx c, size_t count) { return partials<double>(c, c->red, count); }
 float *partials32(hala_ctx c, size_t count) { return partials<float>(c, c->red32, count); }
 
+int fetch_i32(hala_ctx c, const int *p);
+
+void check_wait_error_host(hala_ctx c, int flag) {
+    if (flag == 0) return;
+    HALA_HIP(hipMemsetAsync(wait_error_flag(c), 0, sizeof(int), c->stream));
+    if (flag & 4) {
+        // a collector gave up on a partial: wait for every block, then re-arm all slots
+        HALA_HIP(hipDeviceSynchronize());
+        if (c->red.p) {
+            int64_t n = (int64_t)(c->red.bytes / sizeof(double));
+            fill_sentinel_kernel<double><<<grid_for(n, 256 * 4, 4096), 256, 0, c->stream>>>(c->red.as<double>(), n);
+        }
+        if (c->red32.p) {
+            int64_t n = (int64_t)(c->red32.bytes / sizeof(float));
+            fill_sentinel_kernel<float><<<grid_for(n, 256 * 4, 4096), 256, 0, c->stream>>>(c->red32.as<float>(), n);
+        }
+        HALA_CHECK_LAUNCH();
+    }
+    HALA_HIP(hipStreamSynchronize(c->stream));
+    if (flag & 4)
+        throw status_error{HALA_ERR_NOCONV, "a reduction gave up waiting for a block's partial (device stalled?)"};
+    throw status_error{HALA_ERR_NOCONV, "a triangular-solve dependency wait gave up (corrupt schedule?)"};
+}
+
+void check_wait_error(hala_ctx c) { check_wait_error_host(c, fetch_i32(c, wait_error_flag(c))); }
+
 }  // namespace hala_amd
 
 hala_ctx hala_amd::aux_ctx(hala_ctx c) {
@@ -164,7 +190,7 @@ int hala_ctx_sync(hala_ctx c) {
     return guard([&] {
         HALA_REQUIRE(c, HALA_ERR_ARG, "null context");
         set_device(c);
-        HALA_HIP(hipStreamSynchronize(c->stream));
+        check_wait_error(c);  // synchronises; reports a wait of earlier asynchronous work that gave up
     });
 }
 
@@ -188,6 +214,7 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "cg_batch_max") return &c->opt.cg_batch_max;
     if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
     if (k == "ilu32_jacobi") return &c->opt.ilu32_jacobi;
+    if (k == "sync_checks") return &c->opt.sync_checks;
     return nullptr;
 }
 

@@ -143,7 +143,7 @@ __global__ void gm_copy_kernel(int n, const double *__restrict__ src, double *__
 // Start of a cycle: inner_res = norm2(r); Z = [inner_res]; the inner loop runs
 // while inner_res > tol and k < restart.
 __global__ __launch_bounds__(256) void gm_cycle_norm_kernel(int n, const double *__restrict__ r, double *part,
-                                                            unsigned *cnt, gm_state *st, gm_arrays ga) {
+                                                            int *err, gm_state *st, gm_arrays ga) {
     if (st->all_done) return;
     __shared__ double sh[CB * GK];
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
@@ -159,8 +159,8 @@ __global__ __launch_bounds__(256) void gm_cycle_norm_kernel(int n, const double 
             int64_t b = (int64_t)blockIdx.x * GK + q;
             if (b < nb) st_agent(part + b, v[q]);
         }
-    if (last_block(cnt, gridDim.x)) {
-        double rr = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double rr = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) {
             double res = sqrt(rr);
             st->total = st->total + 1;
@@ -192,7 +192,7 @@ __global__ void gm_scale_first_kernel(int n, double *__restrict__ r, double *__r
 
 // cf[j] = cdot(W_j, r), one column per blockIdx.y (columns j <= k)
 __global__ __launch_bounds__(256) void gm_mdot_kernel(int n, const double *__restrict__ W,
-                                                      const double *__restrict__ r, double *part, unsigned *cnt,
+                                                      const double *__restrict__ r, double *part, int *err,
                                                       const gm_state *st, gm_arrays ga) {
     if (st->all_done || !st->in_cycle) return;
     const int col = blockIdx.y;
@@ -213,8 +213,8 @@ __global__ __launch_bounds__(256) void gm_mdot_kernel(int n, const double *__res
             int64_t b = (int64_t)blockIdx.x * GK + q;
             if (b < nb) st_agent(pc + b, v[q]);
         }
-    if (last_block(cnt + col * (LB_GROUPS + 1), gridDim.x)) {
-        double t = final_pass<double>(pc, nb, sh);
+    if (is_collector()) {
+        double t = final_pass<double>(pc, nb, sh, err);
         if (threadIdx.x == 0) ga.cf[col] = 1.0 * t + 0.0;  // gemv('T', 1, W, r, 0, coeffs)
     }
 }
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void gm_mdot_kernel(int n, const double *__res
 // with norm2(r); the last block then performs the host-side steps of the
 // reference inner loop (rotations, new rotation, H column, residual estimate).
 __global__ __launch_bounds__(256) void gm_orth_kernel(int n, const double *__restrict__ W, double *__restrict__ r,
-                                                      double *part, unsigned *cnt, gm_state *st, gm_arrays ga) {
+                                                      double *part, int *err, gm_state *st, gm_arrays ga) {
     if (st->all_done || !st->in_cycle) return;
     __shared__ double sh[CB * GK];
     __shared__ double scf[GM_MAX_RESTART];
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(256) void gm_orth_kernel(int n, const double *__res
             int64_t b = (int64_t)blockIdx.x * GK + q;
             if (b < nb) st_agent(part + b, v[q]);
         }
-    if (last_block(cnt, gridDim.x)) {
-        double rr = final_pass<double>(part, nb, sh);
+    if (is_collector()) {
+        double rr = final_pass<double>(part, nb, sh, err);
         if (threadIdx.x == 0) {
             double nrm = sqrt(rr);
             double *cf = ga.cf;
@@ -383,7 +383,7 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
         if (!P.M) gm_copy_kernel<<<gp, 256, 0, c->stream>>>(n, t, r, st, 0);
         else if (P.vals32) ilu_apply32x64(P.M, t, r, &st->all_done);
         else ilu_apply64(P.M, t, r, &st->all_done);
-        gm_cycle_norm_kernel<<<ge, 256, 0, c->stream>>>(n, r, part, lb_counter(c, LB_GM_NRM), st, ga);
+        gm_cycle_norm_kernel<<<ge, 256, 0, c->stream>>>(n, r, part, wait_error_flag(c), st, ga);
         gm_scale_first_kernel<<<gp, 256, 0, c->stream>>>(n, r, W, st);
         HALA_CHECK_LAUNCH();
         // ---- inner steps (gated on the device once the cycle stops)
@@ -398,10 +398,10 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
             if (!P.M) gm_copy_kernel<<<gp, 256, 0, c->stream>>>(n, t, r, st, 1);
             else if (P.vals32) ilu_apply32x64(P.M, t, r, &st->skip_inner);
             else ilu_apply64(P.M, t, r, &st->skip_inner);
-            gm_mdot_kernel<<<dim3(ge, restart), 256, 0, c->stream>>>(n, W, r, part, lb_counter(c, LB_GM_COL), st,
+            gm_mdot_kernel<<<dim3(ge, restart), 256, 0, c->stream>>>(n, W, r, part, wait_error_flag(c), st,
                                                                       ga);
             gm_orth_kernel<<<ge, 256, 0, c->stream>>>(n, W, r, part + (size_t)restart * nb,
-                                                      lb_counter(c, LB_GM_ORTH), st, ga);
+                                                      wait_error_flag(c), st, ga);
             gm_scale_next_kernel<<<gp, 256, 0, c->stream>>>(n, r, W, st);
             HALA_CHECK_LAUNCH();
         }

@@ -395,6 +395,7 @@ int hala_ilu_apply_f64(hala_ilu M, const double *x, double *r) {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f64: null argument");
         set_device(M->ctx);
         ilu_apply64(M, x, r);
+        if (M->ctx->opt.sync_checks) check_wait_error(M->ctx);
     });
 }
 
@@ -407,6 +408,7 @@ int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
         M->tmp32.reserve((size_t)M->n * sizeof(float));
         tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>());
         tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r);
+        if (c->opt.sync_checks) check_wait_error(c);
     });
 }
 
@@ -415,6 +417,7 @@ int hala_ilu_apply_f32x64(hala_ilu M, const double *x, double *r) {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f32x64: null argument");
         set_device(M->ctx);
         ilu_apply32x64(M, x, r);
+        if (M->ctx->opt.sync_checks) check_wait_error(M->ctx);
     });
 }
 
@@ -423,6 +426,7 @@ int hala_ilu_apply_t_f64(hala_ilu M, const double *x, double *r) {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_t_f64: null argument");
         set_device(M->ctx);
         ilu_apply_t64(M, x, r);
+        if (M->ctx->opt.sync_checks) check_wait_error(M->ctx);
     });
 }
 

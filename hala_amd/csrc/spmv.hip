@@ -93,7 +93,21 @@ __global__ __launch_bounds__(256) void spmv_rows_kernel(int nrows, const int *__
     else y[row] = out;
 }
 
-// One wavefront per row for long rows: lanes stride the row, butterfly sum.
+// Value of v held by lane k (k wave-uniform), as a wave-uniform operand.
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float lane_value(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// One wavefront per row for long rows.  The lanes form the products of 64
+// consecutive entries at once (the next 64 are loaded while these are added);
+// the sum then adds them one by one in CSR order, as a wave-uniform chain over
+// the lanes' values: the reference's sequential row sum, bit for bit.
 template <class TV, class TX, class TY, class TA>
 __global__ __launch_bounds__(256) void spmv_wave_kernel(int nrows, const int *__restrict__ pntr,
                                                         const int *__restrict__ indx, const TV *__restrict__ vals,
@@ -103,9 +117,18 @@ __global__ __launch_bounds__(256) void spmv_wave_kernel(int nrows, const int *__
     int row = (blockIdx.x * 256 + threadIdx.x) >> 6;
     const int nw = (gridDim.x * 256) >> 6;
     for (; row < nrows; row += nw) {
+        const int s = pntr[row], e = pntr[row + 1];
         TA acc = TA(0);
-        for (int j = pntr[row] + lane; j < pntr[row + 1]; j += 64) acc = acc + (TA)vals[j] * (TA)x[indx[j]];
-        for (int off = 32; off >= 1; off >>= 1) acc = acc + __shfl_xor(acc, off, 64);
+        TA p = TA(0);
+        if (s + lane < e) p = (TA)vals[s + lane] * (TA)x[indx[s + lane]];
+        for (int base = s; base < e; base += 64) {
+            const int jn = base + 64 + lane;
+            TA pn = TA(0);
+            if (jn < e) pn = (TA)vals[jn] * (TA)x[indx[jn]];
+            const int cnt = min(64, e - base);
+            for (int k = 0; k < cnt; k++) acc = acc + lane_value(p, k);
+            p = pn;
+        }
         if (lane == 0) {
             TY s = f32_inner ? (TY)((TA)1 * acc + (TA)0) : (TY)acc;
             y[row] = (beta == TY(0)) ? alpha * s + TY(0) : alpha * s + beta * y[row];

@@ -159,7 +159,7 @@ int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, cons
         tri_plan &p = tr ? T->T : T->N;
         load_values64(c, p, vals);
         tri_exec<double, double, double>(c, p, TRI_SUM, alpha, b, x);
-        check_wait_error(c);
+        if (c->opt.sync_checks) check_wait_error(c);
     });
 }
 
@@ -183,7 +183,7 @@ int hala_trsv_f32(hala_tri T, char trans, float alpha, const float *vals, const 
             HALA_HIP(hipStreamSynchronize(c->stream));
         }
         tri_exec<float, float, float>(c, p, TRI_SUM, alpha, b, x);
-        check_wait_error(c);
+        if (c->opt.sync_checks) check_wait_error(c);
     });
 }
 

@@ -7,17 +7,6 @@
 namespace hala_amd {
 
 
-int *wait_error_flag(hala_ctx c) { return reinterpret_cast<int *>(counters(c) + 120); }
-
-void check_wait_error(hala_ctx c) {
-    int v = fetch_i32(c, wait_error_flag(c));
-    if (v != 0) {
-        HALA_HIP(hipMemsetAsync(wait_error_flag(c), 0, sizeof(int), c->stream));
-        HALA_HIP(hipStreamSynchronize(c->stream));
-        throw status_error{HALA_ERR_NOCONV, "a triangular-solve dependency wait gave up (corrupt schedule?)"};
-    }
-}
-
 // ---------------------------------------------------------------------------
 // dependency lists
 // ---------------------------------------------------------------------------

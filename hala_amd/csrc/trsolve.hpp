@@ -140,13 +140,9 @@ void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate);
 void tri_rearm(hala_ctx c, tri_plan &p);
 void tri_rearm_prec(hala_ctx c, tri_plan &p, bool is64);
 
-// Device error flag raised by a dependency wait that gave up; read and cleared by check_wait_error.
-int *wait_error_flag(hala_ctx c);
-
 constexpr int WAVE = 64;
 constexpr int SLOT_PAD = 1024;             // slot-indexed arrays run this far past their end (idle-lane reads)
 constexpr int MAXD = 16;                   // dependencies held in registers per row (longer rows loop)
 constexpr unsigned SPIN_LIMIT = 1u << 22;  // polls before a wait gives up (a few seconds)
-void check_wait_error(hala_ctx c);
 
 }  // namespace hala_amd

@@ -161,9 +161,11 @@ def test_oracle_matches_hala_fixtures(name):
     assert np.array_equal(bits(oracle.spmv_f32(p, i, v.astype(np.float32), x32)), bits(g[name + "/spmv32"]))
 
 
-@pytest.mark.skipif(oracle.ref() is None, reason="reference tree not available to build oracle/_ref")
 def test_oracle_matches_live_reference_build():
+    # resolved inside the test: collecting this module loads nothing built from the reference
     R, P = oracle.ref(), oracle.P
+    if R is None:
+        pytest.skip("reference tree not available to build oracle/_ref")
     for p, i, v in (st.laplace3d(14, 12, 10), st.convdiff27(10, seed=11), st.discretize_pde(33)[:3]):
         n = len(p) - 1
         d, _ = oracle.diag(p, i)

@@ -88,23 +88,45 @@ def test_spmv_f64_bitexact(engine, name):
     assert np.array_equal(bits(host(gy)), bits(ref))
 
 
-def test_spmv_wave_kernel(engine):
-    # long rows take the wavefront-per-row kernel; its butterfly sum differs in order only
-    n = 2000
-    rng = np.random.default_rng(5)
-    rows = [np.unique(np.concatenate([[r], rng.choice(n, 150, replace=False)])) for r in range(n)]
+def _long_row_pattern(n, per_row, seed, one_long=0):
+    rng = np.random.default_rng(seed)
+    rows = [np.unique(np.concatenate([[r], rng.choice(n, per_row, replace=False)])) for r in range(n)]
+    if one_long:  # one row longer than 8192 entries (a ragged last chunk of the 64-wide chain)
+        rows[n // 2] = np.arange(one_long, dtype=np.int64)
     p = np.zeros(n + 1, np.int32)
     p[1:] = np.cumsum([len(r) for r in rows])
     i = np.concatenate(rows).astype(np.int32)
-    v = rng.uniform(-1, 1, len(i))
+    return p, i, rng.uniform(-1, 1, len(i))
+
+
+@pytest.mark.parametrize("case", ["mean150", "row9001"])
+def test_spmv_wave_kernel(engine, case):
+    # long rows take the wavefront-per-row kernel; it adds each row's products in CSR
+    # order (sparse/hala_sparse_utils.hpp:106-108), so it matches the oracle bit for bit
+    import torch
+
+    if case == "mean150":
+        n = 2000
+        p, i, v = _long_row_pattern(n, 150, 5)
+    else:
+        n = 9001
+        p, i, v = _long_row_pattern(n, 3, 6, one_long=9001)
     gp, gi, gv = upload(engine, p, i, v)
     A = H.make_sparse_matrix(engine, n, gp, gi, gv)
     assert A.info()["kernel"] == 1
     x = st.uniform(8, n)
-    gx, gy = dev(engine, x), engine.vector(n)
-    A.gemv("N", 1.0, gx, 0.0, gy)
-    ref = oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))
-    np.testing.assert_allclose(host(gy), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    y0 = st.uniform(9, n)
+    gx, gy = dev(engine, x), dev(engine, y0)
+    A.gemv("N", 1.5, gx, 0.25, gy)
+    ref = oracle.spmv("N", n, n, 1.5, p, i, v, x, 0.25, y0)
+    assert np.array_equal(bits(host(gy)), bits(ref))
+    # FP32 values and vectors: the same order in FP32
+    v32 = v.astype(np.float32)
+    A32 = H.make_sparse_matrix(engine, n, gp, gi, dev(engine, v32))
+    gx32 = dev(engine, x.astype(np.float32))
+    gy32 = engine.vector(n, dtype=torch.float32)
+    A32.gemv("N", 1.0, gx32, 0.0, gy32)
+    assert np.array_equal(bits(host(gy32)), bits(oracle.spmv_f32(p, i, v32, x.astype(np.float32))))
 
 
 def test_spmv_f32_variants(engine):

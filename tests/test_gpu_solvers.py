@@ -304,6 +304,23 @@ def test_adjoint_gradient_bitexact(engine):
     assert np.array_equal(bits(gk), bits(go))
 
 
+def test_adjoint_zero_upstream_gradient(engine):
+    # a loss that does not depend on x: the gradient is zero (no adjoint solve of a zero rhs)
+    import torch
+    from hala_amd.autograd import SparseLinearProblem, sparse_solve
+
+    m = 8
+    d = st.diffusion3d(m, seed=5)
+    prob = SparseLinearProblem(engine, d["pntr"], d["indx"], d["cptr"], d["cnnz"], d["ccoef"],
+                               st.gaussian_source(m), rtol=1e-12)
+    kappa = dev(engine, d["kappa"]).requires_grad_(True)
+    x = sparse_solve(kappa, prob)
+    (0.0 * x.sum() + kappa.sum()).backward()
+    assert np.array_equal(host(kappa.grad), np.ones(len(d["kappa"])))
+    g = prob.gradient(kappa.detach(), torch.zeros_like(x))
+    assert not host(g).any()
+
+
 def test_adjoint_finite_differences(engine):
     import torch
     from hala_amd.autograd import SparseLinearProblem
