@@ -28,7 +28,7 @@
 #include <valarray>
 #include <vector>
 
-#include "../hala_amd.h"
+#include "hala_amd.h"
 
 namespace hala {
 
@@ -312,6 +312,51 @@ void set_zero(gpu_engine const &engine, size_t num_entries, VectorLike &&x) {
 template <class V>
 using scalar_of = std::remove_cv_t<typename std::remove_reference_t<V>::value_type>;
 
+// Scalar type of a vector and its precision (float for std::complex<float>):
+// common/hala_types_checker.hpp:74, :84.  The solvers' stop_criteria are typed
+// by the precision, as in the reference.
+namespace detail {
+template <typename T>
+struct precision_of {
+    using type = T;
+};
+template <typename R>
+struct precision_of<std::complex<R>> {
+    using type = R;
+};
+}  // namespace detail
+template <class V>
+using get_standard_type = scalar_of<V>;
+template <class V>
+using get_precision_type = typename detail::precision_of<scalar_of<V>>::type;
+
+// The vector behind an engine binding, the identity for a plain vector (the
+// specialisation for engined_vector is in hala_noengine.hpp).
+namespace detail {
+template <class V>
+struct unbind {
+    static V &get(V &v) { return v; }
+};
+template <class V>
+auto &raw(V &v) {
+    return unbind<V>::get(v);
+}
+}  // namespace detail
+
+// Vector sizes (common/hala_vector_defines.hpp:358, :420).
+template <class V>
+size_t get_size(V const &x) {
+    return (size_t)x.size();
+}
+template <class V>
+int get_size_int(V const &x) {
+    return static_cast<int>(get_size(x));
+}
+template <class V>
+void set_size(size_t new_size, V &x) {
+    x.resize(new_size);
+}
+
 // ---------------------------------------------------------------------------
 // BLAS-1 on the engine (gpu/hala_gpu_blas1.hpp, gpu/hala_gpu_overloads.hpp)
 // ---------------------------------------------------------------------------
@@ -566,7 +611,7 @@ auto make_sparse_matrix(gpu_engine const &engine, int num_cols, VP const &pntr, 
 template <typename MT, typename FSA, class VB, typename FSB, class VC>
 void sparse_gemm(MT const &matrix, char transa, char transb, int b_rows, int b_cols, FSA alpha, VB const &B, int ldb,
                  FSB beta, VC &&C, int ldc) {
-    matrix.gemm(transa, transb, b_rows, b_cols, alpha, B, ldb, beta, C, ldc);
+    matrix.gemm(transa, transb, b_rows, b_cols, alpha, detail::raw(B), ldb, beta, detail::raw(C), ldc);
 }
 template <typename FPa, class VP, class VI, class VV, class VX, typename FPb, class VY>
 void sparse_gemv(gpu_engine const &engine, char trans, int M, int N, FPa alpha, VP const &pntr, VI const &indx,
@@ -575,11 +620,15 @@ void sparse_gemv(gpu_engine const &engine, char trans, int M, int N, FPa alpha, 
 }
 template <class M, typename FPa, class VX, typename FPb, class VY, class VB>
 void sparse_gemv(M const &matrix, char trans, FPa alpha, VX const &x, FPb beta, VY &&y, VB &&buf) {
-    matrix.gemv(trans, alpha, x, beta, y, buf);
+    matrix.gemv(trans, alpha, detail::raw(x), beta, detail::raw(y), buf);
+}
+template <class M, typename FPa, class VX, typename FPb, class VY>
+void sparse_gemv(M const &matrix, char trans, FPa alpha, VX const &x, FPb beta, VY &&y) {
+    matrix.gemv(trans, alpha, detail::raw(x), beta, detail::raw(y));
 }
 template <class M, typename FPa, class VX, typename FPb, class VY>
 size_t sparse_gemv_buffer_size(M const &matrix, char trans, FPa alpha, VX const &x, FPb beta, VY &&y) {
-    return matrix.gemv_buffer_size(trans, alpha, x, beta, y);
+    return matrix.gemv_buffer_size(trans, alpha, detail::raw(x), beta, detail::raw(y));
 }
 
 // ---------------------------------------------------------------------------
@@ -663,12 +712,12 @@ auto make_triangular_matrix(gpu_engine const &engine, char uplo, char diag, VP c
 }
 template <class TM, typename FP, class VB, class VX>
 void sparse_trsv(char trans, TM const &tri, FP alpha, VB const &b, VX &&x) {
-    tri.trsv(trans, alpha, b, x);
+    tri.trsv(trans, alpha, detail::raw(b), detail::raw(x));
 }
 // sparse/hala_sparse_blas.hpp:203-205
 template <class TM, typename FP, class VB>
 void sparse_trsm(char transa, char transb, int nrhs, TM const &tri, FP alpha, VB &&B, int ldb = -1) {
-    tri.trsm(transa, transb, nrhs, alpha, B, ldb);
+    tri.trsm(transa, transb, nrhs, alpha, detail::raw(B), ldb);
 }
 
 // ---------------------------------------------------------------------------
@@ -755,21 +804,22 @@ auto make_ilu(gpu_engine const &engine, VP const &pntr, VI const &indx, VV const
 }
 template <class ILU, class VX, class VR, class VT>
 void ilu_apply(ILU const &ilu, VX const &x, VR &&r, int num_rhs, VT &&tmp) {
-    ilu.apply(x, r, num_rhs, tmp);
+    ilu.apply(detail::raw(x), detail::raw(r), num_rhs, tmp);
 }
 template <class ILU, class VX, class VR>
 void ilu_apply(ILU const &ilu, VX const &x, VR &&r, int num_rhs = 1) {
-    ilu.apply(x, r, num_rhs);
+    ilu.apply(detail::raw(x), detail::raw(r), num_rhs);
 }
 template <class ILU, class VX, class VR>
 auto ilu_temp_buffer(ILU const &ilu, VX const &x, VR &&r, int num_rhs) {
-    return ilu.get_temp_buffer(x, r, num_rhs);
+    return ilu.get_temp_buffer(detail::raw(x), detail::raw(r), num_rhs);
 }
 template <class ILU, class VX, class VR>
 size_t ilu_buffer_size(ILU const &ilu, VX const &x, VR &&r, int num_rhs) {
-    return ilu.buffer_size(x, r, num_rhs);
+    return ilu.buffer_size(detail::raw(x), detail::raw(r), num_rhs);
 }
 
 }  // namespace hala
 
 #include "hala_mixed.hpp"
+#include "hala_noengine.hpp"

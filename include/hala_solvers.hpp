@@ -27,9 +27,48 @@ struct stop_criteria {
     stop_criteria(U const &tolerance, int max_iterations) : max_iter(max_iterations), tol((precision)tolerance) {}
 };
 
-// hex/solvers/hala_solvers_core.hpp:80-143 (gpu_engine form)
+// hex/solvers/hala_solvers_core.hpp:80-143: a preconditioner r = P^-1 x over the
+// engine's default vectors; the gpu_engine and the mixed_engine both use device
+// vectors (the mixed solvers load the data once and work on the GPU).
+template <class engine, class VectorLike>
+struct define_preconditioner;
+template <class VectorLike>
+struct define_preconditioner<gpu_engine, VectorLike> {
+    using type = std::function<void(gpu_vector<scalar_of<VectorLike>> const &x, gpu_vector<scalar_of<VectorLike>> &r)>;
+};
+template <class VectorLike>
+struct define_preconditioner<mixed_engine, VectorLike> {
+    using type = typename define_preconditioner<gpu_engine, VectorLike>::type;
+};
+template <class engine, class VectorLike>
+using preconditioner = typename define_preconditioner<engine, VectorLike>::type;
+
+// No-engine form (hex/solvers/hala_solvers_core.hpp:120-143): the vectors are
+// the bound owned vectors new_vector() makes for VectorLike.
+template <class VectorLike>
+using preconditioner_noe = std::function<void(decltype(new_vector(std::declval<VectorLike>())) const &x,
+                                              decltype(new_vector(std::declval<VectorLike>())) &r)>;
+
+// hex/solvers/hala_solvers_core.hpp:159-165 (ROCm: plain int)
+template <class compute_engine>
+using hala_int = int;
+
+namespace detail {
+// temporaries of the host-driven loops: the engine's device vector, or a
+// bound owned vector for the no-engine form (its preconditioner receives those)
+template <class TV>
+struct temp_of;
 template <typename T>
-using preconditioner = std::function<void(gpu_vector<T> const &x, gpu_vector<T> &r)>;
+struct temp_of<gpu_vector<T>> {
+    static gpu_vector<T> make(gpu_engine const &e) { return gpu_vector<T>(e.device()); }
+};
+template <typename T>
+struct temp_of<engined_vector<gpu_engine, gpu_vector<T>>> {
+    static engined_vector<gpu_engine, gpu_vector<T>> make(gpu_engine const &e) {
+        return engined_vector<gpu_engine, gpu_vector<T>>(e);
+    }
+};
+}  // namespace detail
 
 template <class VX>
 void prepare_x(gpu_engine const &engine, int n, VX &x) {
@@ -42,15 +81,19 @@ void prepare_x(gpu_engine const &engine, int n, VX &x) {
 // ---- CG ------------------------------------------------------------------
 // solve_cg(engine, stop, pntr, indx, vals, precon, b, x):
 // hex/solvers/hala_solvers_cg.hpp:181-244 with solve_cg_core :92-156.
-template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-             VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+// The host loop runs over temporaries of type TV (see temp_of): device vectors
+// for the engine form, bound vectors for the no-engine form.
+namespace detail {
+template <class TV, class VP, class VI, class VV, class P, class VB, class VX>
+int cg_loop(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
+            VI const &indx, VV const &vals, P &precon, VB const &b, VX &x) {
     using T = scalar_of<VV>;
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_cg: size mismatch");
     prepare_x(engine, n, x);
     auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
-    gpu_vector<T> r(engine.device()), p(engine.device()), ap(engine.device()), z(engine.device());
+    TV r = temp_of<TV>::make(engine), p = temp_of<TV>::make(engine), ap = temp_of<TV>::make(engine),
+       z = temp_of<TV>::make(engine);
     vcopy(engine, b, r);
     matrix.gemv('N', 1.0, x, 0.0, p);
     axpy(engine, -1.0, p, r);
@@ -75,6 +118,13 @@ int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP con
     }
     return iterations;
 }
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_cg(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
+             VI const &indx, VV const &vals, preconditioner<gpu_engine, VX> precon, VB const &b, VX &x) {
+    return detail::cg_loop<gpu_vector<scalar_of<VV>>>(engine, stop, pntr, indx, vals, precon, b, x);
+}
 
 // solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x): hex/solvers/hala_solvers_cg.hpp:312-326.
 // Runs device-resident (hala_cg_f64); identical arithmetic to solve_cg with
@@ -82,7 +132,7 @@ int solve_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP con
 // Other scalar types run solve_cg's host loop with the ILU apply as preconditioner.
 namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
            VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_cg_ilu: size mismatch");
@@ -95,7 +145,7 @@ int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const
     return it;
 }
 template <class VP, class VI, class VV, class T, class VB, class VX>
-int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
            VV const &vals, gpu_ilu<T> const &ilu, VB const &b, VX &x, std::false_type) {
     return solve_cg(engine, stop, pntr, indx, vals,
                     [&](gpu_vector<T> const &in, gpu_vector<T> &out) { ilu.apply(in, out); }, b, x);
@@ -103,14 +153,14 @@ int cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const
 }  // namespace detail
 
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x, std::is_same<scalar_of<VV>, double>());
 }
 
 // solve_cg_ilu(engine, stop, pntr, indx, vals, b, x, policy): factor on the fly (:354-361).
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, VB const &b, VX &x, char policy = 'N') {
     auto ilu = make_ilu(engine, pntr, indx, vals, policy);
     return solve_cg_ilu(engine, stop, pntr, indx, vals, ilu, b, x);
@@ -129,17 +179,19 @@ gpu_wrapped_array<scalar_of<V>> batch_column(gpu_engine const &engine, V &v, int
 // vdivide, batch_axpy, batch_scal and batch_max_norm2 of
 // wax/hala_blas_extensions.hpp).  The preconditioner receives the whole batch.
 // Host loop over the engine's per-column BLAS, the reference's step order.
-template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                   VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &B, VX &X) {
+namespace detail {
+template <class TV, class VP, class VI, class VV, class P, class VB, class VX>
+int batch_cg_loop(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
+                  VI const &indx, VV const &vals, P &precon, VB const &B, VX &X) {
     using T = scalar_of<VV>;
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg: size mismatch");
     const int nrhs = (int)(B.size() / (size_t)n);
     prepare_x(engine, (int)B.size(), X);
     auto matrix = make_sparse_matrix(engine, n, pntr, indx, vals);
-    gpu_vector<T> r(B.size(), engine.device()), p(B.size(), engine.device()), ap(B.size(), engine.device()),
-        z(B.size(), engine.device());
+    TV r = temp_of<TV>::make(engine), p = temp_of<TV>::make(engine), ap = temp_of<TV>::make(engine),
+       z = temp_of<TV>::make(engine);
+    for (TV *v : {&r, &p, &ap, &z}) v->resize(B.size());
     auto col = [&](auto &v, int k) { return batch_column(engine, v, n, k); };
     vcopy(engine, B, r);
     for (int k = 0; k < nrhs; k++) matrix.gemv('N', 1.0, col(X, k), 0.0, col(p, k));
@@ -171,6 +223,13 @@ int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, 
     }
     return iterations;
 }
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_batch_cg(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
+                   VI const &indx, VV const &vals, preconditioner<gpu_engine, VX> precon, VB const &B, VX &X) {
+    return detail::batch_cg_loop<gpu_vector<scalar_of<VV>>>(engine, stop, pntr, indx, vals, precon, B, X);
+}
 
 // solve_batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X):
 // hex/solvers/hala_solvers_cg_batch.hpp:156-200.  Runs device-resident
@@ -178,7 +237,7 @@ int solve_batch_cg(gpu_engine const &engine, stop_criteria<double> const &stop, 
 // applied to every column.
 namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int batch_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, gpu_ilu<double> const &ilu, VB const &B, VX &X, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && B.size() % (size_t)n == 0, "solve_batch_cg_ilu: size mismatch");
@@ -192,7 +251,7 @@ int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP
     return it;
 }
 template <class VP, class VI, class VV, class T, class VB, class VX>
-int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int batch_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, gpu_ilu<T> const &ilu, VB const &B, VX &X, std::false_type) {
     const int nrhs = (int)(B.size() / (pntr.size() - 1));
     return solve_batch_cg(engine, stop, pntr, indx, vals,
@@ -201,14 +260,14 @@ int batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP
 }  // namespace detail
 
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                        VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &B, VX &X) {
     return detail::batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X, std::is_same<scalar_of<VV>, double>());
 }
 
 // solve_batch_cg_ilu(engine, stop, pntr, indx, vals, B, X, policy): factor on the fly (:238-249).
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                        VV const &vals, VB const &B, VX &X, char policy = 'N') {
     auto ilu = make_ilu(engine, pntr, indx, vals, policy);
     return solve_batch_cg_ilu(engine, stop, pntr, indx, vals, ilu, B, X);
@@ -216,7 +275,7 @@ int solve_batch_cg_ilu(gpu_engine const &engine, stop_criteria<double> const &st
 
 // Device-resident CG with the identity (z = r) or Jacobi (z = r ./ d) preconditioner.
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_device(gpu_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_cg_device(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                     VV const &vals, double const *jacobi_diag, VB const &b, VX &x) {
     const int n = (int)pntr.size() - 1;
     prepare_x(engine, n, x);
@@ -302,17 +361,17 @@ void tpsv_un(int n, std::vector<T> const &ap, std::vector<T> &x) {
 }
 // Krylov projection: coeffs = W_k^H r, r -= W_k coeffs (classical Gram-Schmidt,
 // hex/solvers/hala_solvers_gmres.hpp:47-77); FP64 as two dense gemv calls
-inline std::vector<double> project(gpu_engine const &engine, int n, int k, gpu_vector<double> &W,
-                                   gpu_vector<double> &r, gpu_vector<double> &cd) {
+template <class VR>
+std::vector<double> project(gpu_engine const &engine, int n, int k, gpu_vector<double> &W, VR &r,
+                            gpu_vector<double> &cd) {
     auto Wk = wrap_gpu_array(W.data(), (size_t)n * k, engine.device());
     cd.resize(k);
     gemv(engine, 'T', n, k, 1.0, Wk, r, 0.0, cd);
     gemv(engine, 'N', n, k, -1.0, Wk, cd, 1.0, r);
     return cd.unload();
 }
-template <typename T>
-std::vector<T> project(gpu_engine const &engine, int n, int k, gpu_vector<T> &W, gpu_vector<T> &r,
-                       gpu_vector<T> &) {
+template <typename T, class VR>
+std::vector<T> project(gpu_engine const &engine, int n, int k, gpu_vector<T> &W, VR &r, gpu_vector<T> &) {
     std::vector<T> c(k);
     for (int i = 0; i < k; i++) c[i] = dot(engine, wrap_gpu_array(W.data() + (size_t)i * n, (size_t)n, engine.device()), r);
     for (int i = 0; i < k; i++)
@@ -336,9 +395,10 @@ void combine(gpu_engine const &engine, int n, gpu_vector<T> &W, std::vector<T> c
 
 // solve_gmres(engine, stop, restart, pntr, indx, vals, precon, b, x):
 // hex/solvers/hala_solvers_gmres.hpp:125-230 (host Givens, device vectors).
-template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                VI const &indx, VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+namespace detail {
+template <class TV, class VP, class VI, class VV, class P, class VB, class VX>
+int gmres_loop(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart,
+               VP const &pntr, VI const &indx, VV const &vals, P &precon, VB const &b, VX &x) {
     using T = scalar_of<VV>;
     using R = decltype(std::abs(T(0)));
     const int n = (int)pntr.size() - 1;
@@ -349,7 +409,10 @@ int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int
     std::vector<R> C;
     R inner_res = 0, outer_res = R(stop.tol + 1.0);
     int total = 0, outer = 0;
-    gpu_vector<T> t(n, engine.device()), r(n, engine.device()), W((size_t)n * restart, engine.device());
+    TV t = temp_of<TV>::make(engine), r = temp_of<TV>::make(engine);
+    t.resize(n);
+    r.resize(n);
+    gpu_vector<T> W((size_t)n * restart, engine.device());
     gpu_vector<T> cd(engine.device()), zd(engine.device());
     while (outer_res > stop.tol && outer < stop.max_iter) {
         H.clear();
@@ -396,12 +459,20 @@ int solve_gmres(gpu_engine const &engine, stop_criteria<double> const &stop, int
     }
     return total;
 }
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX>
+int solve_gmres(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart,
+                VP const &pntr, VI const &indx, VV const &vals, preconditioner<gpu_engine, VX> precon, VB const &b,
+                VX &x) {
+    return detail::gmres_loop<gpu_vector<scalar_of<VV>>>(engine, stop, restart, pntr, indx, vals, precon, b, x);
+}
 
 // solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x):
 // hex/solvers/hala_solvers_gmres.hpp:250-267, device-resident (hala_gmres_f64).
 namespace detail {
 template <class VP, class VI, class VV, class VB, class VX>
-int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int gmres_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
               VI const &indx, VV const &vals, gpu_ilu<double> const &ilu, VB const &b, VX &x, std::true_type) {
     const int n = (int)pntr.size() - 1;
     runtime_assert(n > 0 && (int)b.size() == n, "solve_gmres_ilu: size mismatch");
@@ -415,7 +486,7 @@ int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int r
     return total;
 }
 template <class VP, class VI, class VV, class T, class VB, class VX>
-int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int gmres_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
               VI const &indx, VV const &vals, gpu_ilu<T> const &ilu, VB const &b, VX &x, std::false_type) {
     return solve_gmres(engine, stop, restart, pntr, indx, vals,
                        [&](gpu_vector<T> const &in, gpu_vector<T> &out) { ilu.apply(in, out); }, b, x);
@@ -423,14 +494,14 @@ int gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int r
 }  // namespace detail
 
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
                     VI const &indx, VV const &vals, gpu_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x,
                              std::is_same<scalar_of<VV>, double>());
 }
 
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int solve_gmres_ilu(gpu_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
                     VI const &indx, VV const &vals, VB const &b, VX &x, char policy = 'N') {
     auto ilu = make_ilu(engine, pntr, indx, vals, policy);
     return solve_gmres_ilu(engine, stop, restart, pntr, indx, vals, ilu, b, x);
@@ -472,8 +543,8 @@ int mixed_solve(mixed_engine const &engine, VP const &pntr, VI const &indx, VV c
 }  // namespace detail
 
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-             VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+int solve_cg(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
+             VV const &vals, preconditioner<mixed_engine, VX> precon, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);
@@ -481,7 +552,7 @@ int solve_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP c
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_cg_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
@@ -490,7 +561,7 @@ int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, 
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
+int solve_cg_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
                  VV const &vals, VB const &b, VX &x, char policy = 'N') {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
@@ -499,8 +570,8 @@ int solve_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, 
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr, VI const &indx,
-                   VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &B, VX &X) {
+int solve_batch_cg(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
+                   VV const &vals, preconditioner<mixed_engine, VX> precon, VB const &B, VX &X) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(B);
         auto gx = gpu_bind_vector(engine.gpu(), X);
@@ -508,7 +579,7 @@ int solve_batch_cg(mixed_engine const &engine, stop_criteria<double> const &stop
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr,
+int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
                        VI const &indx, VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &B, VX &X) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(B);
@@ -517,7 +588,7 @@ int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, VP const &pntr,
+int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr,
                        VI const &indx, VV const &vals, VB const &B, VX &X, char policy = 'N') {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(B);
@@ -526,8 +597,8 @@ int solve_batch_cg_ilu(mixed_engine const &engine, stop_criteria<double> const &
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
-                VI const &indx, VV const &vals, preconditioner<scalar_of<VV>> precon, VB const &b, VX &x) {
+int solve_gmres(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
+                VI const &indx, VV const &vals, preconditioner<mixed_engine, VX> precon, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
         auto gx = gpu_bind_vector(engine.gpu(), x);
@@ -535,7 +606,7 @@ int solve_gmres(mixed_engine const &engine, stop_criteria<double> const &stop, i
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
                     VI const &indx, VV const &vals, mixed_ilu<scalar_of<VV>> const &ilu, VB const &b, VX &x) {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
@@ -544,7 +615,7 @@ int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &sto
     });
 }
 template <class VP, class VI, class VV, class VB, class VX>
-int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<double> const &stop, int restart, VP const &pntr,
+int solve_gmres_ilu(mixed_engine const &engine, stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr,
                     VI const &indx, VV const &vals, VB const &b, VX &x, char policy = 'N') {
     return detail::mixed_solve(engine, pntr, indx, vals, [&](auto &gp, auto &gi, auto &gv) {
         auto gb = engine.gpu().load(b);
@@ -560,6 +631,83 @@ int solve_gmres_ir(mixed_engine const &engine, double rtol, int restart, VP cons
         auto gx = gpu_bind_vector(engine.gpu(), x);
         return solve_gmres_ir(engine.gpu(), rtol, restart, gp, gi, gv, gb, gx, inner_rtol, max_refine);
     });
+}
+
+// ---- no-engine forms: the engine comes from the bound vectors ------------------
+// hex/solvers/hala_solvers_cg.hpp:181-227 (solve_cg), :295-305 and :345-352
+// (solve_cg_ilu), hala_solvers_cg_batch.hpp and hala_solvers_gmres.hpp (their
+// no-engine overloads).  Vectors are bound with bind_engine_vector(); the
+// user's preconditioner receives bound vectors (preconditioner_noe).  The
+// ILU forms run the engine's device-resident solvers.
+namespace detail {
+template <class VV>
+using engine_of = std::decay_t<decltype(std::declval<VV const &>().engine())>;
+template <class VV>
+constexpr bool on_gpu_engine() {
+    return std::is_same<engine_of<VV>, gpu_engine>::value;
+}
+}  // namespace detail
+
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_cg(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx, VV const &vals,
+             preconditioner_noe<VV> precon, VB const &b, VX &x) {
+    static_assert(detail::on_gpu_engine<VV>(), "no-engine solve_cg with a preconditioner: bind to the gpu_engine");
+    using TV = engined_vector<gpu_engine, gpu_vector<scalar_of<VV>>>;
+    return detail::cg_loop<TV>(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals), precon,
+                               detail::raw(b), detail::raw(x));
+}
+template <class VP, class VI, class VV, class ILU, class VB, class VX, class = enable_if_engined<VV>>
+int solve_cg_ilu(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx, VV const &vals,
+                 ILU const &ilu, VB const &b, VX &x) {
+    return solve_cg_ilu(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals), ilu,
+                        detail::raw(b), detail::raw(x));
+}
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_cg_ilu(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx, VV const &vals,
+                 VB const &b, VX &x, char policy = 'N') {
+    return solve_cg_ilu(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals),
+                        detail::raw(b), detail::raw(x), policy);
+}
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_batch_cg(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
+                   VV const &vals, preconditioner_noe<VV> precon, VB const &B, VX &X) {
+    static_assert(detail::on_gpu_engine<VV>(),
+                  "no-engine solve_batch_cg with a preconditioner: bind to the gpu_engine");
+    using TV = engined_vector<gpu_engine, gpu_vector<scalar_of<VV>>>;
+    return detail::batch_cg_loop<TV>(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals),
+                                     precon, detail::raw(B), detail::raw(X));
+}
+template <class VP, class VI, class VV, class ILU, class VB, class VX, class = enable_if_engined<VV>>
+int solve_batch_cg_ilu(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
+                       VV const &vals, ILU const &ilu, VB const &B, VX &X) {
+    return solve_batch_cg_ilu(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals), ilu,
+                              detail::raw(B), detail::raw(X));
+}
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_batch_cg_ilu(stop_criteria<get_precision_type<VV>> const &stop, VP const &pntr, VI const &indx,
+                       VV const &vals, VB const &B, VX &X, char policy = 'N') {
+    return solve_batch_cg_ilu(vals.engine(), stop, detail::raw(pntr), detail::raw(indx), detail::raw(vals),
+                              detail::raw(B), detail::raw(X), policy);
+}
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_gmres(stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr, VI const &indx,
+                VV const &vals, preconditioner_noe<VV> precon, VB const &b, VX &x) {
+    static_assert(detail::on_gpu_engine<VV>(), "no-engine solve_gmres with a preconditioner: bind to the gpu_engine");
+    using TV = engined_vector<gpu_engine, gpu_vector<scalar_of<VV>>>;
+    return detail::gmres_loop<TV>(vals.engine(), stop, restart, detail::raw(pntr), detail::raw(indx),
+                                  detail::raw(vals), precon, detail::raw(b), detail::raw(x));
+}
+template <class VP, class VI, class VV, class ILU, class VB, class VX, class = enable_if_engined<VV>>
+int solve_gmres_ilu(stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr, VI const &indx,
+                    VV const &vals, ILU const &ilu, VB const &b, VX &x) {
+    return solve_gmres_ilu(vals.engine(), stop, restart, detail::raw(pntr), detail::raw(indx), detail::raw(vals), ilu,
+                           detail::raw(b), detail::raw(x));
+}
+template <class VP, class VI, class VV, class VB, class VX, class = enable_if_engined<VV>>
+int solve_gmres_ilu(stop_criteria<get_precision_type<VV>> const &stop, int restart, VP const &pntr, VI const &indx,
+                    VV const &vals, VB const &b, VX &x, char policy = 'N') {
+    return solve_gmres_ilu(vals.engine(), stop, restart, detail::raw(pntr), detail::raw(indx), detail::raw(vals),
+                           detail::raw(b), detail::raw(x), policy);
 }
 
 }  // namespace hala

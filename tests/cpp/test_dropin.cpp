@@ -1,5 +1,5 @@
 // test_dropin.cpp -- user code written against HALA's API, compiled with g++
-// against include/hala/ and libhala_amd.so (no HIP headers).  The cases are
+// against include/ (hala.hpp, hala_solvers.hpp) and libhala_amd.so (no HIP headers).  The cases are
 // the reference's own sparse/solver tests (tests/sparse_tests.hpp,
 // tests/solvers_tests.hpp, tests/cuda_sparse_tests.hpp) and the GPU example's
 // PCG table (examples/example_gpu.cpp:90-98, :363-365).
@@ -11,8 +11,8 @@
 #include <valarray>
 #include <vector>
 
-#include "hala/hala.hpp"
-#include "hala/hala_solvers.hpp"
+#include "hala.hpp"
+#include "hala_solvers.hpp"
 
 static int failures = 0;
 #define CHECK(cond)                                                              \
@@ -78,6 +78,56 @@ template <typename R>
 struct mkv<std::complex<R>> {
     static std::complex<R> get(double re, double im) { return {R(re), R(im)}; }
 };
+// A user's PCG written once in the no-engine style (no engine argument in any
+// call; the engine travels with the bound vectors), as the reference documents
+// it (examples/example_gpu.cpp:208-285): matrix and ILU from the bound arrays,
+// work vectors from new_vector(), the engine-less BLAS and sparse calls.
+template <class VP, class VI, class VV, class VB, class VX>
+static int user_pcg(int max_iter, hala::get_precision_type<VV> tol, VP const &pntr, VI const &indx, VV const &vals,
+                    VB const &b, VX &x) {
+    const int rows = hala::get_size_int(pntr) - 1;
+    auto A = hala::make_sparse_matrix(rows, pntr, indx, vals);
+    auto M = hala::make_ilu(pntr, indx, vals, 'N');
+    auto work = hala::new_vector(x);
+    hala::set_size(std::max(hala::ilu_buffer_size(M, b, x, 1), hala::sparse_gemv_buffer_size(A, 'N', 1.0, b, 0.0, x)),
+                   work);
+    auto res = hala::new_vector(x);
+    auto dir = hala::new_vector(x);
+    auto adir = hala::new_vector(x);
+    auto pres = hala::new_vector(x);
+    hala::vcopy(b, res);
+    hala::sparse_gemv(A, 'N', -1.0, x, 1.0, res, work);
+    hala::ilu_apply(M, res, pres);
+    hala::vcopy(pres, dir);
+    auto rz = hala::dot(res, pres);
+    int it = 1;
+    for (;;) {
+        hala::sparse_gemv(A, 'N', 1.0, dir, 0.0, adir, work);
+        it++;
+        auto step = rz / hala::dot(adir, dir);
+        hala::axpy(step, dir, x);
+        hala::axpy(-step, adir, res);
+        if (it >= max_iter || hala::norm2(res) < tol) break;
+        hala::ilu_apply(M, res, pres, 1, work);
+        auto rz_new = hala::dot(res, pres);
+        hala::scal(rz_new / rz, dir);
+        hala::axpy(1.0, pres, dir);
+        rz = rz_new;
+    }
+    return it;
+}
+// ... and its engine overload: bind every array to the engine, run the no-engine body
+template <class E, class VP, class VI, class VV, class VB, class VX>
+static int user_pcg(E const &engine, int max_iter, hala::get_precision_type<VV> tol, VP const &pntr, VI const &indx,
+                    VV const &vals, VB const &b, VX &x) {
+    auto bp = hala::bind_engine_vector(engine, pntr);
+    auto bi = hala::bind_engine_vector(engine, indx);
+    auto bv = hala::bind_engine_vector(engine, vals);
+    auto bb = hala::bind_engine_vector(engine, b);
+    auto bx = hala::bind_engine_vector(engine, x);
+    return user_pcg(max_iter, tol, bp, bi, bv, bb, bx);
+}
+
 template <typename T>
 static void typed_solvers(hala::gpu_engine const &engine, const char *name, double tol) {
     const bool single = std::is_same<T, float>::value || std::is_same<T, std::complex<float>>::value;
@@ -110,7 +160,9 @@ static void typed_solvers(hala::gpu_engine const &engine, const char *name, doub
         gx.unload(x);
         CHECK(errnorm(x, x5) < tol * 100);
         gx = hala::gpu_vector<T>();
-        hala::solve_gmres_ilu(engine, exact ? hala::stop_criteria<double>(2) : hala::stop_criteria<double>(stop), exact ? 2 : 6,
+        // the stop criteria are typed by the precision (float for float and complex<float>)
+        using P = hala::get_precision_type<std::vector<T>>;
+        hala::solve_gmres_ilu(engine, exact ? hala::stop_criteria<P>(2) : hala::stop_criteria<P>(stop), exact ? 2 : 6,
                               gp, gi, gv, ilu, gb, gx);
         gx.unload(x);
         CHECK(errnorm(x, x5) < tol * 100);
@@ -518,6 +570,118 @@ int main() {
             CHECK(hala::norm2(engine, gu2) == 0.0);  // fused device path == host-driven loop, bit for bit
         }
         std::printf("PASS example_gpu table\n");
+    }
+    // the same table through a user PCG written in the no-engine style, bound to the
+    // gpu_engine: equal iteration counts, and bit for bit the device-resident solve
+    {
+        int sides[6] = {4, 8, 16, 32, 64, 128};
+        int iters[6] = {9, 14, 23, 41, 78, 152};
+        for (int t = 0; t < 6; t++) {
+            int n = 0;
+            int64_t nnz = 0;
+            hala::check_hala(hala_gen_discretize_pde(sides[t], &n, &nnz, nullptr, nullptr, nullptr, nullptr), "gen");
+            std::valarray<int> pntr(n + 1), indx((size_t)nnz);
+            std::valarray<double> vals((size_t)nnz), f(n), u(0.0, n);
+            hala::check_hala(hala_gen_discretize_pde(sides[t], &n, &nnz, std::begin(pntr), std::begin(indx),
+                                                     std::begin(vals), std::begin(f)),
+                             "gen");
+            auto gp = engine.load(pntr);
+            auto gi = engine.load(indx);
+            auto gv = engine.load(vals);
+            auto gf = engine.load(f);
+            auto gu = engine.load(u);
+            int it = user_pcg(engine, 100000, 1.E-8, gp, gi, gv, gf, gu);
+            CHECK(it == iters[t]);
+            auto gu2 = engine.load(u);
+            CHECK(hala::solve_cg_ilu(engine, 1.E-8, gp, gi, gv, gf, gu2) == it);
+            hala::axpy(engine, -1.0, gu, gu2);
+            CHECK(hala::norm2(engine, gu2) == 0.0);
+        }
+        std::printf("PASS no-engine user PCG (example_gpu table)\n");
+    }
+    // the no-engine solver forms on bound vectors (hex/solvers/hala_solvers_cg.hpp:181-227,
+    // :295-352), and float callers passing stop_criteria<float>
+    {
+        std::vector<int> pntr = {0, 3, 6, 9, 12, 15};
+        std::vector<int> indx = {0, 1, 4, 0, 1, 2, 1, 2, 3, 2, 3, 4, 0, 3, 4};
+        std::vector<double> vals = {4.0, 1.0, 1.0, 1.0, 4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 4.0, 1.0, 1.0, 1.0, 4.0};
+        std::vector<double> xref = {1.0, -2.0, 3.0, -4.0, 5.0};
+        auto b = densemul(pntr, indx, vals, xref);
+        auto gp = hala::make_gpu_vector(pntr);
+        auto gi = hala::make_gpu_vector(indx);
+        auto gv = hala::make_gpu_vector(vals);
+        auto gb = hala::make_gpu_vector(b);
+        hala::gpu_vector<double> gx1, gx2, gx3, gx4, gx5;
+        auto bp = hala::bind_engine_vector(engine, gp);
+        auto bi = hala::bind_engine_vector(engine, gi);
+        auto bv = hala::bind_engine_vector(engine, gv);
+        auto bb = hala::bind_engine_vector(engine, gb);
+        CHECK(hala::check_engines(bp, bi, bv, bb));
+        auto ilu = hala::make_ilu(bp, bi, bv);
+        std::vector<double> x;
+        {
+            auto bx = hala::bind_engine_vector(engine, gx1);
+            int it = hala::solve_cg_ilu({1.E-12, 50}, bp, bi, bv, ilu, bb, bx);
+            CHECK(it > 1 && it < 50);
+        }
+        gx1.unload(x);
+        CHECK(diffsum(x, xref) < 1e-10);
+        {  // preconditioner_noe: the lambda receives bound vectors
+            auto bx = hala::bind_engine_vector(engine, gx2);
+            hala::solve_cg(hala::stop_criteria<double>(1.E-12, 50), bp, bi, bv,
+                           [&](auto const &in, auto &out) { hala::ilu_apply(ilu, in, out); }, bb, bx);
+        }
+        hala::axpy(engine, -1.0, gx1, gx2);
+        CHECK(hala::norm2(engine, gx2) == 0.0);  // host loop == device-resident loop, bit for bit
+        {
+            auto bx = hala::bind_engine_vector(engine, gx3);
+            hala::solve_cg_ilu(1.E-12, bp, bi, bv, bb, bx);  // factor on the fly
+            auto bx4 = hala::bind_engine_vector(engine, gx4);
+            hala::solve_gmres_ilu(1.E-12, 5, bp, bi, bv, ilu, bb, bx4);
+            auto bx5 = hala::bind_engine_vector(engine, gx5);
+            hala::solve_gmres(1.E-12, 5, bp, bi, bv, [&](auto const &in, auto &out) { hala::ilu_apply(ilu, in, out); },
+                              bb, bx5);
+        }
+        gx3.unload(x);
+        CHECK(diffsum(x, xref) < 1e-10);
+        gx4.unload(x);
+        CHECK(diffsum(x, xref) < 1e-10);
+        hala::axpy(engine, -1.0, gx4, gx5);
+        CHECK(hala::norm2(engine, gx5) == 0.0);
+        {  // three right-hand sides, column-major
+            std::vector<double> B;
+            for (int k = 0; k < 3; k++) B.insert(B.end(), b.begin(), b.end());
+            auto gB = hala::make_gpu_vector(B);
+            hala::gpu_vector<double> gX;
+            auto bB = hala::bind_engine_vector(engine, gB);
+            auto bX = hala::bind_engine_vector(engine, gX);
+            hala::solve_batch_cg_ilu({1.E-12, 50}, bp, bi, bv, ilu, bB, bX);
+            std::vector<double> X;
+            gX.unload(X);
+            CHECK(X.size() == 15 && diffsum(std::vector<double>(X.begin() + 10, X.end()), xref) < 1e-10);
+        }
+        // float values: the tolerance is a float, compared in float
+        std::vector<float> vals32(vals.begin(), vals.end()), b32(b.begin(), b.end());
+        auto gv32 = hala::make_gpu_vector(vals32);
+        auto gb32 = hala::make_gpu_vector(b32);
+        hala::gpu_vector<float> gx32;
+        auto ilu32 = hala::make_ilu(engine, gp, gi, gv32);
+        static_assert(std::is_same<hala::get_precision_type<hala::gpu_vector<std::complex<float>>>, float>::value,
+                      "precision of complex<float> is float");
+        int it32 = hala::solve_cg_ilu(engine, hala::stop_criteria<float>(1.E-4f, 50), gp, gi, gv32, ilu32, gb32, gx32);
+        std::vector<float> x32;
+        gx32.unload(x32);
+        double e32 = 0;
+        for (int k = 0; k < 5; k++) e32 += std::fabs(x32[k] - xref[k]);
+        CHECK(it32 > 1 && it32 < 50 && e32 < 1e-4);
+        hala::gpu_vector<float> gy32;
+        auto bv32 = hala::bind_engine_vector(engine, gv32);
+        auto bb32 = hala::bind_engine_vector(engine, gb32);
+        auto by32 = hala::bind_engine_vector(engine, gy32);
+        CHECK(hala::solve_cg_ilu(hala::stop_criteria<float>(1.E-4f, 50), bp, bi, bv32, ilu32, bb32, by32) == it32);
+        hala::axpy(engine, -1.0f, gx32, gy32);
+        CHECK(hala::norm2(engine, gy32) == 0.0f);
+        std::printf("PASS no-engine solvers, stop_criteria<float>\n");
     }
     // mixed precision refinement reaches the FP64 residual
     {
