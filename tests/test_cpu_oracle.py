@@ -133,6 +133,68 @@ def test_gmres_reference_cases():
     assert np.abs(x - xref).sum() < 1e-8
 
 
+NONSYM5 = (np.array([0, 2, 5, 8, 11, 12], np.int32), np.array([0, 1, 0, 1, 3, 1, 2, 3, 0, 2, 3, 4], np.int32),
+           np.array([2.0, 1.0, 1.0, 3.0, 1.0, 1.0, 4.0, 1.0, 1.0, 1.0, 5.0, 2.0]))
+
+
+def _drotg_r(a, b):
+    """r of the reference-BLAS drotg: +-sqrt(a^2 + b^2), signed like the larger of |a|, |b|."""
+    roe = a if abs(a) > abs(b) else b
+    return np.copysign(np.hypot(a, b), roe if roe != 0 else 1.0)
+
+
+def test_gmres1_cycle_closed_form_unrotated_last_z():
+    # One GMRES(1) cycle from x0 = 0 (identity preconditioner).  With w = b/|b|,
+    # h = <w, A w> and e = |A w - h w|, the Givens rotation of (h, e) gives
+    # rho = drotg's r and c = h / rho.  The least-squares step is |b| c / rho;
+    # HALA never applies the cycle's last rotation to Z
+    # (hex/solvers/hala_solvers_gmres.hpp:210-216), so its step is |b| / rho.
+    p, i, v = NONSYM5
+    A = dense(p, i, v, 5)
+    b = np.array([1.0, -2.0, 0.5, 3.0, -1.0])
+    _, x, _ = oracle.gmres(p, i, v, b, tol=0.0, restart=1, max_outer=1, prec=oracle.PREC_IDENTITY)
+    beta = np.linalg.norm(b)
+    w = b / beta
+    h = w @ (A @ w)
+    e = np.linalg.norm(A @ w - h * w)
+    rho = _drotg_r(h, e)
+    c = h / rho
+    assert abs(c) < 0.95  # the two variants are far apart on this system
+    np.testing.assert_allclose(x, (beta / rho) * w, rtol=1e-13, atol=1e-15)
+    assert np.linalg.norm(x - (beta * c / rho) * w) > 1e-2 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_gmres_cycle_differs_from_least_squares_by_last_rotation(k):
+    # One GMRES(k) cycle of k steps from x0 = 0.  On the Krylov space W_k the
+    # least-squares step y* solves R y = g (A W_k = Q R, g = Q^T b); HALA keeps
+    # Z's last entry unrotated, i.e. g_k / c_k instead of g_k, so
+    # x - x* = (1/c_k - 1) W_k R^-1 g_k e_k, with |c_k| = sqrt(R_kk^2 - e_k^2) / |R_kk|
+    # and e_k the norm of A w_k outside W_k.  The rotated variant would give x = x*.
+    p, i, v = st.convdiff27(3, seed=2)
+    n = len(p) - 1
+    A = dense(p, i, v, n)
+    b = st.uniform(11, n)
+    _, x, _ = oracle.gmres(p, i, v, b, tol=0.0, restart=k, max_outer=1, prec=oracle.PREC_IDENTITY)
+    K = np.empty((n, k))
+    K[:, 0] = b
+    for j in range(1, k):
+        K[:, j] = A @ K[:, j - 1]
+    W, _ = np.linalg.qr(K)
+    Q, R = np.linalg.qr(A @ W)
+    g = Q.T @ b
+    xs = W @ np.linalg.solve(R, g)
+    d = W @ np.linalg.solve(R, np.eye(k)[:, -1] * g[-1])
+    wk = W[:, -1]
+    aw = A @ wk
+    e_k = np.linalg.norm(aw - W @ (W.T @ aw))
+    c_abs = np.sqrt(R[-1, -1] ** 2 - e_k**2) / abs(R[-1, -1])
+    t = (x - xs) @ d / (d @ d)
+    np.testing.assert_allclose(x - xs, t * d, atol=1e-9 * np.linalg.norm(x))  # along the predicted direction
+    assert min(abs(t - (1 / c_abs - 1)), abs(t - (-1 / c_abs - 1))) < 1e-7 * max(1.0, abs(t))
+    assert abs(t) > 1e-3  # and not the least-squares step
+
+
 # ---------------------------------------------------------------- HALA's compiled kernels
 REF_CASES = {
     "lap2d_16x12": lambda: st.laplace2d(16, 12),
