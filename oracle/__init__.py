@@ -27,6 +27,8 @@ ch = C.c_char
 pi32 = C.POINTER(C.c_int)
 
 _SIG = {
+    "hala_or_set_threads": (None, [i32]),
+    "hala_or_get_threads": (i32, []),
     "hala_or_cdot": (f64, [i64, vp, vp]),
     "hala_or_cnrm2": (f64, [i64, vp]),
     "hala_or_team_dot": (f64, [i64, vp, vp, i32]),
@@ -123,6 +125,24 @@ def ci32(a):
 
 
 # ---- thin numpy-friendly wrappers ---------------------------------------------
+def set_threads(t):
+    """Threads of the oracle (1 by default); every result is bit-identical for any count."""
+    lib().hala_or_set_threads(int(t))
+
+
+def get_threads():
+    return lib().hala_or_get_threads()
+
+
+def host_threads(cap=16):
+    """The CPU threads this process may use, at most cap."""
+    try:
+        k = len(os.sched_getaffinity(0))
+    except AttributeError:
+        k = os.cpu_count() or 1
+    return max(1, min(cap, k))
+
+
 def cdot(x, y):
     x, y = c64(x), c64(y)
     return lib().hala_or_cdot(x.size, P(x), P(y))

@@ -16,6 +16,19 @@
 
 #define CB 256 /* canonical block: elements per block and width of the final pass */
 
+/*
+ * Threads.  The default is one.  With more, only work whose per-element
+ * operation order does not depend on the split runs in parallel: rows of a
+ * product, canonical 256-element blocks of a dot product, elementwise vector
+ * updates, rows of one level of a triangular sweep or of the ILU(0)
+ * factorisation (a level's rows only read rows of earlier levels).  Every
+ * result is therefore bit-identical for any thread count.
+ */
+static int or_threads = 1;
+void hala_or_set_threads(int t) { or_threads = t < 1 ? 1 : t; }
+int hala_or_get_threads(void) { return or_threads; }
+#define OR_PAR(n) if (or_threads > 1 && (n) > 4096) num_threads(or_threads)
+
 /* Pairwise halving tree over a[0..w): a[t] += a[t+off] for off = w/2 .. 1. */
 static double tree_reduce(double *a, int w) {
     for (int off = w / 2; off >= 1; off >>= 1)
@@ -37,12 +50,14 @@ double hala_or_cdot(int64_t n, const double *x, const double *y) {
     int64_t nb = (n + CB - 1) / CB;
     double *part = (double *)malloc((size_t)nb * sizeof(double));
     double a[CB];
+#pragma omp parallel for schedule(static) OR_PAR(n)
     for (int64_t b = 0; b < nb; b++) {
+        double w[CB];
         for (int t = 0; t < CB; t++) {
             int64_t i = b * CB + t;
-            a[t] = (i < n) ? x[i] * y[i] : 0.0;
+            w[t] = (i < n) ? x[i] * y[i] : 0.0;
         }
-        part[b] = tree_reduce(a, CB);
+        part[b] = tree_reduce(w, CB);
     }
     for (int t = 0; t < CB; t++) {
         double s = 0.0;
@@ -74,10 +89,16 @@ double hala_or_team_dot(int64_t n, const double *x, const double *y, int team) {
 
 /* BLAS-1 in reference-BLAS element order (blas/hala_blas_1.hpp:210-217, :317-335). */
 static void axpy(int n, double a, const double *x, double *y) {
+#pragma omp parallel for schedule(static) OR_PAR(n)
     for (int i = 0; i < n; i++) y[i] = y[i] + a * x[i];
 }
 static void scal(int n, double a, double *x) {
+#pragma omp parallel for schedule(static) OR_PAR(n)
     for (int i = 0; i < n; i++) x[i] = a * x[i];
+}
+static void vcopy(int n, const double *x, double *y) {
+#pragma omp parallel for schedule(static) OR_PAR(n)
+    for (int i = 0; i < n; i++) y[i] = x[i];
 }
 
 /*
@@ -90,6 +111,7 @@ static void scal(int n, double a, double *x) {
 void hala_or_spmv(char trans, int nrows, int ncols, double alpha, const int *pntr, const int *indx,
                   const double *vals, const double *x, double beta, double *y) {
     if (trans == 'N' || trans == 'n') {
+#pragma omp parallel for schedule(static) OR_PAR(nrows)
         for (int i = 0; i < nrows; i++) {
             double sum = 0.0;
             for (int j = pntr[i]; j < pntr[i + 1]; j++) sum = sum + vals[j] * x[indx[j]];
@@ -109,6 +131,7 @@ void hala_or_spmv(char trans, int nrows, int ncols, double alpha, const int *pnt
 /* FP32 'N' product with FP32 accumulation, same order as hala_or_spmv. */
 void hala_or_spmv_f32(int nrows, float alpha, const int *pntr, const int *indx, const float *vals,
                       const float *x, float beta, float *y) {
+#pragma omp parallel for schedule(static) OR_PAR(nrows)
     for (int i = 0; i < nrows; i++) {
         float sum = 0.0f;
         for (int j = pntr[i]; j < pntr[i + 1]; j++) sum = sum + vals[j] * x[indx[j]];
@@ -173,6 +196,83 @@ int hala_or_levels(int n, const int *pntr, const int *indx, const int *diag, int
 }
 
 /*
+ * Level schedule of both sweeps over the combined pattern, for the threaded
+ * factorisation and applies: rows of one level of L (of U) depend only on
+ * rows of earlier levels.
+ */
+typedef struct {
+    int nl, nu;
+    int *lptr, *lperm, *uptr, *uperm;
+} sched_t;
+
+static void sched_free(sched_t *s) {
+    if (!s) return;
+    free(s->lptr);
+    free(s->lperm);
+    free(s->uptr);
+    free(s->uperm);
+    free(s);
+}
+
+static sched_t *sched_build(int n, const int *pntr, const int *indx, const int *diag) {
+    if (or_threads <= 1 || n <= 4096) return 0;
+    sched_t *s = (sched_t *)calloc(1, sizeof(sched_t));
+    int *lev = (int *)malloc((size_t)n * sizeof(int));
+    s->lptr = (int *)malloc((size_t)(n + 1) * sizeof(int));
+    s->uptr = (int *)malloc((size_t)(n + 1) * sizeof(int));
+    s->lperm = (int *)malloc((size_t)n * sizeof(int));
+    s->uperm = (int *)malloc((size_t)n * sizeof(int));
+    s->nl = hala_or_levels(n, pntr, indx, diag, 0, lev, s->lptr, s->lperm);
+    s->nu = hala_or_levels(n, pntr, indx, diag, 1, lev, s->uptr, s->uperm);
+    free(lev);
+    return s;
+}
+
+/* the sweeps of one triangle, level by level (rows of a level in parallel) */
+#define OR_ILU_APPLY_SCHED(NAME, R)                                                                    \
+    static void NAME(const sched_t *s, const int *pntr, const int *indx, const int *diag, const R *ilu, \
+                     R *x) {                                                                           \
+        _Pragma("omp parallel num_threads(or_threads)") {                                              \
+            for (int l = 1; l < s->nl; l++) {                                                          \
+                _Pragma("omp for schedule(static)") for (int q = s->lptr[l]; q < s->lptr[l + 1]; q++) { \
+                    const int i = s->lperm[q];                                                         \
+                    for (int j = pntr[i]; j < diag[i]; j++) x[i] = x[i] - ilu[j] * x[indx[j]];         \
+                }                                                                                      \
+            }                                                                                          \
+            for (int l = 0; l < s->nu; l++) {                                                          \
+                _Pragma("omp for schedule(static)") for (int q = s->uptr[l]; q < s->uptr[l + 1]; q++) { \
+                    const int i = s->uperm[q];                                                         \
+                    for (int j = diag[i] + 1; j < pntr[i + 1]; j++) x[i] = x[i] - ilu[j] * x[indx[j]]; \
+                    x[i] = x[i] / ilu[diag[i]];                                                        \
+                }                                                                                      \
+            }                                                                                          \
+        }                                                                                              \
+    }
+OR_ILU_APPLY_SCHED(ilu_apply_sched, double)
+OR_ILU_APPLY_SCHED(ilu_apply_sched_f32, float)
+
+/* IKJ elimination of row j (see hala_or_ilu0) */
+static void ilu0_row(int j, const int *pntr, const int *indx, const int *diag, double *ilu) {
+    for (int jc = pntr[j]; jc < diag[j]; jc++) {
+        int i = indx[jc];
+        ilu[jc] = ilu[jc] / ilu[diag[i]];
+        double l = ilu[jc];
+        int ik = diag[i] + 1, jk = jc + 1;
+        while (ik < pntr[i + 1] && jk < pntr[j + 1]) {
+            if (indx[ik] == indx[jk]) {
+                ilu[jk] = ilu[jk] - l * ilu[ik];
+                ik++;
+                jk++;
+            } else if (indx[ik] < indx[jk]) {
+                ik++;
+            } else {
+                jk++;
+            }
+        }
+    }
+}
+
+/*
  * ILU(0) in IKJ (row) order.  HALA's factorize_ilu_array
  * (sparse/hala_sparse_utils.hpp:229-254) loops KIJ: for each pivot i it
  * updates every later row j holding a_ji.  Entry (j,k) then receives the same
@@ -181,25 +281,17 @@ int hala_or_levels(int n, const int *pntr, const int *indx, const int *diag, int
  * every entry at O(nnz * row length) cost instead of O(n^2).
  */
 void hala_or_ilu0(int n, const int *pntr, const int *indx, const int *diag, double *ilu) {
-    for (int j = 0; j < n; j++) {
-        for (int jc = pntr[j]; jc < diag[j]; jc++) {
-            int i = indx[jc];
-            ilu[jc] = ilu[jc] / ilu[diag[i]];
-            double l = ilu[jc];
-            int ik = diag[i] + 1, jk = jc + 1;
-            while (ik < pntr[i + 1] && jk < pntr[j + 1]) {
-                if (indx[ik] == indx[jk]) {
-                    ilu[jk] = ilu[jk] - l * ilu[ik];
-                    ik++;
-                    jk++;
-                } else if (indx[ik] < indx[jk]) {
-                    ik++;
-                } else {
-                    jk++;
-                }
-            }
-        }
+    sched_t *s = sched_build(n, pntr, indx, diag);
+    if (!s) {
+        for (int j = 0; j < n; j++) ilu0_row(j, pntr, indx, diag, ilu);
+        return;
     }
+#pragma omp parallel num_threads(or_threads)
+    for (int l = 1; l < s->nl; l++) {
+#pragma omp for schedule(static)
+        for (int q = s->lptr[l]; q < s->lptr[l + 1]; q++) ilu0_row(s->lperm[q], pntr, indx, diag, ilu);
+    }
+    sched_free(s);
 }
 
 /* HALA's own KIJ loop order (sparse/hala_sparse_utils.hpp:229-254), O(n^2): small inputs only. */
@@ -269,6 +361,7 @@ void hala_or_ilu_apply_f32_jacobi(int n, const int *pntr, const int *indx, const
     const float *yo = x; /* y0 = x */
     float *yn = ya;
     for (int m = 0; m < k; m++) {
+#pragma omp parallel for schedule(static) OR_PAR(n)
         for (int i = 0; i < n; i++) {
             float acc = x[i];
             for (int j = pntr[i]; j < diag[i]; j++) acc = acc - ilu[j] * yo[indx[j]];
@@ -280,9 +373,11 @@ void hala_or_ilu_apply_f32_jacobi(int n, const int *pntr, const int *indx, const
     /* y(k) is in yo; z0 into x (x no longer needed), then alternate x / zb */
     const float *y = yo;
     float *zo = (yo == ya) ? yb : ya; /* the free y buffer holds z0 */
+#pragma omp parallel for schedule(static) OR_PAR(n)
     for (int i = 0; i < n; i++) zo[i] = y[i] / ilu[diag[i]];
     float *zn = zb;
     for (int m = 0; m < k; m++) {
+#pragma omp parallel for schedule(static) OR_PAR(n)
         for (int i = 0; i < n; i++) {
             float acc = y[i];
             for (int j = diag[i] + 1; j < pntr[i + 1]; j++) acc = acc - ilu[j] * zo[indx[j]];
@@ -364,19 +459,22 @@ typedef struct {
     int n;
     const int *pntr, *indx, *diag;
     const double *data;
+    sched_t *sched; /* level schedule of the ILU sweeps (threaded runs), or null */
 } prec_t;
 
 static void prec_apply(const prec_t *m, const double *r, double *z) {
     int n = m->n;
     switch (m->kind) {
     case HALA_OR_PREC_IDENTITY: /* vcopy */
-        memcpy(z, r, (size_t)n * sizeof(double));
+        vcopy(n, r, z);
         break;
     case HALA_OR_PREC_ILU: /* cpu_ilu::apply: copy then sweep, sparse/hala_sparse_ilu.hpp:105-115 */
-        memcpy(z, r, (size_t)n * sizeof(double));
-        hala_or_ilu_apply(n, m->pntr, m->indx, m->diag, m->data, z);
+        vcopy(n, r, z);
+        if (m->sched) ilu_apply_sched(m->sched, m->pntr, m->indx, m->diag, m->data, z);
+        else hala_or_ilu_apply(n, m->pntr, m->indx, m->diag, m->data, z);
         break;
     case HALA_OR_PREC_JACOBI: /* vdivide, wax/hala_blas_extensions.hpp:139-151 */
+#pragma omp parallel for schedule(static) OR_PAR(n)
         for (int i = 0; i < n; i++) z[i] = r[i] / m->data[i];
         break;
     case HALA_OR_PREC_ILU_T:
@@ -405,11 +503,11 @@ static int cg_core(int n, const int *pntr, const int *indx, const double *vals, 
     double *p = (double *)malloc((size_t)n * sizeof(double));
     double *ap = (double *)malloc((size_t)n * sizeof(double));
     double *z = (double *)malloc((size_t)n * sizeof(double));
-    memcpy(r, b, (size_t)n * sizeof(double));
+    vcopy(n, b, r);
     hala_or_spmv('N', n, n, 1.0, pntr, indx, vals, x, 0.0, p);
     axpy(n, -1.0, p, r);
     prec_apply(m, r, z);
-    memcpy(p, z, (size_t)n * sizeof(double));
+    vcopy(n, z, p);
     double zr = dot(n, r, z, team);
     int it = 1;
     for (;;) {
@@ -439,8 +537,11 @@ static int cg_core(int n, const int *pntr, const int *indx, const double *vals, 
 int hala_or_cg(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *mpntr,
                const int *mindx, const int *mdiag, const double *mdata, const double *b, double *x, double tol,
                int max_iter, double *hist, int max_hist) {
-    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata};
-    return cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, dot_canon, 0, hist, max_hist);
+    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata, 0};
+    if (prec == HALA_OR_PREC_ILU) m.sched = sched_build(n, mpntr, mindx, mdiag);
+    int it = cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, dot_canon, 0, hist, max_hist);
+    sched_free(m.sched);
+    return it;
 }
 
 /*
@@ -458,7 +559,7 @@ int hala_or_cg(int n, const int *pntr, const int *indx, const double *vals, int 
 int hala_or_cg_batch(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *mpntr,
                      const int *mindx, const int *mdiag, const double *mdata, int nrhs, const double *B, double *X,
                      double tol, int max_iter, double *hist, int max_hist) {
-    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata};
+    prec_t m = {prec, n, mpntr, mindx, mdiag, mdata, 0};
     const size_t N = (size_t)n * nrhs;
     double *r = (double *)malloc(N * sizeof(double));
     double *p = (double *)malloc(N * sizeof(double));
@@ -511,7 +612,7 @@ int hala_or_cg_batch(int n, const int *pntr, const int *indx, const double *vals
 /* Jacobi-PCG of one batched system, reductions in team order. */
 int hala_or_cg_team(int n, const int *pntr, const int *indx, const double *vals, const double *dvals,
                     const double *b, double *x, double tol, int max_iter, int team, double *hist, int max_hist) {
-    prec_t m = {HALA_OR_PREC_JACOBI, n, 0, 0, 0, dvals};
+    prec_t m = {HALA_OR_PREC_JACOBI, n, 0, 0, 0, dvals, 0};
     return cg_core(n, pntr, indx, vals, &m, b, x, tol, max_iter, hala_or_team_dot, team, hist, max_hist);
 }
 
@@ -584,6 +685,7 @@ typedef struct {
     float *w32a, *w32b;
     int jacobi;  /* FP32 ILU: k > 0 Jacobi sweeps per triangle instead of the exact sweeps */
     float *wj;   /* 3n floats for the Jacobi apply */
+    sched_t *sched; /* level schedule of the ILU sweeps (threaded runs), or null */
 } gm_op_t;
 
 /* t = alpha*A*x + beta*t in FP64, or the FP32 product converted back. */
@@ -592,8 +694,10 @@ static void gm_spmv(gm_op_t *o, double alpha, const double *x, double beta, doub
         hala_or_spmv('N', o->n, o->n, alpha, o->pntr, o->indx, o->vals, x, beta, t);
         return;
     }
+#pragma omp parallel for schedule(static) OR_PAR(o->n)
     for (int i = 0; i < o->n; i++) o->w32a[i] = (float)x[i];
     hala_or_spmv_f32(o->n, 1.0f, o->pntr, o->indx, o->vals32, o->w32a, 0.0f, o->w32b);
+#pragma omp parallel for schedule(static) OR_PAR(o->n)
     for (int i = 0; i < o->n; i++)
         t[i] = (beta == 0.0) ? alpha * (double)o->w32b[i] + 0.0 : alpha * (double)o->w32b[i] + beta * t[i];
 }
@@ -601,16 +705,21 @@ static void gm_spmv(gm_op_t *o, double alpha, const double *x, double beta, doub
 static void gm_prec(gm_op_t *o, const double *t, double *r) {
     int n = o->n;
     if (o->prec == HALA_OR_PREC_IDENTITY) {
-        memcpy(r, t, (size_t)n * sizeof(double));
+        vcopy(n, t, r);
     } else if (!o->ilu32) {
-        memcpy(r, t, (size_t)n * sizeof(double));
-        hala_or_ilu_apply(n, o->pntr, o->indx, o->diag, o->ilu, r);
+        vcopy(n, t, r);
+        if (o->sched) ilu_apply_sched(o->sched, o->pntr, o->indx, o->diag, o->ilu, r);
+        else hala_or_ilu_apply(n, o->pntr, o->indx, o->diag, o->ilu, r);
     } else {
+#pragma omp parallel for schedule(static) OR_PAR(n)
         for (int i = 0; i < n; i++) o->w32a[i] = (float)t[i];
         if (o->jacobi > 0)
             hala_or_ilu_apply_f32_jacobi(n, o->pntr, o->indx, o->diag, o->ilu32, o->jacobi, o->w32a, o->wj);
+        else if (o->sched)
+            ilu_apply_sched_f32(o->sched, o->pntr, o->indx, o->diag, o->ilu32, o->w32a);
         else
             hala_or_ilu_apply_f32(n, o->pntr, o->indx, o->diag, o->ilu32, o->w32a);
+#pragma omp parallel for schedule(static) OR_PAR(n)
         for (int i = 0; i < n; i++) r[i] = (double)o->w32a[i];
     }
 }
@@ -643,12 +752,8 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
     int first = 1;
     while (outer_res > tol && outer < max_outer) {
         int nH = 0, nZ = 0;
-        if (x_zero && outer == 0) {
-            memcpy(t, b, (size_t)n * sizeof(double));
-        } else {
-            memcpy(t, b, (size_t)n * sizeof(double));
-            gm_spmv(o, -1.0, x, 1.0, t);
-        }
+        vcopy(n, b, t);
+        if (!(x_zero && outer == 0)) gm_spmv(o, -1.0, x, 1.0, t);
         gm_prec(o, t, r);
         total++;
         inner_res = hala_or_cnrm2(n, r);
@@ -658,13 +763,14 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
         }
         scal(n, 1.0 / inner_res, r);
         Z[nZ++] = inner_res;
-        memcpy(W, r, (size_t)n * sizeof(double));
+        vcopy(n, r, W);
         int k = 0;
         while (inner_res > tol && k < restart) {
             gm_spmv(o, 1.0, r, 0.0, t);
             gm_prec(o, t, r);
             total++;
             for (int j = 0; j <= k; j++) cf[j] = hala_or_cdot(n, W + (size_t)j * n, r);
+#pragma omp parallel for schedule(static) OR_PAR(n)
             for (int i = 0; i < n; i++) {
                 double yi = r[i];
                 for (int j = 0; j <= k; j++) yi = yi + (-cf[j]) * W[(size_t)j * n + i];
@@ -683,13 +789,14 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
             nh++;
             k++;
             if (inner_res > tol && k < restart) {
-                memcpy(W + (size_t)k * n, r, (size_t)n * sizeof(double));
+                vcopy(n, r, W + (size_t)k * n);
                 Z[nZ++] = 0.0;
                 hala_or_rot(&Z[k - 1], &Z[k], c, s);
             }
         }
         if (nH > 0) {
             hala_or_tpsv_un(nZ, H, Z);
+#pragma omp parallel for schedule(static) OR_PAR(n)
             for (int i = 0; i < n; i++) {
                 double yi = x[i];
                 for (int j = 0; j < nZ; j++) yi = yi + (1.0 * Z[j]) * W[(size_t)j * n + i];
@@ -714,9 +821,12 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
 int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,
                   const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
                   double *hist, int max_hist, int *nhist) {
-    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0, 0, 0};
+    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0, 0, 0, 0};
     if (nhist) *nhist = 0;
-    return gmres_core(&o, b, x, tol, 0.0, 0, restart, max_outer, hist, max_hist, nhist);
+    if (prec == HALA_OR_PREC_ILU) o.sched = sched_build(n, pntr, indx, diag);
+    int total = gmres_core(&o, b, x, tol, 0.0, 0, restart, max_outer, hist, max_hist, nhist);
+    sched_free(o.sched);
+    return total;
 }
 
 /*
@@ -741,11 +851,11 @@ int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals
     double *r = (double *)malloc((size_t)n * sizeof(double));
     double *d = (double *)malloc((size_t)n * sizeof(double));
     gm_op_t o = {n, pntr, indx, diag, vals, vals32, ilu, vals32 ? ilu32 : 0, HALA_OR_PREC_ILU, w32a, w32b,
-                 vals32 ? jacobi : 0, wj};
+                 vals32 ? jacobi : 0, wj, sched_build(n, pntr, indx, diag)};
     double bn = hala_or_cnrm2(n, b);
     int total = 0, nh = 0;
     for (int it = 0; it <= max_refine; it++) {
-        memcpy(r, b, (size_t)n * sizeof(double));
+        vcopy(n, b, r);
         hala_or_spmv('N', n, n, -1.0, pntr, indx, vals, x, 1.0, r);
         double rel = hala_or_cnrm2(n, r) / bn;
         if (hist && nh < max_hist) hist[nh] = rel;
@@ -756,6 +866,7 @@ int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals
         axpy(n, 1.0, d, x);
     }
     if (nhist) *nhist = nh;
+    sched_free(o.sched);
     free(w32a);
     free(w32b);
     free(wj);

@@ -26,6 +26,10 @@ extern "C" {
 #endif
 
 /* ---- canonical reductions ------------------------------------------------ */
+/* Threads of the oracle (default 1); results are bit-identical for any count. */
+void hala_or_set_threads(int t);
+int hala_or_get_threads(void);
+
 double hala_or_cdot(int64_t n, const double *x, const double *y);
 double hala_or_cnrm2(int64_t n, const double *x);
 double hala_or_team_dot(int64_t n, const double *x, const double *y, int team);

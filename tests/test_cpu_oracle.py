@@ -517,3 +517,38 @@ def test_generator_sizes_and_structure():
     sides = np.diff(bl["row_off"])
     assert len(sides) == 4096 and sides.min() >= 196 and sides.max() <= 4900
     assert np.array_equal(st.uniform(3, 10), st.uniform(3, 10))
+
+
+def test_threaded_oracle_is_bit_identical():
+    # the threaded oracle splits only order-free work (rows, canonical blocks, rows of a level)
+    p, i, v = st.laplace3d(24, 20, 18)
+    n = len(p) - 1
+    b = st.uniform(21, n)
+    d, _ = oracle.diag(p, i)
+    q, j, w = st.convdiff27(18, seed=5)  # n = 5832: above the threading threshold
+    m = len(q) - 1
+    c = st.uniform(22, m)
+    dq, _ = oracle.diag(q, j)
+
+    def run():
+        f = oracle.ilu0(p, i, v, d)
+        out = [f, oracle.spmv("N", n, n, 1.5, p, i, v, b, -0.5, b), np.array([oracle.cdot(b, b[::-1].copy())])]
+        out += list(oracle.cg(p, i, v, b, tol=1e-10 * oracle.cnrm2(b), prec=oracle.PREC_ILU, mpntr=p, mindx=i,
+                              mdiag=d, mdata=f)[1:])
+        fq = oracle.ilu0(q, j, w, dq)
+        out += list(oracle.gmres(q, j, w, c, tol=1e-11, restart=20, dg=dq, ilu=fq)[1:])
+        for kw in ({"fp32": True}, {"fp32": True, "jacobi": 3}, {"fp32": False}):
+            out += list(oracle.gmres_ir(q, j, w, dq, fq, c, **kw)[1:])
+        return out
+
+    t0 = oracle.get_threads()
+    try:
+        oracle.set_threads(1)
+        one = run()
+        oracle.set_threads(4)
+        four = run()
+    finally:
+        oracle.set_threads(t0)
+    assert len(one) == len(four)
+    for a, b_ in zip(one, four):
+        assert np.array_equal(bits(np.asarray(a)), bits(np.asarray(b_)))

@@ -1,0 +1,255 @@
+"""Every benchmark workload at its stated size on the GPU, against the oracle.
+
+  C1  plain CG, 256^2 5-point Laplacian               history, iterations, x bit for bit
+  C2  ILU(0)-PCG, 128^3 7-point Poisson               symbolic results, factor, history, x bit for bit
+  C3  GMRES(30) + ILU(0), 96^3 27-point conv.-diff.    FP32-inner and Jacobi-applied-FP32-inner
+                                                       refinement and plain FP64 GMRES bit for bit
+  C4  4096 batched 2-D Laplacians, sides 14..70        every system's iterations and x bit for bit
+  C5  adjoint gradient, 64^3 variable diffusion        gradient bit for bit vs the oracle gradient;
+                                                       central differences (h = 1e-6) on 32 entries
+
+The oracle runs threaded here (its results are bit-identical for any thread
+count) so that each case takes seconds of host time.
+"""
+import concurrent.futures as cf
+
+import numpy as np
+import pytest
+
+import hala_amd as H
+import oracle
+from hala_amd import stencils as st
+from tests.conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(engine, a, dtype=None):
+    return engine.load(a, dtype)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def say(msg):
+    print(f"[config] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def oracle_threads():
+    t0 = oracle.get_threads()
+    oracle.set_threads(oracle.host_threads())
+    yield
+    oracle.set_threads(t0)
+
+
+def relres(p, i, v, x, b):
+    n = len(p) - 1
+    r = b - oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def test_c1_cg_poisson2d_256(engine):
+    import torch
+
+    say("C1: plain CG, 256^2 5-point")
+    p, i, v = st.laplace2d(256, 256)
+    n = len(p) - 1
+    b = st.uniform(1, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    it, x, hist = oracle.cg(p, i, v, b, tol=tol)
+    gp, gi, gv, gb = dev(engine, p), dev(engine, i), dev(engine, v), dev(engine, b)
+    gx = engine.vector(n)
+    gh = torch.zeros(4096, dtype=torch.float64, device=gx.device)
+    git = H.solve_cg(engine, tol, gp, gi, gv, None, gb, gx, hist=gh)
+    assert git == it
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    assert hist[-1] < tol and relres(p, i, v, x, b) < 2e-10
+
+
+def test_c2_ilu_pcg_poisson3d_128(engine):
+    import torch
+
+    say("C2: ILU(0)-PCG, 128^3 7-point")
+    p, i, v = st.laplace3d(128, 128, 128)
+    n = len(p) - 1
+    b = st.uniform(2, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    dg, bad = oracle.diag(p, i)
+    assert bad == -1
+    f = oracle.ilu0(p, i, v, dg)
+    gp, gi, gv, gb = dev(engine, p), dev(engine, i), dev(engine, v), dev(engine, b)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    # symbolic results: diagonal, level sets and (level, row) permutations of both triangles
+    for upper in (False, True):
+        lev, lptr, perm = oracle.levels(p, i, dg, upper)
+        gd, glev, glptr, gperm = (host(t) for t in ilu.symbolic(upper))
+        assert np.array_equal(gd, dg)
+        assert np.array_equal(glev, lev) and np.array_equal(glptr, lptr) and np.array_equal(gperm, perm)
+        assert len(glptr) == 383  # level = i + j + k: 382 levels
+    assert np.array_equal(bits(host(ilu.get_factor())), bits(f))
+    say("C2: oracle PCG")
+    it, x, hist = oracle.cg(p, i, v, b, tol=tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg, mdata=f)
+    gx = engine.vector(n)
+    gh = torch.zeros(1024, dtype=torch.float64, device=gx.device)
+    git = H.solve_cg_ilu(engine, tol, gp, gi, gv, ilu, gb, gx, hist=gh)
+    assert git == it
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    assert relres(p, i, v, x, b) < 2e-10
+
+
+@pytest.fixture(scope="module")
+def c3():
+    p, i, v = st.convdiff27(96, seed=3)
+    n = len(p) - 1
+    b = st.uniform(33, n)
+    dg, bad = oracle.diag(p, i)
+    assert bad == -1
+    return p, i, v, b, dg, oracle.ilu0(p, i, v, dg)
+
+
+@pytest.mark.parametrize("variant", ["fp32_inner", "fp32_inner_jacobi3", "fp64_inner"])
+def test_c3_gmres_ir_convdiff27_96(engine, c3, variant):
+    import torch
+
+    say(f"C3: GMRES(30) refinement, 96^3 27-point, {variant}")
+    p, i, v, b, dg, f = c3
+    n = len(p) - 1
+    k = 3 if variant.endswith("jacobi3") else 0
+    fp32 = variant != "fp64_inner"
+    tot, x, hist = oracle.gmres_ir(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40,
+                                   fp32=fp32, jacobi=k)
+    gp, gi, gv, gb = dev(engine, p), dev(engine, i), dev(engine, v), dev(engine, b)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+    ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
+    gx = engine.vector(n)
+    gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
+    saved = engine.get_option("ilu32_jacobi")
+    try:
+        engine.set_option("ilu32_jacobi", k)
+        gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv.float() if fp32 else None, ilu, gb, gx, rtol=1e-10,
+                                     inner_rtol=1e-4, restart=30, max_refine=40, hist=gh)
+    finally:
+        engine.set_option("ilu32_jacobi", saved)
+    assert gtot == tot and gnh == len(hist)
+    assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    assert hist[-1] < 1e-10 and relres(p, i, v, x, b) < 1e-10
+
+
+def test_c3_plain_fp64_gmres30_ilu0(engine, c3):
+    # the plain FP64 path: HALA's solve_gmres_ilu (no refinement), stopped on its
+    # preconditioned Givens estimate; 1e-13 of the preconditioned rhs norm is the
+    # largest power of ten that brings the true relative residual under 1e-10 here
+    # (1e-12 stops at 2.4e-10)
+    import torch
+
+    say("C3: plain FP64 GMRES(30) + ILU(0), 96^3 27-point")
+    p, i, v, b, dg, f = c3
+    n = len(p) - 1
+    mb = oracle.ilu_apply(p, i, dg, f, b)
+    tol = 1e-13 * oracle.cnrm2(mb)
+    tot, x, hist = oracle.gmres(p, i, v, b, tol=tol, restart=30, dg=dg, ilu=f)
+    gp, gi, gv, gb = dev(engine, p), dev(engine, i), dev(engine, v), dev(engine, b)
+    ilu = H.make_ilu(engine, gp, gi, gv)
+    gx = engine.vector(n)
+    gh = torch.zeros(4096, dtype=torch.float64, device=gx.device)
+    gtot, gnh = H.solve_gmres_ilu(engine, tol, 30, gp, gi, gv, ilu, gb, gx, hist=gh)
+    assert gtot == tot and gnh == len(hist)
+    assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
+    assert np.array_equal(bits(host(gx)), bits(x))
+    assert relres(p, i, v, x, b) < 1e-10
+
+
+def test_c4_batched_4096(engine):
+    import torch
+
+    say("C4: 4096 batched Jacobi-PCG systems")
+    d = st.batch_laplace(4096, seed=4, side_lo=14, side_hi=70)
+    N = len(d["dvals"])
+    b = st.uniform(104, N)
+    g = {k: dev(engine, d[k]) for k in ("row_off", "pntr", "indx", "vals", "dvals")}
+    gb = dev(engine, b)
+    bs = H.batch_solver(engine, g["row_off"], g["pntr"], g["indx"])
+    gx = engine.vector(N)
+    team, wg, _ = bs.tune(g["vals"], g["dvals"], gb, gx, rtol=1e-10, reps=1)
+    iters = torch.zeros(4096, dtype=torch.int32, device=gb.device)
+    gx = engine.vector(N)
+    bs.solve(g["vals"], g["dvals"], gb, gx, rtol=1e-10, iters=iters)
+    xg, itg = host(gx), host(iters)
+    say(f"C4: oracle, team {team}, 4096 systems")
+    ro = d["row_off"]
+
+    def one(s):
+        r0, r1 = ro[s], ro[s + 1]
+        z0, z1 = d["pntr"][r0], d["pntr"][r1]
+        bb = b[r0:r1]
+        tol = 1e-10 * np.sqrt(oracle.team_dot(bb, bb, team))
+        it, x, _ = oracle.cg_team(d["pntr"][r0:r1 + 1] - z0, d["indx"][z0:z1], d["vals"][z0:z1], d["dvals"][r0:r1],
+                                  bb, tol, team)
+        return s, it, x
+
+    t0 = oracle.get_threads()
+    oracle.set_threads(1)  # parallel over systems instead
+    try:
+        with cf.ThreadPoolExecutor(oracle.host_threads()) as ex:
+            res = list(ex.map(one, range(4096)))
+    finally:
+        oracle.set_threads(t0)
+    for s, it, x in res:
+        assert itg[s] == it, s
+        assert np.array_equal(bits(xg[ro[s]:ro[s + 1]]), bits(x)), s
+
+
+def test_c5_adjoint_gradient_64(engine):
+    import torch
+    from hala_amd.autograd import SparseLinearProblem
+
+    say("C5: adjoint gradient, 64^3 variable-coefficient diffusion")
+    m = 64
+    d = st.diffusion3d(m, seed=5)
+    dt = st.diffusion3d(m, seed=6)
+    f = st.gaussian_source(m)
+    prob = SparseLinearProblem(engine, d["pntr"], d["indx"], d["cptr"], d["cnnz"], d["ccoef"], f, rtol=1e-12)
+    xt, _ = prob.forward_solve(dev(engine, dt["kappa"]))
+    xt = xt.clone()
+    k0 = dev(engine, d["kappa"])
+    x, _ = prob.forward_solve(k0)
+    gx = x - xt
+    g = host(prob.gradient(k0, gx))
+    # the oracle gradient: forward ILU-PCG, adjoint (LU)^-T-PCG on A^T, -lam_i x_j onto the entries and kappa
+    say("C5: oracle gradient")
+    p, i, v = d["pntr"], d["indx"], d["vals"]
+    n = len(p) - 1
+    dg, _ = oracle.diag(p, i)
+    fac = oracle.ilu0(p, i, v, dg)
+    _, xo, _ = oracle.cg(p, i, v, f, tol=prob.tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg, mdata=fac)
+    assert np.array_equal(bits(host(x)), bits(xo))
+    ga = host(gx)
+    tp, ti, tv = oracle.transpose(n, n, p, i, v)
+    _, lam, _ = oracle.cg(tp, ti, tv, ga, tol=1e-12 * float(np.linalg.norm(ga)), prec=oracle.PREC_ILU_T, mpntr=p,
+                          mindx=i, mdiag=dg, mdata=fac)
+    go = oracle.grad_coef(d["cptr"], d["cnnz"], d["ccoef"], oracle.grad_vals(p, i, lam, xo))
+    assert np.array_equal(bits(g), bits(go))
+    # central differences on 32 random conductivities, h = 1e-6
+    say("C5: central differences on 32 conductivities")
+
+    def loss(k):
+        xk, _ = prob.forward_solve(k)
+        return 0.5 * float(torch.sum((xk - xt) ** 2))
+
+    rng = np.random.default_rng(32)
+    h = 1e-6
+    scale = float(np.abs(g).max())
+    worst = 0.0
+    for c in rng.choice(len(d["kappa"]), 32, replace=False):
+        kp, km = k0.clone(), k0.clone()
+        kp[c] += h
+        km[c] -= h
+        fd = (loss(kp) - loss(km)) / (2 * h)
+        worst = max(worst, abs(fd - g[c]) / scale)
+    say(f"C5: worst |fd - grad| / max|grad| = {worst:.3e}")
+    assert worst < 1e-5
