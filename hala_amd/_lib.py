@@ -100,6 +100,7 @@ SIGNATURES = {
     "hala_trsm_z": [vp, ch, ch, i32, f64, f64, vp, vp, i32],
     "hala_trsm_c": [vp, ch, ch, i32, f32, f32, vp, vp, i32],
     "hala_cg_f64": [vp, vp, i32, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
+    "hala_cg_f64_noalloc": [vp, vp, i32, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_cg_t_f64": [vp, vp, vp, vp, vp, f64, i32, pi32, vp, i32],
     "hala_cg_batch_f64": [vp, vp, i32, vp, i32, vp, vp, f64, i32, pi32, vp, i32],
     "hala_gmres_f64": [vp, vp, i32, vp, vp, vp, f64, i32, i32, pi32, vp, i32, pi32],

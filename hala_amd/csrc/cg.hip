@@ -291,8 +291,15 @@ struct cg_problem {
     const double *tvals;  // values permuted into the transposed pattern (adjoint)
 };
 
+// Work space of one CG run: r, p, Ap, z (4n doubles) and two sets of reduction
+// partials (2 ceil(n/256) doubles, sentinel-filled).  Null: the context's buffers.
+struct cg_work {
+    double *vec;
+    double *part;
+};
+
 static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, double tol, int max_iter,
-                   int *iterations, double *hist, int max_hist) {
+                   int *iterations, double *hist, int max_hist, const cg_work *w = nullptr) {
     const int n = P.A->nrows;
     HALA_REQUIRE(P.A->nrows == P.A->ncols, HALA_ERR_ARG, "CG needs a square matrix");
     HALA_REQUIRE(P.prec != PREC_ILU || (P.M && P.M->factored && P.M->n == n), HALA_ERR_ARG,
@@ -300,10 +307,17 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     HALA_REQUIRE(P.prec != PREC_JAC || P.pdiag, HALA_ERR_ARG, "CG: Jacobi preconditioner needs the diagonal");
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
     // work vectors: r, p, Ap, z
-    c->vec.reserve((size_t)4 * n * sizeof(double));
-    double *r = c->vec.as<double>();
+    double *r, *part;
+    if (w) {
+        r = w->vec;
+        part = w->part;
+    } else {
+        c->vec.reserve((size_t)4 * n * sizeof(double));
+        r = c->vec.as<double>();
+        part = partials64(c, (size_t)2 * nb);
+    }
     double *p = r + n, *ap = p + n, *z = ap + n;
-    double *part = partials64(c, (size_t)2 * nb), *part2 = part + nb;
+    double *part2 = part + nb;
     cg_state *st = cg_state_ptr(c);
     const int *pntr = P.transposed ? P.A->tp.tpntr.as<int>() : P.A->pntr;
     const int *indx = P.transposed ? P.A->tp.trow.as<int>() : P.A->indx;
@@ -378,6 +392,28 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     check_wait_error(c);
     if (iterations) *iterations = it;
     HALA_REQUIRE(status == 0, HALA_ERR_NOCONV, "CG: residual norm became non-finite");
+}
+
+__global__ void sentinel_fill(double *__restrict__ x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = sentinel64();
+}
+
+// CG of hala_cg_f64 over a caller-provided work space of cg_noalloc_doubles(n)
+// doubles: r, p, Ap, z, then the reduction partials.  The partials are re-armed
+// (filled with the sentinel by a kernel) before the run; nothing is allocated.
+void cg_noalloc(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag, const double *b,
+                double *x, double tol, int max_iter, int *iterations, double *hist, int max_hist, double *work) {
+    hala_ctx c = A->ctx;
+    const int n = A->nrows;
+    const int64_t nb = ((int64_t)n + CB - 1) / CB;
+    cg_work w{work, work + (size_t)4 * n};
+    if (n > 0) {
+        sentinel_fill<<<grid_for(2 * nb, 256, 1024), 256, 0, c->stream>>>(w.part, 2 * nb);
+        HALA_CHECK_LAUNCH();
+    }
+    cg_problem P{A, vals, prec, ilu, pdiag, false, nullptr};
+    run_cg(c, P, b, x, tol, max_iter, iterations, hist, max_hist, &w);
 }
 
 }  // namespace hala_amd

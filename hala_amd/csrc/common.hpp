@@ -214,6 +214,7 @@ struct hala_ctx_s {
     hala_amd::dbuf vec;      // solver work vectors
     hala_amd::dbuf lbc;      // ticket counters of the triangular-solve pool (zeroed at creation, self re-arming)
     double *host_pinned = nullptr;  // pinned host landing zone for scalars
+    double *noalloc = nullptr;      // this device's scratch of the allocation-free entry point
     // temporaries of the analysis passes (see tmp_scope)
     std::vector<std::unique_ptr<hala_amd::dbuf>> tmp;
     size_t tmp_top = 0;

@@ -65,6 +65,7 @@ double *partials64(hala_ctx c, size_t count) { return partials<double>(c, c->red
 float *partials32(hala_ctx c, size_t count) { return partials<float>(c, c->red32, count); }
 
 int fetch_i32(hala_ctx c, const int *p);
+double *noalloc_scratch(int device);
 
 void check_wait_error_host(hala_ctx c, int flag) {
     if (flag == 0) return;
@@ -153,6 +154,7 @@ int hala_ctx_create(int device, void *stream, hala_ctx *out) {
             partials64(c, 64 * 1024);
             partials32(c, 64 * 1024);
             HALA_HIP(hipHostMalloc((void **)&c->host_pinned, 1024 * sizeof(double), hipHostMallocDefault));
+            c->noalloc = noalloc_scratch(device);  // the device is current
             HALA_HIP(hipStreamSynchronize(c->stream));
         } catch (...) {
             if (c->host_pinned) (void)hipHostFree(c->host_pinned);

@@ -212,6 +212,16 @@ int hala_trsm_c(hala_tri tri, char transa, char transb, int nrhs, float alpha_re
  * hist (device, nullable) receives norm2(r) per pass, at most max_hist. */
 int hala_cg_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag, const double *b,
                 double *x, double tol, int max_iter, int *iterations, double *hist, int max_hist);
+/* hala_cg_f64 without any device allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as hala_cg_f64; the work vectors and reduction partials live in a
+ * static device array of 64 MiB per GPU, so A may have at most 2^21 rows
+ * (larger: HALA_ERR_ARG).  Calls that overlap on one GPU share that array and
+ * race; the preconditioner (ilu) is built beforehand, as for hala_cg_f64. */
+int hala_cg_f64_noalloc(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag,
+                        const double *b, double *x, double tol, int max_iter, int *iterations, double *hist,
+                        int max_hist);
 /* Multi-RHS PCG with one shared matrix, resident on the device: the
    solve_batch_cg / solve_batch_cg_ilu of hex/solvers/hala_solvers_cg_batch.hpp:68-249
    (solve_cg_core with the batch operations of wax/hala_blas_extensions.hpp).
