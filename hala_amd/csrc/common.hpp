@@ -196,6 +196,8 @@ struct hala_options {
     int tri_col_split = 0;     // columns per slab of planes (0: automatic, 1: contiguous row blocks)
     int cg_fuse = 1;           // ILU-PCG: the sweeps' gather/scatter fused into the update and r.z kernels
     int ilu32_jacobi = 0;      // FP32 ILU applies between FP64 vectors: k > 0 Jacobi sweeps per triangle (0: exact)
+    int ilu64_jacobi = 0;      // FP64 ILU applies inside GMRES and hala_ilu_apply_f64: k > 0 Jacobi sweeps per
+                               // triangle (0: exact); CG keeps the exact sweeps
     int sync_checks = 1;       // standalone sweeps (ILU apply, trsv) wait for the device and report a wait that
                                // gave up before returning; 0: they return at once and such an error is reported
                                // by the next call that synchronises the context (hala_ctx_sync included)

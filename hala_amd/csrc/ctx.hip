@@ -216,6 +216,7 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "cg_batch_max") return &c->opt.cg_batch_max;
     if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
     if (k == "ilu32_jacobi") return &c->opt.ilu32_jacobi;
+    if (k == "ilu64_jacobi") return &c->opt.ilu64_jacobi;
     if (k == "sync_checks") return &c->opt.sync_checks;
     return nullptr;
 }

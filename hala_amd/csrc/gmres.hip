@@ -382,6 +382,7 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
             gm_residual_kernel<double><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals64, x, b, t, st, x_zero);
         if (!P.M) gm_copy_kernel<<<gp, 256, 0, c->stream>>>(n, t, r, st, 0);
         else if (P.vals32) ilu_apply32x64(P.M, t, r, &st->all_done);
+        else if (c->opt.ilu64_jacobi > 0) ilu_apply64_jacobi(P.M, t, r, &st->all_done, c->opt.ilu64_jacobi);
         else ilu_apply64(P.M, t, r, &st->all_done);
         gm_cycle_norm_kernel<<<ge, 256, 0, c->stream>>>(n, r, part, wait_error_flag(c), st, ga);
         gm_scale_first_kernel<<<gp, 256, 0, c->stream>>>(n, r, W, st);
@@ -397,6 +398,7 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
             }
             if (!P.M) gm_copy_kernel<<<gp, 256, 0, c->stream>>>(n, t, r, st, 1);
             else if (P.vals32) ilu_apply32x64(P.M, t, r, &st->skip_inner);
+            else if (c->opt.ilu64_jacobi > 0) ilu_apply64_jacobi(P.M, t, r, &st->skip_inner, c->opt.ilu64_jacobi);
             else ilu_apply64(P.M, t, r, &st->skip_inner);
             gm_mdot_kernel<<<dim3(ge, restart), 256, 0, c->stream>>>(n, W, r, part, wait_error_flag(c), st,
                                                                       ga);

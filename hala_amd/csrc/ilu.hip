@@ -376,6 +376,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         M->factored = false;
         M->adjoint_loaded = false;
         M->jac_values = false;
+        M->jac_values64 = false;
         HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
                      "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
         load_values64(c, M->L, f);
@@ -394,7 +395,9 @@ int hala_ilu_apply_f64(hala_ilu M, const double *x, double *r) {
     return guard([&] {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f64: null argument");
         set_device(M->ctx);
-        ilu_apply64(M, x, r);
+        HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
+        if (M->ctx->opt.ilu64_jacobi > 0) ilu_apply64_jacobi(M, x, r, nullptr, M->ctx->opt.ilu64_jacobi);
+        else ilu_apply64(M, x, r);
         if (M->ctx->opt.sync_checks) check_wait_error(M->ctx);
     });
 }

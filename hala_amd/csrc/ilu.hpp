@@ -19,10 +19,12 @@ struct hala_ilu_s {
     hala_amd::csr_transpose tp;
     hala_amd::dbuf tmp64, tmp32;  // natural-order hand-off between the two solves
     bool factored = false, has32 = false, adjoint = false, adjoint_loaded = false;
-    // Jacobi-applied FP32 factor (ilujac.hip): packed strict-L / strict-U parts
-    // with FP32 values, FP32 diagonal, four FP32 iterates (built on first use)
+    // Jacobi-applied factor (ilujac.hip): packed strict-L / strict-U parts with
+    // FP32 (and, for FP64 applies, FP64) values and diagonal, four iterates of
+    // each precision (built on first use)
     hala_amd::dbuf jl_ptr, jl_idx, jl_val, ju_ptr, ju_idx, ju_val, jd_val, jwork;
-    bool jac_pattern = false, jac_values = false;
+    hala_amd::dbuf jl_val64, ju_val64, jd_val64, jwork64;
+    bool jac_pattern = false, jac_values = false, jac_values64 = false;
     // complex factors (cplx.hip): interleaved real/imaginary, CSR order
     hala_amd::dbuf zfactor, cfactor;
     bool zfactored = false, cfactored = false;
@@ -60,5 +62,6 @@ void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate = nul
 void ilu_prepare_adjoint(hala_ilu M);
 // r = (LU)^-1 x with the FP32 factor applied by k Jacobi sweeps per triangle (ilujac.hip)
 void ilu_apply32x64_jacobi(hala_ilu M, const double *x, double *r, const int *gate, int k);
-void ilu_jacobi_load(hala_ilu M);
+// r = (LU)^-1 x with the FP64 factor applied by k Jacobi sweeps per triangle (ilujac.hip)
+void ilu_apply64_jacobi(hala_ilu M, const double *x, double *r, const int *gate, int k);
 }  // namespace hala_amd

@@ -41,6 +41,7 @@ _SIG = {
     "hala_or_ilu_apply": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_ilu_apply_f32": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_ilu_apply_f32_jacobi": (None, [i32, vp, vp, vp, vp, i32, vp, vp]),
+    "hala_or_ilu_apply_jacobi": (None, [i32, vp, vp, vp, vp, i32, vp, vp]),
     "hala_or_ilu_apply_t": (None, [i32, vp, vp, vp, vp, vp]),
     "hala_or_trsv": (None, [ch, ch, ch, i32, f64, vp, vp, vp, vp, vp]),
     "hala_or_trsv_f32": (None, [ch, ch, ch, i32, f32, vp, vp, vp, vp, vp]),
@@ -287,6 +288,15 @@ def ilu_apply_f32_jacobi(pntr, indx, dg, ilu32, x32, k):
     return x
 
 
+def ilu_apply_jacobi(pntr, indx, dg, ilu, x, k):
+    """(LU)^-1 x with the FP64 factor applied by k Jacobi sweeps per triangle (hala_or_ilu_apply_jacobi)."""
+    x = c64(x).copy()
+    w = np.empty(3 * x.size, np.float64)
+    lib().hala_or_ilu_apply_jacobi(len(pntr) - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)), P(c64(ilu)), int(k),
+                                   P(x), P(w))
+    return x
+
+
 def ilu_apply_t(pntr, indx, dg, ilu, x):
     x = c64(x).copy()
     lib().hala_or_ilu_apply_t(len(pntr) - 1, P(ci32(pntr)), P(ci32(indx)), P(ci32(dg)), P(c64(ilu)), P(x))
@@ -377,7 +387,7 @@ def gmres(pntr, indx, vals, b, x0=None, tol=0.0, restart=30, max_outer=2**31 - 1
 def gmres_ir(pntr, indx, vals, dg, ilu64, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=50, fp32=True,
              jacobi=0):
     """Iterative refinement around ILU(0)-GMRES; fp32 selects the FP32 product and ILU inside;
-    jacobi > 0 applies that FP32 ILU by Jacobi sweeps."""
+    jacobi > 0 applies the inner ILU (FP32 or FP64) by Jacobi sweeps."""
     n = len(pntr) - 1
     x = np.zeros(n)
     hist = np.zeros(max_refine + 2)

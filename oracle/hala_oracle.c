@@ -345,50 +345,52 @@ void hala_or_ilu_apply_f32(int n, const int *pntr, const int *indx, const int *d
 }
 
 /*
- * x <- (LU)^-1 x with the FP32 factor applied by k >= 1 Jacobi sweeps per
- * triangle: a method of this build for the FP32 inner solve of iterative
- * refinement, not a HALA operation.  L (unit): y0 = x,
+ * x <- (LU)^-1 x with the factor applied by k >= 1 Jacobi sweeps per
+ * triangle, in FP32 or FP64: a method of this build for the inner solve of
+ * iterative refinement, not a HALA operation.  L (unit): y0 = x,
  *   y(m+1)_i = x_i - l_ij y(m)_j - ...   (j < i, one term at a time in row order);
  * U: z0_i = y_i / u_ii,
  *   z(m+1)_i = (y_i - u_ij z(m)_j - ...) / u_ii   (j > i, row order),
  * with y = y(k).  Every row applies its terms in the exact sweep's order, so
  * with k at least the level count of a triangle minus one the result equals
- * hala_or_ilu_apply_f32 bit for bit.  w: 3n floats of scratch.
+ * the exact apply of the same precision bit for bit.  w: 3n values of scratch.
  */
-void hala_or_ilu_apply_f32_jacobi(int n, const int *pntr, const int *indx, const int *diag, const float *ilu, int k,
-                                  float *x, float *w) {
-    float *ya = w, *yb = w + n, *zb = w + 2 * (size_t)n;
-    const float *yo = x; /* y0 = x */
-    float *yn = ya;
-    for (int m = 0; m < k; m++) {
-#pragma omp parallel for schedule(static) OR_PAR(n)
-        for (int i = 0; i < n; i++) {
-            float acc = x[i];
-            for (int j = pntr[i]; j < diag[i]; j++) acc = acc - ilu[j] * yo[indx[j]];
-            yn[i] = acc;
-        }
-        yo = yn;
-        yn = (yn == ya) ? yb : ya;
+#define OR_ILU_JACOBI(NAME, R)                                                                                \
+    void NAME(int n, const int *pntr, const int *indx, const int *diag, const R *ilu, int k, R *x, R *w) {      \
+        R *ya = w, *yb = w + n, *zb = w + 2 * (size_t)n;                                                    \
+        const R *yo = x; /* y0 = x */                                                                       \
+        R *yn = ya;                                                                                         \
+        for (int m = 0; m < k; m++) {                                                                       \
+            _Pragma("omp parallel for schedule(static) if (or_threads > 1 && n > 4096) num_threads(or_threads)") \
+            for (int i = 0; i < n; i++) {                                                                   \
+                R acc = x[i];                                                                               \
+                for (int j = pntr[i]; j < diag[i]; j++) acc = acc - ilu[j] * yo[indx[j]];                   \
+                yn[i] = acc;                                                                                \
+            }                                                                                               \
+            yo = yn;                                                                                        \
+            yn = (yn == ya) ? yb : ya;                                                                      \
+        }                                                                                                   \
+        /* y(k) is in yo; z0 into the free y buffer, then alternate it with zb */                          \
+        const R *y = yo;                                                                                    \
+        R *zo = (yo == ya) ? yb : ya;                                                                       \
+        _Pragma("omp parallel for schedule(static) if (or_threads > 1 && n > 4096) num_threads(or_threads)")     \
+        for (int i = 0; i < n; i++) zo[i] = y[i] / ilu[diag[i]];                                            \
+        R *zn = zb;                                                                                         \
+        for (int m = 0; m < k; m++) {                                                                       \
+            _Pragma("omp parallel for schedule(static) if (or_threads > 1 && n > 4096) num_threads(or_threads)") \
+            for (int i = 0; i < n; i++) {                                                                   \
+                R acc = y[i];                                                                               \
+                for (int j = diag[i] + 1; j < pntr[i + 1]; j++) acc = acc - ilu[j] * zo[indx[j]];           \
+                zn[i] = acc / ilu[diag[i]];                                                                 \
+            }                                                                                               \
+            R *t = zo;                                                                                      \
+            zo = zn;                                                                                        \
+            zn = t;                                                                                         \
+        }                                                                                                   \
+        memcpy(x, zo, (size_t)n * sizeof(R));                                                               \
     }
-    /* y(k) is in yo; z0 into x (x no longer needed), then alternate x / zb */
-    const float *y = yo;
-    float *zo = (yo == ya) ? yb : ya; /* the free y buffer holds z0 */
-#pragma omp parallel for schedule(static) OR_PAR(n)
-    for (int i = 0; i < n; i++) zo[i] = y[i] / ilu[diag[i]];
-    float *zn = zb;
-    for (int m = 0; m < k; m++) {
-#pragma omp parallel for schedule(static) OR_PAR(n)
-        for (int i = 0; i < n; i++) {
-            float acc = y[i];
-            for (int j = diag[i] + 1; j < pntr[i + 1]; j++) acc = acc - ilu[j] * zo[indx[j]];
-            zn[i] = acc / ilu[diag[i]];
-        }
-        float *t = zo;
-        zo = zn;
-        zn = t;
-    }
-    memcpy(x, zo, (size_t)n * sizeof(float));
-}
+OR_ILU_JACOBI(hala_or_ilu_apply_f32_jacobi, float)
+OR_ILU_JACOBI(hala_or_ilu_apply_jacobi, double)
 
 /*
  * x <- (LU)^-T x = L^-T U^-T x (the adjoint preconditioner; HALA has no
@@ -685,6 +687,7 @@ typedef struct {
     float *w32a, *w32b;
     int jacobi;  /* FP32 ILU: k > 0 Jacobi sweeps per triangle instead of the exact sweeps */
     float *wj;   /* 3n floats for the Jacobi apply */
+    double *wj64; /* 3n doubles for the FP64 Jacobi apply */
     sched_t *sched; /* level schedule of the ILU sweeps (threaded runs), or null */
 } gm_op_t;
 
@@ -708,7 +711,8 @@ static void gm_prec(gm_op_t *o, const double *t, double *r) {
         vcopy(n, t, r);
     } else if (!o->ilu32) {
         vcopy(n, t, r);
-        if (o->sched) ilu_apply_sched(o->sched, o->pntr, o->indx, o->diag, o->ilu, r);
+        if (o->jacobi > 0) hala_or_ilu_apply_jacobi(n, o->pntr, o->indx, o->diag, o->ilu, o->jacobi, r, o->wj64);
+        else if (o->sched) ilu_apply_sched(o->sched, o->pntr, o->indx, o->diag, o->ilu, r);
         else hala_or_ilu_apply(n, o->pntr, o->indx, o->diag, o->ilu, r);
     } else {
 #pragma omp parallel for schedule(static) OR_PAR(n)
@@ -821,7 +825,7 @@ static int gmres_core(gm_op_t *o, const double *b, double *x, double tol, double
 int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, int prec, const int *diag,
                   const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
                   double *hist, int max_hist, int *nhist) {
-    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0, 0, 0, 0};
+    gm_op_t o = {n, pntr, indx, diag, vals, 0, ilu, 0, prec, 0, 0, 0, 0, 0, 0};
     if (nhist) *nhist = 0;
     if (prec == HALA_OR_PREC_ILU) o.sched = sched_build(n, pntr, indx, diag);
     int total = gmres_core(&o, b, x, tol, 0.0, 0, restart, max_outer, hist, max_hist, nhist);
@@ -848,10 +852,11 @@ int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals
     float *w32a = (float *)malloc((size_t)n * sizeof(float));
     float *w32b = (float *)malloc((size_t)n * sizeof(float));
     float *wj = (float *)malloc((size_t)3 * n * sizeof(float));
+    double *wj64 = (double *)malloc((size_t)3 * n * sizeof(double));
     double *r = (double *)malloc((size_t)n * sizeof(double));
     double *d = (double *)malloc((size_t)n * sizeof(double));
     gm_op_t o = {n, pntr, indx, diag, vals, vals32, ilu, vals32 ? ilu32 : 0, HALA_OR_PREC_ILU, w32a, w32b,
-                 vals32 ? jacobi : 0, wj, sched_build(n, pntr, indx, diag)};
+                 jacobi, wj, wj64, sched_build(n, pntr, indx, diag)};
     double bn = hala_or_cnrm2(n, b);
     int total = 0, nh = 0;
     for (int it = 0; it <= max_refine; it++) {
@@ -870,6 +875,7 @@ int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals
     free(w32a);
     free(w32b);
     free(wj);
+    free(wj64);
     free(r);
     free(d);
     return total;

@@ -82,6 +82,8 @@ void hala_or_ilu_apply(int n, const int *pntr, const int *indx, const int *diag,
 void hala_or_ilu_apply_f32(int n, const int *pntr, const int *indx, const int *diag, const float *ilu, float *x);
 void hala_or_ilu_apply_f32_jacobi(int n, const int *pntr, const int *indx, const int *diag, const float *ilu, int k,
                                   float *x, float *w);
+void hala_or_ilu_apply_jacobi(int n, const int *pntr, const int *indx, const int *diag, const double *ilu, int k,
+                              double *x, double *w);
 void hala_or_ilu_apply_t(int n, const int *pntr, const int *indx, const int *diag, const double *ilu, double *x);
 
 /* ---- standalone triangular solve ---------------------------------------- */
@@ -110,7 +112,7 @@ int hala_or_gmres(int n, const int *pntr, const int *indx, const double *vals, i
                   const double *ilu, const double *b, double *x, double tol, int restart, int max_outer,
                   double *hist, int max_hist, int *nhist);
 /* vals32 != NULL: FP32 product and FP32 ILU (ilu32) inside; NULL: FP64 product and ILU (ilu) inside.
-   jacobi > 0 (FP32 only): the inner ILU is applied by that many Jacobi sweeps per triangle. */
+   jacobi > 0: the inner ILU (FP32 or FP64) is applied by that many Jacobi sweeps per triangle. */
 int hala_or_gmres_ir(int n, const int *pntr, const int *indx, const double *vals, const float *vals32,
                      const int *diag, const double *ilu, const float *ilu32, const double *b, double *x,
                      double rtol, double inner_rtol, int restart, int max_refine, double *hist, int max_hist,

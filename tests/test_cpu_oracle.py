@@ -381,39 +381,50 @@ def _lu_dense(p, i, f, n):
     return np.tril(a, -1), np.triu(a, 1), np.diag(a).copy()
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("case", ["lap2d_12x9", "cd27_8"])
-def test_jacobi_ilu_apply_pins(case):
-    """FP32 ILU applied by k Jacobi sweeps per triangle: equals the exact FP32 apply once k reaches
-    the level depth (same per-row term order); one sweep matches the dense algebra
-    y = x - (L-I)x, z = (y - (U-D) y/d) / d; the error falls with every sweep."""
+def test_jacobi_ilu_apply_pins(case, prec):
+    """ILU applied by k Jacobi sweeps per triangle (FP32 or FP64 factor): equals the exact apply
+    of the same precision once k reaches the level depth (same per-row term order); one sweep
+    matches the dense algebra y = x - (L-I)x, z = (y - (U-D) y/d) / d; the error falls with
+    every sweep."""
     p, i, v = st.laplace2d(12, 9) if case == "lap2d_12x9" else st.convdiff27(8, seed=3)
     n = len(p) - 1
     d, _ = oracle.diag(p, i)
     f = oracle.ilu0(p, i, v, d)
-    f32 = f.astype(np.float32)
-    x = st.uniform(9, n).astype(np.float32)
-    exact = oracle.ilu_apply_f32(p, i, d, f32, x)
+    if prec == "f32":
+        fp = f.astype(np.float32)
+        x = st.uniform(9, n).astype(np.float32)
+        exact = oracle.ilu_apply_f32(p, i, d, fp, x)
+        jac = lambda k: oracle.ilu_apply_f32_jacobi(p, i, d, fp, x, k)  # noqa: E731
+        rtol = 1e-5
+    else:
+        fp = f
+        x = st.uniform(9, n)
+        exact = oracle.ilu_apply(p, i, d, f, x)
+        jac = lambda k: oracle.ilu_apply_jacobi(p, i, d, f, x, k)  # noqa: E731
+        rtol = 1e-13
     nl = max(len(oracle.levels(p, i, d, up)[1]) - 1 for up in (False, True))
-    assert np.array_equal(oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, nl).view(np.int32), exact.view(np.int32))
-    lo, up, dg = _lu_dense(p, i, f32.astype(np.float64), n)
+    assert np.array_equal(bits(jac(nl)), bits(exact))
+    lo, up, dg = _lu_dense(p, i, fp.astype(np.float64), n)
     x64 = x.astype(np.float64)
     y = x64 - lo @ x64
     z = (y - up @ (y / dg)) / dg
-    one = oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, 1).astype(np.float64)
-    assert np.linalg.norm(one - z) <= 1e-5 * np.linalg.norm(z)
-    errs = [np.linalg.norm(oracle.ilu_apply_f32_jacobi(p, i, d, f32, x, k).astype(np.float64) - exact)
-            for k in range(1, 7)]
+    one = jac(1).astype(np.float64)
+    assert np.linalg.norm(one - z) <= rtol * np.linalg.norm(z)
+    errs = [np.linalg.norm(jac(k).astype(np.float64) - exact.astype(np.float64)) for k in range(1, 7)]
     assert all(b < a for a, b in zip(errs, errs[1:])), errs
 
 
-def test_refinement_with_jacobi_inner_reaches_fp64_accuracy():
+@pytest.mark.parametrize("fp32", [True, False])
+def test_refinement_with_jacobi_inner_reaches_fp64_accuracy(fp32):
     p, i, v = st.convdiff27(10, seed=3)
     n = len(p) - 1
     b = st.uniform(8, n)
     d, _ = oracle.diag(p, i)
     f = oracle.ilu0(p, i, v, d)
-    tot_exact, _, _ = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=True)
-    tot, x, hist = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=True, jacobi=3)
+    tot_exact, _, _ = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=fp32)
+    tot, x, hist = oracle.gmres_ir(p, i, v, d, f, b, rtol=1e-10, fp32=fp32, jacobi=3)
     assert hist[-1] < 1e-10
     assert tot >= tot_exact  # an inexact preconditioner needs at least as many inner steps
     xs = np.linalg.solve(dense(p, i, v, n), b)
@@ -537,7 +548,7 @@ def test_threaded_oracle_is_bit_identical():
                               mdiag=d, mdata=f)[1:])
         fq = oracle.ilu0(q, j, w, dq)
         out += list(oracle.gmres(q, j, w, c, tol=1e-11, restart=20, dg=dq, ilu=fq)[1:])
-        for kw in ({"fp32": True}, {"fp32": True, "jacobi": 3}, {"fp32": False}):
+        for kw in ({"fp32": True}, {"fp32": True, "jacobi": 3}, {"fp32": False}, {"fp32": False, "jacobi": 3}):
             out += list(oracle.gmres_ir(q, j, w, dq, fq, c, **kw)[1:])
         return out
 

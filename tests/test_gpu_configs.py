@@ -111,7 +111,7 @@ def c3():
     return p, i, v, b, dg, oracle.ilu0(p, i, v, dg)
 
 
-@pytest.mark.parametrize("variant", ["fp32_inner", "fp32_inner_jacobi3", "fp64_inner"])
+@pytest.mark.parametrize("variant", ["fp32_inner", "fp32_inner_jacobi3", "fp64_inner", "fp64_inner_jacobi3"])
 def test_c3_gmres_ir_convdiff27_96(engine, c3, variant):
     import torch
 
@@ -127,13 +127,14 @@ def test_c3_gmres_ir_convdiff27_96(engine, c3, variant):
     ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
     gx = engine.vector(n)
     gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
-    saved = engine.get_option("ilu32_jacobi")
+    opt = "ilu32_jacobi" if fp32 else "ilu64_jacobi"
+    saved = engine.get_option(opt)
     try:
-        engine.set_option("ilu32_jacobi", k)
+        engine.set_option(opt, k)
         gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv.float() if fp32 else None, ilu, gb, gx, rtol=1e-10,
                                      inner_rtol=1e-4, restart=30, max_refine=40, hist=gh)
     finally:
-        engine.set_option("ilu32_jacobi", saved)
+        engine.set_option(opt, saved)
     assert gtot == tot and gnh == len(hist)
     assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
     assert np.array_equal(bits(host(gx)), bits(x))

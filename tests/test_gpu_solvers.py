@@ -115,36 +115,44 @@ def test_gmres_ir_bitexact(engine, fp32):
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("k", [1, 2, 3, -1], ids=["k1", "k2", "k3", "k_levels"])
-def test_ilu_jacobi_apply_bitexact(engine, k):
-    """FP32 ILU applied by Jacobi sweeps (option ilu32_jacobi) matches the oracle bit for bit; with as
-    many sweeps as levels it is the exact FP32 apply."""
-    saved = engine.get_option("ilu32_jacobi")
+def test_ilu_jacobi_apply_bitexact(engine, k, prec):
+    """ILU applied by Jacobi sweeps (options ilu32_jacobi / ilu64_jacobi) matches the oracle bit for
+    bit; with as many sweeps as levels it is the exact apply of that precision."""
+    opt = "ilu32_jacobi" if prec == "f32" else "ilu64_jacobi"
+    saved = engine.get_option(opt)
     try:
         for gen in (lambda: st.laplace3d(24, 24, 24), lambda: st.convdiff27(16, seed=3)):
             p, i, v = gen()
             n = len(p) - 1
             dg, _ = oracle.diag(p, i)
-            f32 = oracle.ilu0(p, i, v, dg).astype(np.float32)
+            f = oracle.ilu0(p, i, v, dg)
             kk = k if k > 0 else max(len(oracle.levels(p, i, dg, up)[1]) - 1 for up in (False, True))
             M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
             x = st.uniform(61, n)
             gr = engine.vector(n)
-            engine.set_option("ilu32_jacobi", kk)
+            engine.set_option(opt, kk)
             for _ in range(2):
-                M.apply_f32x64(dev(engine, x), gr)
-                ref = oracle.ilu_apply_f32_jacobi(p, i, dg, f32, x.astype(np.float32), kk).astype(np.float64)
+                if prec == "f32":
+                    M.apply_f32x64(dev(engine, x), gr)
+                    ref = oracle.ilu_apply_f32_jacobi(p, i, dg, f.astype(np.float32), x.astype(np.float32),
+                                                      kk).astype(np.float64)
+                else:
+                    M.apply(dev(engine, x), gr)
+                    ref = oracle.ilu_apply_jacobi(p, i, dg, f, x, kk)
                 assert np.array_equal(bits(host(gr)), bits(ref))
             if k < 0:
-                engine.set_option("ilu32_jacobi", 0)
+                engine.set_option(opt, 0)
                 g2 = engine.vector(n)
-                M.apply_f32x64(dev(engine, x), g2)
+                (M.apply_f32x64 if prec == "f32" else M.apply)(dev(engine, x), g2)
                 assert np.array_equal(bits(host(g2)), bits(host(gr)))
     finally:
-        engine.set_option("ilu32_jacobi", saved)
+        engine.set_option(opt, saved)
 
 
-def test_gmres_ir_jacobi_inner_bitexact(engine):
+@pytest.mark.parametrize("fp32", [True, False], ids=["fp32_inner", "fp64_inner"])
+def test_gmres_ir_jacobi_inner_bitexact(engine, fp32):
     import torch
 
     p, i, v = st.convdiff27(16, seed=3)
@@ -153,19 +161,20 @@ def test_gmres_ir_jacobi_inner_bitexact(engine):
     dg, _ = oracle.diag(p, i)
     f = oracle.ilu0(p, i, v, dg)
     tot, x, hist = oracle.gmres_ir(p, i, v, dg, f, b, rtol=1e-10, inner_rtol=1e-4, restart=30, max_refine=40,
-                                   fp32=True, jacobi=3)
-    saved = engine.get_option("ilu32_jacobi")
+                                   fp32=fp32, jacobi=3)
+    opt = "ilu32_jacobi" if fp32 else "ilu64_jacobi"
+    saved = engine.get_option(opt)
     try:
-        engine.set_option("ilu32_jacobi", 3)
+        engine.set_option(opt, 3)
         gp, gi, gv = upload(engine, p, i, v)
         ilu = H.make_ilu(engine, gp, gi, gv, want_f32=True)
         A = H.make_sparse_matrix(engine, n, gp, gi, gv)
         gx = engine.vector(n)
         gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
-        gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv.float(), ilu, dev(engine, b), gx, rtol=1e-10,
+        gtot, gnh = H.solve_gmres_ir(engine, A, gv, gv.float() if fp32 else None, ilu, dev(engine, b), gx, rtol=1e-10,
                                      inner_rtol=1e-4, restart=30, max_refine=40, hist=gh)
     finally:
-        engine.set_option("ilu32_jacobi", saved)
+        engine.set_option(opt, saved)
     assert gtot == tot and gnh == len(hist)
     assert np.array_equal(bits(host(gh)[:gnh]), bits(hist))
     assert np.array_equal(bits(host(gx)), bits(x))
