@@ -196,6 +196,12 @@ class gpu_ilu:
         return {"column": bool(o[0]), "ncol": o[1], "cells": o[2], "window": o[3], "max_ext": o[4], "why": o[5],
                 "split": o[6], "plane": o[7]}
 
+    def stencil_info(self):
+        """Pencil schedule of a stencil pattern: kind (0 none, 1 5-pt, 2 9-pt, 3 7-pt, 4 27-pt), grid, in use."""
+        o = (C.c_int * 5)()
+        call("hala_ilu_stencil_info", self.handle, o)
+        return {"kind": o[0], "nx": o[1], "ny": o[2], "nz": o[3], "in_use": bool(o[4])}
+
     def apply(self, x, r, num_rhs=1):
         assert num_rhs == 1
         if _dt(x) == 64:

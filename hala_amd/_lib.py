@@ -78,6 +78,7 @@ SIGNATURES = {
     "hala_ilu_destroy": [vp],
     "hala_ilu_info": [vp, C.POINTER(IluInfo)],
     "hala_ilu_sweep_info": [vp, i32, pi32],
+    "hala_ilu_stencil_info": [vp, pi32],
     "hala_ilu_factor_f64": [vp, vp, i32],
     "hala_ilu_apply_f64": [vp, vp, vp],
     "hala_ilu_apply_f32": [vp, vp, vp],

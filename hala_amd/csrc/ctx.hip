@@ -218,6 +218,7 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "ilu32_jacobi") return &c->opt.ilu32_jacobi;
     if (k == "ilu64_jacobi") return &c->opt.ilu64_jacobi;
     if (k == "sync_checks") return &c->opt.sync_checks;
+    if (k == "tri_stencil") return &c->opt.tri_stencil;
     return nullptr;
 }
 

@@ -152,8 +152,11 @@ __global__ void pivot_check(int n, const int *__restrict__ diag, const double *_
 
 __global__ void set_int(int *p, int v) { *p = v; }
 
+// the pencil schedule applies (a stencil pattern, and the option allows it)
+static bool use_stencil(hala_ilu M) { return M->sp && M->ctx->opt.tri_stencil; }
+
 bool ilu_fused64(hala_ilu M) {
-    return M->factored && col_usable(M->ctx, M->L, true) && col_usable(M->ctx, M->U, true);
+    return M->factored && !use_stencil(M) && col_usable(M->ctx, M->L, true) && col_usable(M->ctx, M->U, true);
 }
 
 void ilu_apply64_mid(hala_ilu M, const int *gate) {
@@ -166,6 +169,10 @@ void ilu_apply64_mid(hala_ilu M, const int *gate) {
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
     hala_ctx c = M->ctx;
     HALA_REQUIRE(M->factored, HALA_ERR_ARG, "ILU apply before factorisation");
+    if (use_stencil(M)) {  // natural order in and out, no hand-off
+        stencil_apply<double, double, double>(M, x, r, gate);
+        return;
+    }
     if (ilu_fused64(M)) {  // L's output handed to U in slot order
         tri_col_exec<double, double>(c, M->L, TRI_SUB, 1.0, x, nullptr, gate);
         col_handoff64(c, M->U, M->L, gate);
@@ -182,6 +189,10 @@ void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate) {
     HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs hala_ilu_factor_f64(..., want_f32=1)");
     if (c->opt.ilu32_jacobi > 0) {
         ilu_apply32x64_jacobi(M, x, r, gate, c->opt.ilu32_jacobi);
+        return;
+    }
+    if (use_stencil(M)) {
+        stencil_apply<float, double, double>(M, x, r, gate);
         return;
     }
     tri_exec<float, double, double>(c, M->L, TRI_SUB, 1.0f, x, M->tmp64.as<double>(), gate);
@@ -305,6 +316,8 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
             if (uerr) std::rethrow_exception(uerr);
             M->factor.reserve((size_t)nnz * sizeof(double));
             M->tmp64.reserve((size_t)n * sizeof(double));
+            // a stencil pattern also gets the pencil schedule of its sweeps
+            if (!stencil_build(M)) M->sp.reset();
         } catch (...) {
             delete M;
             throw;
@@ -333,6 +346,18 @@ int hala_ilu_info(hala_ilu M, hala_ilu_info_t *info) {
         info->nchunk_upper = M->U.nchunk;
         info->slots_lower = M->L.nent;
         info->slots_upper = M->U.nent;
+    });
+}
+
+int hala_ilu_stencil_info(hala_ilu M, int *out) {
+    return guard([&] {
+        HALA_REQUIRE(M && out, HALA_ERR_ARG, "hala_ilu_stencil_info: null argument");
+        const stencil_plan *sp = M->sp.get();
+        out[0] = sp ? sp->kind : 0;
+        out[1] = sp ? sp->nx : 0;
+        out[2] = sp ? sp->ny : 0;
+        out[3] = sp ? sp->nz : 0;
+        out[4] = use_stencil(M) ? 1 : 0;
     });
 }
 
@@ -381,6 +406,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
                      "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
         load_values64(c, M->L, f);
         load_values64(c, M->U, f);
+        if (M->sp) stencil_load(M, want_f32);
         if (want_f32) {
             load_values32(c, M->L, f);
             load_values32(c, M->U, f);
@@ -408,9 +434,13 @@ int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
         HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs want_f32 at factorisation");
         hala_ctx c = M->ctx;
         set_device(c);
-        M->tmp32.reserve((size_t)M->n * sizeof(float));
-        tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>());
-        tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r);
+        if (use_stencil(M)) {
+            stencil_apply<float, float, float>(M, x, r, nullptr);
+        } else {
+            M->tmp32.reserve((size_t)M->n * sizeof(float));
+            tri_exec<float, float, float>(c, M->L, TRI_SUB, 1.0f, x, M->tmp32.as<float>());
+            tri_exec<float, float, float>(c, M->U, TRI_SUB, 1.0f, M->tmp32.as<float>(), r);
+        }
         if (c->opt.sync_checks) check_wait_error(c);
     });
 }

@@ -1,8 +1,10 @@
 // ilu.hpp -- ILU(0) handle (the gpu_ilu of gpu/hala_gpu_ilu.hpp:45-195).
 #pragma once
 #include <functional>
+#include <memory>
 #include "spmv.hpp"
 #include "trsolve.hpp"
+#include "trstencil.hpp"
 
 struct hala_ilu_s {
     hala_ctx ctx = nullptr;
@@ -27,6 +29,8 @@ struct hala_ilu_s {
     bool jac_pattern = false, jac_values = false, jac_values64 = false;
     // complex factors (cplx.hip): interleaved real/imaginary, CSR order
     hala_amd::dbuf zfactor, cfactor;
+    // pencil schedule of the two sweeps when the pattern is a stencil (trstencil.hip), else null
+    std::unique_ptr<hala_amd::stencil_plan> sp;
     bool zfactored = false, cfactored = false;
 };
 
