@@ -198,7 +198,7 @@ struct hala_options {
     int ilu32_jacobi = 0;      // FP32 ILU applies between FP64 vectors: k > 0 Jacobi sweeps per triangle (0: exact)
     int ilu64_jacobi = 0;      // FP64 ILU applies inside GMRES and hala_ilu_apply_f64: k > 0 Jacobi sweeps per
                                // triangle (0: exact); CG keeps the exact sweeps
-    int tri_stencil = 1;       // ILU(0) sweeps of a stencil pattern on the pencil schedule (0: general sweeps)
+    int tri_stencil = 0;       // 1: ILU(0) sweeps of a stencil pattern on the pencil schedule (0: general sweeps)
     int sync_checks = 1;       // standalone sweeps (ILU apply, trsv) wait for the device and report a wait that
                                // gave up before returning; 0: they return at once and such an error is reported
                                // by the next call that synchronises the context (hala_ctx_sync included)

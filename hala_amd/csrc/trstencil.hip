@@ -179,13 +179,29 @@ __global__ void stencil_pack(pack_args a) {
     if (a.up) a.dpos[prow] = valid ? a.diag[r] : -1;
 }
 
+// One thread per (step record g, lane): the factor's values of packed row
+// g * 64 + lane into the record (coefficients, diagonal for U, mask).
 template <class V>
-__global__ void stencil_values(int64_t count, const int *__restrict__ pos, const double *__restrict__ f,
-                               V *__restrict__ out, V absent) {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; e < count; e += (int64_t)gridDim.x * blockDim.x) {
-        const int p = pos[e];
-        out[e] = p >= 0 ? (V)f[p] : absent;
+__global__ void stencil_records(int64_t nrec, int NS, int up, int rec_bytes, const int *__restrict__ pos,
+                                const int *__restrict__ dpos, const unsigned short *__restrict__ mask,
+                                const double *__restrict__ f, unsigned char *__restrict__ rec) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t < nrec * 64; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = t >> 6;
+        const int lane = (int)(t & 63);
+        unsigned char *r = rec + g * rec_bytes;
+        V *coef = reinterpret_cast<V *>(r);
+        for (int q = 0; q < NS; q++) {
+            const int p = pos[(g * NS + q) * 64 + lane];
+            coef[q * 64 + lane] = p >= 0 ? (V)f[p] : V(0);
+        }
+        int off = NS * 64 * (int)sizeof(V);
+        if (up) {
+            const int p = dpos[t];
+            reinterpret_cast<V *>(r + off)[lane] = p >= 0 ? (V)f[p] : V(1);
+            off += 64 * (int)sizeof(V);
+        }
+        reinterpret_cast<unsigned short *>(r + off)[lane] = mask[t];
     }
 }
 
@@ -196,51 +212,135 @@ __global__ void fill_sent(T *__restrict__ x, int64_t n) {
 }
 
 // ---- the sweep ---------------------------------------------------------------
+// Step record of a tile (written by stencil_records): the coefficients
+// [NS][64], the diagonal [64] (U only) and the presence masks [64] (u16), so
+// that the loader wave moves a step's operands with NCH 16-byte loads per lane.
+template <int KIND, bool UP, class V>
+struct shape {
+    static constexpr int NS = sk<KIND>::NS;
+    static constexpr int OFF_DIAG = NS * 64 * (int)sizeof(V);
+    static constexpr int OFF_MASK = OFF_DIAG + (UP ? 64 * (int)sizeof(V) : 0);
+    static constexpr int REC = OFF_MASK + 128;       // bytes per step record (a multiple of 128)
+    static constexpr int NCH = (REC + 1023) / 1024;  // 16-byte loads per lane and step
+    // steps of loads in flight: at most 56 of the wave's 63 outstanding loads
+    static constexpr int U = (NCH + 2) * 8 <= 56 ? 8 : 4;
+    static constexpr int RL = 8;   // staged steps in LDS = compute unroll = own-result ring
+    static constexpr int R = 16;   // halo ring (steps), a power of two
+    static constexpr int RS = R + 1;  // halo line stride (odd: lanes of different lines hit different banks)
+};
+
+// Lines around a tile whose values its rows read ("frame"): in 3-D the 36
+// lines (rj, rk) with rj in [-1, TJ], rk in [-1, TK] outside the tile, in 2-D
+// the two lines beside it.
+template <int KIND>
+__host__ __device__ constexpr int frame_count() {
+    return sk<KIND>::TK == 1 ? 2 : 2 * (sk<KIND>::TJ + 2) + 2 * sk<KIND>::TK;
+}
+template <int KIND>
+__host__ __device__ constexpr int frame_id(int rj, int rk) {
+    constexpr int TJ = sk<KIND>::TJ, TK = sk<KIND>::TK, W = TJ + 2;
+    if (TK == 1) return rk != 0 ? -1 : (rj == -1 ? 0 : (rj == TJ ? 1 : -1));
+    if (rk == -1 && rj >= -1 && rj <= TJ) return rj + 1;
+    if (rk == TK && rj >= -1 && rj <= TJ) return W + rj + 1;
+    if (rj == -1 && rk >= 0 && rk < TK) return 2 * W + rk;
+    if (rj == TJ && rk >= 0 && rk < TK) return 2 * W + TK + rk;
+    return -1;
+}
+template <int KIND>
+__host__ __device__ constexpr int frame_rj(int f) {
+    constexpr int TJ = sk<KIND>::TJ, TK = sk<KIND>::TK, W = TJ + 2;
+    if (TK == 1) return f == 0 ? -1 : TJ;
+    return f < W ? f - 1 : (f < 2 * W ? f - W - 1 : (f < 2 * W + TK ? -1 : TJ));
+}
+template <int KIND>
+__host__ __device__ constexpr int frame_rk(int f) {
+    constexpr int TJ = sk<KIND>::TJ, TK = sk<KIND>::TK, W = TJ + 2;
+    if (TK == 1) return 0;
+    return f < W ? -1 : (f < 2 * W ? TK : (f < 2 * W + TK ? f - 2 * W : f - 2 * W - TK));
+}
+// bit f: some lane of the tile reads frame line f through some slot
+template <int KIND, bool UP>
+constexpr unsigned long long frame_used() {
+    using K = sk<KIND>;
+    unsigned long long m = 0;
+    for (int kk = 0; kk < K::TK; kk++)
+        for (int jj = 0; jj < K::TJ; jj++)
+            for (int q = 0; q < K::NS; q++) {
+                const int rj = jj + sdj<KIND, UP>(q), rk = kk + sdk<KIND, UP>(q);
+                if (rj < 0 || rj >= K::TJ || rk < 0 || rk >= K::TK) m |= 1ull << frame_id<KIND>(rj, rk);
+            }
+    return m;
+}
+template <int KIND, bool UP>
+constexpr int delta_max() {
+    int d = 0;
+    for (int q = 0; q < sk<KIND>::NS; q++) d = sdelta<KIND, UP>(q) > d ? sdelta<KIND, UP>(q) : d;
+    return d;
+}
+
 template <class V, class TB, class TO, class TR>
 struct sweep_args {
-    int nx, ny, nz, P, ntiles;
+    int nx, ny, nz, P, ntiles, n;
     const int4 *meta;
     const long long *base;
     const int *tile_of_ticket;
     unsigned *ticket, *reset_ticket;
-    const V *coef, *diag;
-    const unsigned short *mask;
-    const TB *bin;  // right-hand side, natural order
-    TO *out;        // result, natural order (polled by the other tiles' loaders)
-    TR *rearm;      // gets the sentinel at each row this sweep computes
+    const unsigned char *rec;  // the triangle's step records in precision V
+    const TB *bin;             // right-hand side, natural order
+    TO *out;                   // result, natural order (polled by the other tiles' loaders)
+    TR *rearm;                 // gets the sentinel at each row this sweep computes
     int *err;
     const int *gate;
 };
 
 constexpr unsigned ST_SPIN_LIMIT = 1u << 22;
 
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Polls a result until its producer has written it.  Loads only: on the spin
+// limit, or when another wave has raised the error flag, sets bad and returns 0.
+// Every load it issues is examined before it returns, so no load of the poll
+// is left in flight for the caller's waits to count.
 template <class T>
-__device__ __forceinline__ T st_poll(const T *p, int *err) {
+__device__ __forceinline__ T st_poll(const T *p, const int *err, bool &bad) {
     T v = ld_agent(p);
     unsigned spins = 0;
     while (is_sent(v)) {
+        if (++spins > ST_SPIN_LIMIT || ((spins & 1023u) == 0 && ld_agent(err) != 0)) {
+            bad = true;
+            v = T(0);
+            break;
+        }
         __builtin_amdgcn_s_sleep(1);
         v = ld_agent(p);
-        ++spins;
-        if (spins > ST_SPIN_LIMIT || ((spins & 1023u) == 0 && ld_agent(err) != 0)) {
-            atomicOr(err, 1);
-            return T(0);
-        }
     }
     return v;
 }
 
-__device__ __forceinline__ double lane_shfl(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ float lane_shfl(float v, int src) { return __shfl(v, src, 64); }
-
+// Two waves per tile.  The loader wave issues every global load of the tile
+// and nothing else (no store, no atomic until its last step), so the
+// compiler's wait for a step's loads counts only the later steps' loads and
+// the U steps in flight stay in flight; the compute wave reads LDS only and
+// stores the results.
+//
+// LDS: s_rec / s_bin: RL staged steps (records and right-hand sides); s_halo:
+// for each of the tile's 64 lines and of the frame lines, a ring of the last R
+// results by local step.  The compute wave writes its lines' entries, the
+// loader the frame lines' entries (polled in global memory: value as flag).
 template <int KIND, bool UP, class V, class TB, class TO, class TR>
 __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A) {
     using K = sk<KIND>;
+    using SH = shape<KIND, UP, V>;
     constexpr int NS = K::NS, TJ = K::TJ, TK = K::TK;
-    constexpr int U = NS > 4 ? 8 : 16;   // unroll = ring length = prefetch distance (steps)
-    constexpr int RL = NS > 4 ? 8 : 16;  // LDS ring of staged external values (steps)
+    constexpr int U = SH::U, RL = SH::RL, R = SH::R, RS = SH::RS, NCH = SH::NCH;
+    constexpr int NF = frame_count<KIND>();
+    constexpr int DM = delta_max<KIND, UP>();
+    constexpr unsigned long long USED = frame_used<KIND, UP>();
+    static_assert(DM < RL && R >= RL + DM && NF <= 64, "ring sizes");
     if (A.gate && *A.gate) return;
-    __shared__ V ext[RL][NS][64];
+    __shared__ __align__(16) unsigned char s_rec[RL][NCH * 1024];
+    __shared__ V s_bin[RL][64];
+    __shared__ V s_halo[(64 + NF) * RS];
     __shared__ int s_tile, s_ready, s_cons;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (threadIdx.x == 0) {
@@ -259,105 +359,110 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
     const int tile = s_tile;
     if (tile < 0) return;
     const int4 m = A.meta[tile];
-    const long long base = A.base[tile];
+    const long long rec0 = A.base[tile] / 64;  // the tile's first step record
     const int S = m.w;
     const int jj = lane % TJ, kk = lane / TJ;
     const int j = m.x + jj, k = m.y + kk;
-    const bool active = j < A.ny && k < A.nz;
     const int tfirst = UP ? m.z + m.w - 1 : m.z;
     const int ibase = tfirst - K::A * j - K::B * k;  // i = ibase + s (L), ibase - s (U)
-    const int rowbase = A.nx * j + A.P * k;
-    // slot q's line lies in this tile (its value comes from a lane's ring)
-    auto in_tile = [&](int q) {
-        const int dj = sdj<KIND, UP>(q), dk = sdk<KIND, UP>(q);
-        return jj + dj >= 0 && jj + dj < TJ && kk + dk >= 0 && kk + dk < TK;
-    };
-    auto slot_off = [&](int q) { return sdi<KIND, UP>(q) + A.nx * sdj<KIND, UP>(q) + A.P * sdk<KIND, UP>(q); };
+    const long long rowbase = (long long)A.nx * j + (long long)A.P * k;
 
     if (wave == 1) {
-        // ---------------- loader: the other tiles' results, U steps ahead, staged in LDS
-        V lq[U][NS];
-        unsigned lm[U];
+        // ---------------- loader: lane f < NF also watches frame line f
+        const int f = lane < NF ? lane : 0;
+        const int jn = m.x + frame_rj<KIND>(f), kn = m.y + frame_rk<KIND>(f);
+        const bool watch = lane < NF && ((USED >> f) & 1ull) && jn >= 0 && jn < A.ny && kn >= 0 && kn < A.nz;
+        const int nbase = tfirst - K::A * jn - K::B * kn;
+        const long long nrow = (long long)A.nx * jn + (long long)A.P * kn;
+        const long long last = (long long)A.n - 1;
+        auto clampi = [&](long long e) { return e < 0 ? 0 : (e > last ? last : e); };
+        auto npos = [&](int s) { return UP ? nbase - s : nbase + s; };  // watched line's row at local step s
+        const u32x4 *rec = reinterpret_cast<const u32x4 *>(A.rec);
+        // loads of U steps in flight; every load is issued (clamped addresses, padded
+        // records), so each step adds the same count
+        u32x4 ch[U][NCH];
+        TO ex[U];
+        TB bv[U];
         auto issue = [&](int s, int u) {
-            const int i = UP ? ibase - s : ibase + s;
-            const bool valid = active && s < S && i >= 0 && i < A.nx;
-            lm[u] = valid ? (unsigned)A.mask[base + (long long)s * 64 + lane] : 0u;
-            const int r = i + rowbase;
+            const long long g = (rec0 + s) * (SH::REC / 16);
 #pragma unroll
-            for (int q = 0; q < NS; q++) {
-                lq[u][q] = V(0);
-                if (valid && !in_tile(q) && ((lm[u] >> q) & 1u)) lq[u][q] = (V)ld_agent(A.out + (r + slot_off(q)));
-            }
+            for (int c = 0; c < NCH; c++) ch[u][c] = __builtin_nontemporal_load(rec + g + c * 64 + lane);
+            ex[u] = ld_agent(A.out + clampi(nrow + npos(s - 1)));
+            bv[u] = A.bin[clampi(rowbase + (UP ? ibase - s : ibase + s))];
         };
+        bool bad = false;
+        // the watched line's result of local step s into its halo ring
+        auto stage_line = [&](int s, TO v) {
+            const int p = npos(s);
+            const bool need = watch && p >= 0 && p < A.nx;
+            if (need && is_sent(v)) v = st_poll(A.out + (nrow + p), A.err, bad);
+            if (lane < NF) s_halo[(64 + lane) * RS + (s & (R - 1))] = need ? (V)v : V(0);
+        };
+        // frame results of local steps -DM .. -2: rows before the tile's first step that it reads
+#pragma unroll
+        for (int d = 2; d <= DM; d++) stage_line(-d, ld_agent(A.out + clampi(nrow + npos(-d))));
 #pragma unroll
         for (int u = 0; u < U; u++) issue(u, u);
         int cons = 0;
-        for (int s0 = 0; s0 < S; s0 += U) {
+        auto consume = [&](int s, int u) {
+            // LDS slot s % RL is free once the compute wave has finished step s - RL
+            for (unsigned spins = 0; cons < s - RL + 1; spins++) {
+                cons = __hip_atomic_load(&s_cons, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cons >= s - RL + 1) break;
+                if (spins > ST_SPIN_LIMIT) {  // never expected: the compute wave stalled
+                    bad = true;
+                    cons = s;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __asm__ __volatile__("" ::: "memory");  // no staging write moves above the check
+            unsigned char *slot = s_rec[s % RL];
+#pragma unroll
+            for (int c = 0; c < NCH; c++) *reinterpret_cast<u32x4 *>(slot + c * 1024 + lane * 16) = ch[u][c];
+            s_bin[s % RL][lane] = (V)bv[u];
+            stage_line(s - 1, ex[u]);
+            // LDS operations of one wave complete in order; the wait makes the staged
+            // values land before the progress (the loads in flight are not waited for)
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            if (lane == 0) __hip_atomic_store(&s_ready, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        int s0 = 0;
+        for (; s0 + U <= S; s0 += U) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int s = s0 + u;
-                if (s < S) {
-                    // the LDS slot of step s is free once the compute wave has consumed step s - RL
-                    for (unsigned spins = 0; cons < s - RL + 1; spins++) {
-                        cons = __hip_atomic_load(&s_cons, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (cons >= s - RL + 1) break;
-                        if (spins > ST_SPIN_LIMIT) {  // never expected: the compute wave stalled
-                            atomicOr(A.err, 1);
-                            cons = s;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    const int r = (UP ? ibase - s : ibase + s) + rowbase;
-#pragma unroll
-                    for (int q = 0; q < NS; q++) {
-                        if (!in_tile(q) && ((lm[u] >> q) & 1u)) {
-                            V v = lq[u][q];
-                            if (is_sent(v)) v = (V)st_poll(A.out + (r + slot_off(q)), A.err);
-                            ext[s % RL][q][lane] = v;
-                        }
-                    }
-                    if (lane == 0)
-                        __hip_atomic_store(&s_ready, s + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    issue(s + U, u);
-                }
+                consume(s0 + u, u);
+                issue(s0 + u + U, u);
             }
         }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (s0 + u < S) consume(s0 + u, u);
+        if (__any(bad) && lane == 0) atomicOr(A.err, 1);
         return;
     }
 
     // ---------------- compute
-    V ring[U];
-    V qc[U][NS];
-    V qd[U];
-    TB qb[U];
-    unsigned qm[U];
-    auto fetch = [&](int s, int u) {
-        const int i = UP ? ibase - s : ibase + s;
-        const bool valid = active && s < S && i >= 0 && i < A.nx;
-        if (s < S) {
-            const long long pr = base + (long long)s * 64;
+    const bool active = j < A.ny && k < A.nz;
+    int hl[NS];  // halo line (times the stride) that slot q reads
 #pragma unroll
-            for (int q = 0; q < NS; q++) qc[u][q] = __builtin_nontemporal_load(A.coef + pr * NS + q * 64 + lane);
-            qm[u] = __builtin_nontemporal_load(A.mask + pr + lane);
-            if (UP) qd[u] = __builtin_nontemporal_load(A.diag + pr + lane);
-        }
-        qb[u] = valid ? A.bin[i + rowbase] : TB(0);
-    };
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        ring[u] = V(0);
-        qd[u] = V(1);
-        fetch(u, u);
+    for (int q = 0; q < NS; q++) {
+        const int rj = jj + sdj<KIND, UP>(q), rk = kk + sdk<KIND, UP>(q);
+        const bool in = rj >= 0 && rj < TJ && rk >= 0 && rk < TK;
+        hl[q] = (in ? rj + TJ * rk : 64 + frame_id<KIND>(rj, rk)) * RS;
     }
-    int rdy = 0;
-    for (int s0 = 0; s0 < S; s0 += U) {
+    V *const myhalo = s_halo + lane * RS;
+    V ring[RL];  // own results of the last RL steps (the i-1 terms)
 #pragma unroll
-        for (int u = 0; u < U; u++) {
+    for (int u = 0; u < RL; u++) ring[u] = V(0);
+    int rdy = 0;
+    for (int s0 = 0; s0 < S; s0 += RL) {
+#pragma unroll
+        for (int u = 0; u < RL; u++) {
             const int s = s0 + u;
             if (s < S) {
                 for (unsigned spins = 0; rdy <= s; spins++) {
-                    rdy = __hip_atomic_load(&s_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    rdy = __hip_atomic_load(&s_ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     if (rdy > s) break;
                     if (spins > ST_SPIN_LIMIT) {  // never expected: the loader stalled
                         atomicOr(A.err, 1);
@@ -366,32 +471,29 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
                     }
                     __builtin_amdgcn_s_sleep(1);
                 }
-                const int i = UP ? ibase - s : ibase + s;
-                const bool valid = active && i >= 0 && i < A.nx;
-                V acc = (V)qb[u];
-                const unsigned msk = qm[u];
+                __asm__ __volatile__("" ::: "memory");  // no staged read moves above the check
+                const unsigned char *rs = s_rec[u];
+                V acc = s_bin[u][lane];
+                const unsigned msk = *reinterpret_cast<const unsigned short *>(rs + SH::OFF_MASK + lane * 2);
 #pragma unroll
                 for (int q = 0; q < NS; q++) {
-                    const int dj = sdj<KIND, UP>(q), dk = sdk<KIND, UP>(q);
-                    const int d = sdelta<KIND, UP>(q);  // 1 <= d < U for every kind
-                    const V own = ring[(u - d + U) % U];
+                    const V c = *reinterpret_cast<const V *>(rs + q * 64 * (int)sizeof(V) + lane * (int)sizeof(V));
+                    const int d = sdelta<KIND, UP>(q);  // 1 <= d <= DM
                     V nv;
-                    if (dj == 0 && dk == 0) {
-                        nv = own;
-                    } else {
-                        const V sh = lane_shfl(own, lane + dj + TJ * dk);
-                        nv = in_tile(q) ? sh : ext[s % RL][q][lane];
-                    }
-                    if ((msk >> q) & 1u) acc = acc - qc[u][q] * nv;
+                    if (sdj<KIND, UP>(q) == 0 && sdk<KIND, UP>(q) == 0) nv = ring[(u - d + RL) % RL];
+                    else nv = s_halo[hl[q] + ((s - d) & (R - 1))];
+                    if ((msk >> q) & 1u) acc = acc - c * nv;
                 }
-                if (UP) acc = acc / qd[u];
+                if (UP) acc = acc / *reinterpret_cast<const V *>(rs + SH::OFF_DIAG + lane * (int)sizeof(V));
                 ring[u] = acc;
-                if (valid) {
-                    st_agent(A.out + (i + rowbase), (TO)acc);
-                    A.rearm[i + rowbase] = sentinel<TR>();
+                myhalo[s & (R - 1)] = acc;
+                const int i = UP ? ibase - s : ibase + s;
+                if (active && i >= 0 && i < A.nx) {
+                    st_agent(A.out + (rowbase + i), (TO)acc);
+                    A.rearm[rowbase + i] = sentinel<TR>();
                 }
+                __asm__ __volatile__("" ::: "memory");  // the staged reads stay before the release of their slot
                 if (lane == 0) __hip_atomic_store(&s_cons, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                fetch(s + U, u);
             }
         }
     }
@@ -405,20 +507,19 @@ void sweep_pair(hala_ilu_s *M, const TIN *x, TOUT *r, const int *gate) {
     V *y = f64 ? reinterpret_cast<V *>(sp.y64.p) : reinterpret_cast<V *>(sp.y32.p);
     {
         stencil_tri &L = sp.L;
-        sweep_args<V, TIN, V, TOUT> a{sp.nx, sp.ny, sp.nz, sp.P, L.ntiles, L.meta.as<int4>(), L.base.as<long long>(),
-                                      L.tile_of_ticket.as<int>(), L.ticket.as<unsigned>(), sp.U.ticket.as<unsigned>(),
-                                      reinterpret_cast<const V *>(f64 ? L.coef64.p : L.coef32.p), nullptr,
-                                      L.mask.as<unsigned short>(), x, y, r, wait_error_flag(c), gate};
+        sweep_args<V, TIN, V, TOUT> a{sp.nx, sp.ny, sp.nz, sp.P, L.ntiles, M->n, L.meta.as<int4>(),
+                                      L.base.as<long long>(), L.tile_of_ticket.as<int>(), L.ticket.as<unsigned>(),
+                                      sp.U.ticket.as<unsigned>(), (f64 ? L.rec64 : L.rec32).as<unsigned char>(),
+                                      x, y, r, wait_error_flag(c), gate};
         stencil_sweep<KIND, false, V, TIN, V, TOUT><<<L.ntiles, 128, 0, c->stream>>>(a);
         HALA_CHECK_LAUNCH();
     }
     {
         stencil_tri &Up = sp.U;
-        sweep_args<V, V, TOUT, V> a{sp.nx, sp.ny, sp.nz, sp.P, Up.ntiles, Up.meta.as<int4>(), Up.base.as<long long>(),
-                                    Up.tile_of_ticket.as<int>(), Up.ticket.as<unsigned>(), sp.L.ticket.as<unsigned>(),
-                                    reinterpret_cast<const V *>(f64 ? Up.coef64.p : Up.coef32.p),
-                                    reinterpret_cast<const V *>(f64 ? Up.diag64.p : Up.diag32.p),
-                                    Up.mask.as<unsigned short>(), y, r, y, wait_error_flag(c), gate};
+        sweep_args<V, V, TOUT, V> a{sp.nx, sp.ny, sp.nz, sp.P, Up.ntiles, M->n, Up.meta.as<int4>(),
+                                    Up.base.as<long long>(), Up.tile_of_ticket.as<int>(), Up.ticket.as<unsigned>(),
+                                    sp.L.ticket.as<unsigned>(), (f64 ? Up.rec64 : Up.rec32).as<unsigned char>(),
+                                    y, r, y, wait_error_flag(c), gate};
         stencil_sweep<KIND, true, V, V, TOUT, V><<<Up.ntiles, 128, 0, c->stream>>>(a);
         HALA_CHECK_LAUNCH();
     }
@@ -585,32 +686,28 @@ bool stencil_build(hala_ilu_s *M) {
     return false;
 }
 
+// bytes of one step record (the layout of shape<>::REC)
+static int record_bytes(int NS, bool up, int vsize) { return NS * 64 * vsize + (up ? 64 * vsize : 0) + 128; }
+
 void stencil_load(hala_ilu_s *M, bool f32) {
     hala_ctx c = M->ctx;
     stencil_plan &sp = *M->sp;
     const kind_info K = info_of(sp.kind);
     const double *f = M->factor.as<double>();
     auto load = [&](stencil_tri &T, bool up) {
-        const int64_t nc = T.nprow * K.NS;
-        T.coef64.reserve((size_t)nc * sizeof(double));
-        stencil_values<double><<<grid_for(nc, 1024, 8192), 256, 0, c->stream>>>(nc, T.pos.as<int>(), f,
-                                                                                T.coef64.as<double>(), 0.0);
-        if (up) {
-            T.diag64.reserve((size_t)T.nprow * sizeof(double));
-            stencil_values<double><<<grid_for(T.nprow, 1024, 8192), 256, 0, c->stream>>>(
-                T.nprow, T.dpos.as<int>(), f, T.diag64.as<double>(), 1.0);
-        }
-        if (f32) {
-            T.coef32.reserve((size_t)nc * sizeof(float));
-            stencil_values<float><<<grid_for(nc, 1024, 8192), 256, 0, c->stream>>>(nc, T.pos.as<int>(), f,
-                                                                                  T.coef32.as<float>(), 0.0f);
-            if (up) {
-                T.diag32.reserve((size_t)T.nprow * sizeof(float));
-                stencil_values<float><<<grid_for(T.nprow, 1024, 8192), 256, 0, c->stream>>>(
-                    T.nprow, T.dpos.as<int>(), f, T.diag32.as<float>(), 1.0f);
-            }
-        }
-        HALA_CHECK_LAUNCH();
+        const int64_t nrec = T.nprow / 64;
+        auto fill = [&](dbuf &rec, auto zero) {
+            using V = decltype(zero);
+            const int rb = record_bytes(K.NS, up, (int)sizeof(V));
+            // the loader reads up to 8 records past a tile and 1 KiB past a record
+            rec.reserve((size_t)(nrec + 8) * rb + 1024);
+            stencil_records<V><<<grid_for(nrec * 64, 1024, 8192), 256, 0, c->stream>>>(
+                nrec, K.NS, up ? 1 : 0, rb, T.pos.as<int>(), up ? T.dpos.as<int>() : nullptr,
+                T.mask.as<unsigned short>(), f, rec.as<unsigned char>());
+            HALA_CHECK_LAUNCH();
+        };
+        fill(T.rec64, 0.0);
+        if (f32) fill(T.rec32, 0.0f);
     };
     load(sp.L, false);
     load(sp.U, true);

@@ -31,7 +31,7 @@ struct stencil_tri {
     dbuf pos;             // int per packed coefficient: CSR position, -1 absent
     dbuf dpos;            // U: int per packed row: CSR position of the diagonal, -1 idle
     dbuf mask;            // uint16 per packed row: bit q set when slot q's entry is present
-    dbuf coef64, coef32, diag64, diag32;
+    dbuf rec64, rec32;    // step records (coefficients, U's diagonal, masks) in FP64 / FP32
     dbuf ticket;          // one counter (reset by the other triangle's sweep)
 };
 

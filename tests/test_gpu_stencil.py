@@ -2,7 +2,8 @@
 oracle's apply (the reference's apply_ilu_array order), in FP64, FP32 and FP32
 between FP64 vectors, on grids whose sides are and are not multiples of the
 8 x 8 (64 x 1 in 2-D) tiles, repeated (the sentinels re-arm themselves) and in
-place; patterns that are not stencils keep the general sweeps."""
+place; patterns that are not stencils keep the general sweeps.  The schedule
+is opt-in (option tri_stencil = 1)."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,15 @@ CASES = {
 
 def dev(engine, a):
     return engine.load(a)
+
+
+@pytest.fixture(autouse=True)
+def pencil_sweeps(engine):
+    # the pencil schedule is opt-in (option tri_stencil)
+    saved = engine.get_option("tri_stencil")
+    engine.set_option("tri_stencil", 1)
+    yield
+    engine.set_option("tri_stencil", saved)
 
 
 def host(t):
