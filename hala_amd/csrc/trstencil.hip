@@ -144,7 +144,7 @@ struct pack_args {
     const int4 *meta;
     const long long *base;
     const int *pntr, *indx, *diag;
-    int *pos, *dpos;
+    int *pos, *dpos, *prow;
     unsigned short *mask;
 };
 
@@ -177,6 +177,7 @@ __global__ void stencil_pack(pack_args a) {
     }
     a.mask[prow] = (unsigned short)msk;
     if (a.up) a.dpos[prow] = valid ? a.diag[r] : -1;
+    else a.prow[prow] = r;  // the packed vectors' row map (the L order)
 }
 
 // One thread per (step record g, lane): the factor's values of packed row
@@ -214,7 +215,7 @@ __global__ void fill_sent(T *__restrict__ x, int64_t n) {
 // ---- the sweep ---------------------------------------------------------------
 // Step record of a tile (written by stencil_records): the coefficients
 // [NS][64], the diagonal [64] (U only) and the presence masks [64] (u16), so
-// that the loader wave moves a step's operands with NCH 16-byte loads per lane.
+// that the streaming wave moves a step's operands with NCH 16-byte loads per lane.
 template <int KIND, bool UP, class V>
 struct shape {
     static constexpr int NS = sk<KIND>::NS;
@@ -222,11 +223,13 @@ struct shape {
     static constexpr int OFF_MASK = OFF_DIAG + (UP ? 64 * (int)sizeof(V) : 0);
     static constexpr int REC = OFF_MASK + 128;       // bytes per step record (a multiple of 128)
     static constexpr int NCH = (REC + 1023) / 1024;  // 16-byte loads per lane and step
-    // steps of loads in flight: at most 56 of the wave's 63 outstanding loads
-    static constexpr int U = (NCH + 2) * 8 <= 56 ? 8 : 4;
-    static constexpr int RL = 8;   // staged steps in LDS = compute unroll = own-result ring
-    static constexpr int R = 16;   // halo ring (steps), a power of two
-    static constexpr int RS = R + 1;  // halo line stride (odd: lanes of different lines hit different banks)
+    // streamed steps in flight: (NCH + 1) loads each, at most 56 of a wave's 63
+    static constexpr int U = 56 / (NCH + 1) < 16 ? 56 / (NCH + 1) : 16;
+    static constexpr int UE = 8;                     // polled steps in flight
+    static constexpr int RL = NCH > 4 ? 8 : 16;      // staged steps in LDS
+    static constexpr int RG = 8;                     // compute unroll = own-result ring
+    static constexpr int R = 32;                     // halo ring (steps), a power of two
+    static constexpr int RS = R + 1;                 // halo line stride (odd: lanes of different lines hit different banks)
 };
 
 // Lines around a tile whose values its rows read ("frame"): in 3-D the 36
@@ -278,17 +281,21 @@ constexpr int delta_max() {
     return d;
 }
 
-template <class V, class TB, class TO, class TR>
+// Vectors of a sweep live in the packed order of the L records: row (i, j, k)
+// of tile t, lane l, at L step s = i + A j + B k - first(t) sits at
+// base(t) + 64 s + l, so a tile step reads and writes 512 contiguous bytes.
+template <class V>
 struct sweep_args {
-    int nx, ny, nz, P, ntiles, n;
+    int nx, ny, nz, P, ntiles, ntj;
+    long long nprow;
     const int4 *meta;
     const long long *base;
     const int *tile_of_ticket;
     unsigned *ticket, *reset_ticket;
     const unsigned char *rec;  // the triangle's step records in precision V
-    const TB *bin;             // right-hand side, natural order
-    TO *out;                   // result, natural order (polled by the other tiles' loaders)
-    TR *rearm;                 // gets the sentinel at each row this sweep computes
+    const V *bin;              // right-hand side (packed)
+    V *out;                    // result (packed; polled by the other tiles)
+    V *rearm;                  // gets the sentinel at each packed row this sweep computes
     int *err;
     const int *gate;
 };
@@ -298,9 +305,9 @@ constexpr unsigned ST_SPIN_LIMIT = 1u << 22;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Polls a result until its producer has written it.  Loads only: on the spin
-// limit, or when another wave has raised the error flag, sets bad and returns 0.
-// Every load it issues is examined before it returns, so no load of the poll
-// is left in flight for the caller's waits to count.
+// limit, or when another wave has raised the error flag, sets bad and returns
+// 0.  Every load it issues is examined before it returns, so no load of the
+// poll is left in flight for the caller's waits to count.
 template <class T>
 __device__ __forceinline__ T st_poll(const T *p, const int *err, bool &bad) {
     T v = ld_agent(p);
@@ -317,32 +324,52 @@ __device__ __forceinline__ T st_poll(const T *p, const int *err, bool &bad) {
     return v;
 }
 
-// Two waves per tile.  The loader wave issues every global load of the tile
-// and nothing else (no store, no atomic until its last step), so the
-// compiler's wait for a step's loads counts only the later steps' loads and
-// the U steps in flight stay in flight; the compute wave reads LDS only and
-// stores the results.
-//
-// LDS: s_rec / s_bin: RL staged steps (records and right-hand sides); s_halo:
-// for each of the tile's 64 lines and of the frame lines, a ring of the last R
-// results by local step.  The compute wave writes its lines' entries, the
-// loader the frame lines' entries (polled in global memory: value as flag).
-template <int KIND, bool UP, class V, class TB, class TO, class TR>
-__global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A) {
+// Bounded LDS wait of one wave for a progress counter (another wave of the
+// workgroup publishes it); false when the spin limit is reached.
+__device__ __forceinline__ bool lds_wait(int *flag, int &seen, int want) {
+    for (unsigned spins = 0; seen < want; spins++) {
+        seen = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (seen >= want) break;
+        if (spins > ST_SPIN_LIMIT) {
+            seen = want;
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// Three waves per tile, each with one kind of memory traffic, so that the
+// compiler's wait for a load counts only younger loads of the same stream:
+//   wave 0 (compute): lane (jj, kk) owns line (j0 + jj, k0 + kk); per step it
+//     forms its row from LDS: acc = b - sum of c_q v_q over the present slots
+//     in CSR order (U: / d), v_q from its own register ring (i-1 terms) or the
+//     halo ring; stores the result and re-arms the row of the other vector;
+//   wave 1 (streamer): the step records and right-hand sides, U steps ahead,
+//     into an LDS ring of RL steps;
+//   wave 2 (poller): lane f watches frame line f: the result of each step of
+//     that line, polled in the other tile's output (value as flag) UE steps
+//     ahead, into the line's halo ring.  A poll that finds the sentinel means
+//     the tile has caught up with its producer, so the polls in flight behind
+//     it were issued too early as well: they are issued again together.
+// Tiles take tickets in dependency order; every wait is bounded (error flag,
+// then zeros).
+template <int KIND, bool UP, class V>
+__global__ __launch_bounds__(192) void stencil_sweep(sweep_args<V> A) {
     using K = sk<KIND>;
     using SH = shape<KIND, UP, V>;
     constexpr int NS = K::NS, TJ = K::TJ, TK = K::TK;
-    constexpr int U = SH::U, RL = SH::RL, R = SH::R, RS = SH::RS, NCH = SH::NCH;
+    constexpr int U = SH::U, UE = SH::UE, RL = SH::RL, RG = SH::RG, R = SH::R, RS = SH::RS, NCH = SH::NCH;
     constexpr int NF = frame_count<KIND>();
     constexpr int DM = delta_max<KIND, UP>();
     constexpr unsigned long long USED = frame_used<KIND, UP>();
-    static_assert(DM < RL && R >= RL + DM && NF <= 64, "ring sizes");
+    static_assert(DM < RG && RL % RG == 0 && R >= RL + DM && NF <= 64, "ring sizes");
     if (A.gate && *A.gate) return;
     __shared__ __align__(16) unsigned char s_rec[RL][NCH * 1024];
     __shared__ V s_bin[RL][64];
     __shared__ V s_halo[(64 + NF) * RS];
-    __shared__ int s_tile, s_ready, s_cons;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ int s_tile, s_rdy, s_ext, s_cons;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (threadIdx.x == 0) {
         const unsigned tk = atomicAdd(A.ticket, 1u);
         if (tk < (unsigned)A.ntiles) {
@@ -351,7 +378,8 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
             s_tile = -1;
             atomicOr(A.err, 8);
         }
-        s_ready = 0;
+        s_rdy = 0;
+        s_ext = 0;
         s_cons = 0;
         if (tk == 0) *A.reset_ticket = 0u;  // the other triangle's counter, for its next sweep
     }
@@ -359,73 +387,43 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
     const int tile = s_tile;
     if (tile < 0) return;
     const int4 m = A.meta[tile];
-    const long long rec0 = A.base[tile] / 64;  // the tile's first step record
+    const long long base = A.base[tile];
     const int S = m.w;
-    const int jj = lane % TJ, kk = lane / TJ;
-    const int j = m.x + jj, k = m.y + kk;
-    const int tfirst = UP ? m.z + m.w - 1 : m.z;
-    const int ibase = tfirst - K::A * j - K::B * k;  // i = ibase + s (L), ibase - s (U)
-    const long long rowbase = (long long)A.nx * j + (long long)A.P * k;
+    const int tfirst = UP ? m.z + S - 1 : m.z;  // global step of local step 0
+    const long long last = A.nprow - 1;
+    auto clampi = [&](long long e) { return e < 0 ? 0 : (e > last ? last : e); };
+    // packed row of this tile's lane at local step s
+    auto pidx = [&](int s) { return base + (long long)(UP ? S - 1 - s : s) * 64 + lane; };
 
     if (wave == 1) {
-        // ---------------- loader: lane f < NF also watches frame line f
-        const int f = lane < NF ? lane : 0;
-        const int jn = m.x + frame_rj<KIND>(f), kn = m.y + frame_rk<KIND>(f);
-        const bool watch = lane < NF && ((USED >> f) & 1ull) && jn >= 0 && jn < A.ny && kn >= 0 && kn < A.nz;
-        const int nbase = tfirst - K::A * jn - K::B * kn;
-        const long long nrow = (long long)A.nx * jn + (long long)A.P * kn;
-        const long long last = (long long)A.n - 1;
-        auto clampi = [&](long long e) { return e < 0 ? 0 : (e > last ? last : e); };
-        auto npos = [&](int s) { return UP ? nbase - s : nbase + s; };  // watched line's row at local step s
+        // ---------------- streamer
         const u32x4 *rec = reinterpret_cast<const u32x4 *>(A.rec);
-        // loads of U steps in flight; every load is issued (clamped addresses, padded
-        // records), so each step adds the same count
+        const long long rec0 = base / 64;
+        // every load is issued (clamped indices, padded records), so each step adds the same count
         u32x4 ch[U][NCH];
-        TO ex[U];
-        TB bv[U];
+        V bv[U];
         auto issue = [&](int s, int u) {
             const long long g = (rec0 + s) * (SH::REC / 16);
 #pragma unroll
             for (int c = 0; c < NCH; c++) ch[u][c] = __builtin_nontemporal_load(rec + g + c * 64 + lane);
-            ex[u] = ld_agent(A.out + clampi(nrow + npos(s - 1)));
-            bv[u] = A.bin[clampi(rowbase + (UP ? ibase - s : ibase + s))];
+            bv[u] = __builtin_nontemporal_load(A.bin + clampi(pidx(s)));
         };
-        bool bad = false;
-        // the watched line's result of local step s into its halo ring
-        auto stage_line = [&](int s, TO v) {
-            const int p = npos(s);
-            const bool need = watch && p >= 0 && p < A.nx;
-            if (need && is_sent(v)) v = st_poll(A.out + (nrow + p), A.err, bad);
-            if (lane < NF) s_halo[(64 + lane) * RS + (s & (R - 1))] = need ? (V)v : V(0);
-        };
-        // frame results of local steps -DM .. -2: rows before the tile's first step that it reads
-#pragma unroll
-        for (int d = 2; d <= DM; d++) stage_line(-d, ld_agent(A.out + clampi(nrow + npos(-d))));
 #pragma unroll
         for (int u = 0; u < U; u++) issue(u, u);
+        bool bad = false;
         int cons = 0;
         auto consume = [&](int s, int u) {
             // LDS slot s % RL is free once the compute wave has finished step s - RL
-            for (unsigned spins = 0; cons < s - RL + 1; spins++) {
-                cons = __hip_atomic_load(&s_cons, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (cons >= s - RL + 1) break;
-                if (spins > ST_SPIN_LIMIT) {  // never expected: the compute wave stalled
-                    bad = true;
-                    cons = s;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __asm__ __volatile__("" ::: "memory");  // no staging write moves above the check
+            if (!lds_wait(&s_cons, cons, s - RL + 1)) bad = true;
+            __asm__ __volatile__("" ::: "memory");  // no staging write moves above the wait
             unsigned char *slot = s_rec[s % RL];
 #pragma unroll
             for (int c = 0; c < NCH; c++) *reinterpret_cast<u32x4 *>(slot + c * 1024 + lane * 16) = ch[u][c];
-            s_bin[s % RL][lane] = (V)bv[u];
-            stage_line(s - 1, ex[u]);
+            s_bin[s % RL][lane] = bv[u];
             // LDS operations of one wave complete in order; the wait makes the staged
             // values land before the progress (the loads in flight are not waited for)
             __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-            if (lane == 0) __hip_atomic_store(&s_ready, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (lane == 0) __hip_atomic_store(&s_rdy, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         };
         int s0 = 0;
         for (; s0 + U <= S; s0 += U) {
@@ -442,8 +440,75 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
         return;
     }
 
+    if (wave == 2) {
+        // ---------------- poller: lane f < NF watches frame line f
+        const int f = lane < NF ? lane : 0;
+        const int jn = m.x + frame_rj<KIND>(f), kn = m.y + frame_rk<KIND>(f);
+        const bool watch = lane < NF && ((USED >> f) & 1ull) && jn >= 0 && jn < A.ny && kn >= 0 && kn < A.nz;
+        if (!__any(watch)) {  // no frame line in use: nothing to wait for
+            if (lane == 0) __hip_atomic_store(&s_ext, S + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+        const int tp = watch ? jn / TJ + A.ntj * (kn / TK) : tile;  // the producing tile
+        const int4 mp = A.meta[tp];
+        const long long bp = A.base[tp];
+        const int lanep = watch ? (jn - mp.x) + TJ * (kn - mp.y) : 0;
+        const int nbase = tfirst - K::A * jn - K::B * kn;
+        auto npos = [&](int s) { return UP ? nbase - s : nbase + s; };  // watched line's row at local step s
+        // its packed row: producer base + 64 * (its L step) + its lane
+        const long long ib = bp + (long long)(K::A * jn + K::B * kn - mp.z) * 64 + lanep;
+        auto eptr = [&](int s) { return A.out + clampi(ib + (long long)npos(s) * 64); };
+        auto need = [&](int s) {
+            const int p = npos(s);
+            return watch && p >= 0 && p < A.nx;
+        };
+        bool bad = false;
+        auto stage = [&](int s, V v) {
+            if (lane < NF) s_halo[(64 + lane) * RS + (s & (R - 1))] = need(s) ? v : V(0);
+        };
+        // results of local steps -DM .. -2: rows before the tile's first step that it reads
+#pragma unroll
+        for (int d = 2; d <= DM; d++) {
+            V v = ld_agent(eptr(-d));
+            if (need(-d) && is_sent(v)) v = st_poll(eptr(-d), A.err, bad);
+            stage(-d, v);
+        }
+        // poller step s stages local step s - 1
+        V ev[UE];
+#pragma unroll
+        for (int u = 0; u < UE; u++) ev[u] = ld_agent(eptr(u - 1));
+        int cons = 0;
+        auto consume = [&](int s, int u) {
+            // halo entry s - 1 is free once the compute wave has finished step s - 1 - R + DM
+            if (!lds_wait(&s_cons, cons, s - R + DM)) bad = true;
+            V v = ev[u];
+            const bool miss = need(s - 1) && is_sent(v);
+            if (__any(miss)) {
+                if (miss) v = st_poll(eptr(s - 1), A.err, bad);
+#pragma unroll
+                for (int k = 1; k < UE; k++) ev[(u + k) % UE] = ld_agent(eptr(s - 1 + k));
+            }
+            stage(s - 1, v);
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            if (lane == 0) __hip_atomic_store(&s_ext, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        };
+        int s0 = 0;
+        for (; s0 + UE <= S; s0 += UE) {
+#pragma unroll
+            for (int u = 0; u < UE; u++) {
+                consume(s0 + u, u);
+                ev[u] = ld_agent(eptr(s0 + u - 1 + UE));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UE; u++)
+            if (s0 + u < S) consume(s0 + u, u);
+        if (__any(bad) && lane == 0) atomicOr(A.err, 1);
+        return;
+    }
+
     // ---------------- compute
-    const bool active = j < A.ny && k < A.nz;
+    const int jj = lane % TJ, kk = lane / TJ;
     int hl[NS];  // halo line (times the stride) that slot q reads
 #pragma unroll
     for (int q = 0; q < NS; q++) {
@@ -452,50 +517,104 @@ __global__ __launch_bounds__(128) void stencil_sweep(sweep_args<V, TB, TO, TR> A
         hl[q] = (in ? rj + TJ * rk : 64 + frame_id<KIND>(rj, rk)) * RS;
     }
     V *const myhalo = s_halo + lane * RS;
-    V ring[RL];  // own results of the last RL steps (the i-1 terms)
+    V ring[RG];  // own results of the last RG steps (the i-1 terms)
 #pragma unroll
-    for (int u = 0; u < RL; u++) ring[u] = V(0);
-    int rdy = 0;
-    for (int s0 = 0; s0 < S; s0 += RL) {
+    for (int u = 0; u < RG; u++) ring[u] = V(0);
+    int rdy = 0, ext = 0;
+    bool bad = false;
+    // bounded wait until a staging wave's counter passes want (cached value first)
+    auto wait_for = [&](int *flag, int &seen, int want) {
+        for (unsigned spins = 0; seen < want; spins++) {
+            seen = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (seen >= want) break;
+            if (spins > ST_SPIN_LIMIT) {  // never expected: a staging wave stalled
+                bad = true;
+                seen = want;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __asm__ __volatile__("" ::: "memory");  // no staged read moves above the wait
+    };
+    // operands of a step, read one step ahead (double buffer b = step & 1), so
+    // that a step waits only for its halo values
+    V rc[2][NS], rd[2], rb[2];
+    unsigned rm[2];
+    auto read_rec = [&](int s, int b) {
+        const int sl = s % RL;
+        const unsigned char *rs = s_rec[sl];
 #pragma unroll
-        for (int u = 0; u < RL; u++) {
-            const int s = s0 + u;
+        for (int q = 0; q < NS; q++)
+            rc[b][q] = *reinterpret_cast<const V *>(rs + q * 64 * (int)sizeof(V) + lane * (int)sizeof(V));
+        rm[b] = *reinterpret_cast<const unsigned short *>(rs + SH::OFF_MASK + lane * 2);
+        rd[b] = UP ? *reinterpret_cast<const V *>(rs + SH::OFF_DIAG + lane * (int)sizeof(V)) : V(1);
+        rb[b] = s_bin[sl][lane];
+    };
+    wait_for(&s_rdy, rdy, 1);
+    read_rec(0, 0);
+    for (int s0 = 0; s0 < S; s0 += RG) {
+#pragma unroll
+        for (int u = 0; u < RG; u++) {
+            const int s = s0 + u, b = u & 1;
             if (s < S) {
-                for (unsigned spins = 0; rdy <= s; spins++) {
-                    rdy = __hip_atomic_load(&s_ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (rdy > s) break;
-                    if (spins > ST_SPIN_LIMIT) {  // never expected: the loader stalled
-                        atomicOr(A.err, 1);
-                        rdy = s + 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                __asm__ __volatile__("" ::: "memory");  // no staged read moves above the check
-                const unsigned char *rs = s_rec[u];
-                V acc = s_bin[u][lane];
-                const unsigned msk = *reinterpret_cast<const unsigned short *>(rs + SH::OFF_MASK + lane * 2);
+                // frame values of step s staged; every halo read issued before the first is used
+                wait_for(&s_ext, ext, s + 1);
+                V nq[NS];
 #pragma unroll
                 for (int q = 0; q < NS; q++) {
-                    const V c = *reinterpret_cast<const V *>(rs + q * 64 * (int)sizeof(V) + lane * (int)sizeof(V));
                     const int d = sdelta<KIND, UP>(q);  // 1 <= d <= DM
-                    V nv;
-                    if (sdj<KIND, UP>(q) == 0 && sdk<KIND, UP>(q) == 0) nv = ring[(u - d + RL) % RL];
-                    else nv = s_halo[hl[q] + ((s - d) & (R - 1))];
-                    if ((msk >> q) & 1u) acc = acc - c * nv;
+                    nq[q] = (sdj<KIND, UP>(q) == 0 && sdk<KIND, UP>(q) == 0) ? ring[(u - d + RG) % RG]
+                                                                             : s_halo[hl[q] + ((s - d) & (R - 1))];
                 }
-                if (UP) acc = acc / *reinterpret_cast<const V *>(rs + SH::OFF_DIAG + lane * (int)sizeof(V));
+                // the next step's operands meanwhile
+                if (s + 1 < S) {
+                    wait_for(&s_rdy, rdy, s + 2);
+                    read_rec(s + 1, b ^ 1);
+                }
+                // the row's terms in CSR order, selected rather than branched around
+                V acc = rb[b];
+#pragma unroll
+                for (int q = 0; q < NS; q++) {
+                    const V t = acc - rc[b][q] * nq[q];
+                    acc = ((rm[b] >> q) & 1u) ? t : acc;
+                }
+                if (UP) acc = acc / rd[b];
                 ring[u] = acc;
                 myhalo[s & (R - 1)] = acc;
-                const int i = UP ? ibase - s : ibase + s;
-                if (active && i >= 0 && i < A.nx) {
-                    st_agent(A.out + (rowbase + i), (TO)acc);
-                    A.rearm[rowbase + i] = sentinel<TR>();
-                }
-                __asm__ __volatile__("" ::: "memory");  // the staged reads stay before the release of their slot
+                // every lane stores (a lane off the grid or past its line writes a padding row)
+                const long long pi = pidx(s);
+                st_agent(A.out + pi, acc);
+                A.rearm[pi] = sentinel<V>();
+                // slot s % RL was read a step ago: released to the streaming wave
                 if (lane == 0) __hip_atomic_store(&s_cons, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
+    }
+    if (__any(bad) && lane == 0) atomicOr(A.err, 1);
+}
+
+// natural order -> packed (padding rows get 0)
+template <class TIN, class V>
+__global__ void stencil_gather(int64_t nprow, const int *__restrict__ prow, const TIN *__restrict__ x,
+                               V *__restrict__ xp, const int *gate) {
+    if (gate && *gate) return;
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; p < nprow; p += (int64_t)gridDim.x * blockDim.x) {
+        const int r = prow[p];
+        xp[p] = r >= 0 ? (V)x[r] : V(0);
+    }
+}
+
+// packed -> natural order
+template <class V, class TOUT>
+__global__ void stencil_scatter(int64_t nprow, const int *__restrict__ prow, const V *__restrict__ yp,
+                                TOUT *__restrict__ y, const int *gate) {
+    if (gate && *gate) return;
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; p < nprow; p += (int64_t)gridDim.x * blockDim.x) {
+        const int r = prow[p];
+        if (r >= 0) y[r] = (TOUT)yp[p];
     }
 }
 
@@ -504,25 +623,31 @@ void sweep_pair(hala_ilu_s *M, const TIN *x, TOUT *r, const int *gate) {
     hala_ctx c = M->ctx;
     stencil_plan &sp = *M->sp;
     const bool f64 = sizeof(V) == 8;
-    V *y = f64 ? reinterpret_cast<V *>(sp.y64.p) : reinterpret_cast<V *>(sp.y32.p);
+    V *xp = reinterpret_cast<V *>((f64 ? sp.xp64 : sp.xp32).p);
+    V *yp = reinterpret_cast<V *>((f64 ? sp.y64 : sp.y32).p);
+    V *up = reinterpret_cast<V *>((f64 ? sp.u64 : sp.u32).p);
+    const long long np = sp.L.nprow;
+    const int g = grid_for(np, 1024, 8192);
+    stencil_gather<TIN, V><<<g, 256, 0, c->stream>>>(np, sp.prow.as<int>(), x, xp, gate);
+    HALA_CHECK_LAUNCH();
     {
         stencil_tri &L = sp.L;
-        sweep_args<V, TIN, V, TOUT> a{sp.nx, sp.ny, sp.nz, sp.P, L.ntiles, M->n, L.meta.as<int4>(),
-                                      L.base.as<long long>(), L.tile_of_ticket.as<int>(), L.ticket.as<unsigned>(),
-                                      sp.U.ticket.as<unsigned>(), (f64 ? L.rec64 : L.rec32).as<unsigned char>(),
-                                      x, y, r, wait_error_flag(c), gate};
-        stencil_sweep<KIND, false, V, TIN, V, TOUT><<<L.ntiles, 128, 0, c->stream>>>(a);
+        sweep_args<V> a{sp.nx, sp.ny, sp.nz, sp.P, L.ntiles, sp.ntj, np, L.meta.as<int4>(), L.base.as<long long>(),
+                        L.tile_of_ticket.as<int>(), L.ticket.as<unsigned>(), sp.U.ticket.as<unsigned>(),
+                        (f64 ? L.rec64 : L.rec32).as<unsigned char>(), xp, yp, up, wait_error_flag(c), gate};
+        stencil_sweep<KIND, false, V><<<L.ntiles, 192, 0, c->stream>>>(a);
         HALA_CHECK_LAUNCH();
     }
     {
         stencil_tri &Up = sp.U;
-        sweep_args<V, V, TOUT, V> a{sp.nx, sp.ny, sp.nz, sp.P, Up.ntiles, M->n, Up.meta.as<int4>(),
-                                    Up.base.as<long long>(), Up.tile_of_ticket.as<int>(), Up.ticket.as<unsigned>(),
-                                    sp.L.ticket.as<unsigned>(), (f64 ? Up.rec64 : Up.rec32).as<unsigned char>(),
-                                    y, r, y, wait_error_flag(c), gate};
-        stencil_sweep<KIND, true, V, V, TOUT, V><<<Up.ntiles, 128, 0, c->stream>>>(a);
+        sweep_args<V> a{sp.nx, sp.ny, sp.nz, sp.P, Up.ntiles, sp.ntj, np, Up.meta.as<int4>(), Up.base.as<long long>(),
+                        Up.tile_of_ticket.as<int>(), Up.ticket.as<unsigned>(), sp.L.ticket.as<unsigned>(),
+                        (f64 ? Up.rec64 : Up.rec32).as<unsigned char>(), yp, up, yp, wait_error_flag(c), gate};
+        stencil_sweep<KIND, true, V><<<Up.ntiles, 192, 0, c->stream>>>(a);
         HALA_CHECK_LAUNCH();
     }
+    stencil_scatter<V, TOUT><<<g, 256, 0, c->stream>>>(np, sp.prow.as<int>(), up, r, gate);
+    HALA_CHECK_LAUNCH();
 }
 
 // tiles, tickets and packed positions of one triangle
@@ -591,12 +716,38 @@ void build_tri(hala_ilu_s *M, stencil_tri &T, const kind_info &K, bool up) {
     pa.pos = T.pos.as<int>();
     pa.dpos = up ? T.dpos.as<int>() : nullptr;
     pa.mask = T.mask.as<unsigned short>();
+    if (!up) {
+        sp.prow.reserve((size_t)acc * sizeof(int));
+        pa.prow = sp.prow.as<int>();
+    }
     dim3 g((unsigned)((maxS * 64 + 255) / 256), (unsigned)ntiles);
     stencil_pack<<<g, 256, 0, c->stream>>>(pa);
     HALA_CHECK_LAUNCH();
 }
 
 }  // namespace
+
+// The packed vectors of one precision: input, L's output and U's output, both
+// outputs all-sentinel (each sweep re-arms the other's rows as it goes).
+static void packed_vectors(hala_ilu_s *M, bool f32) {
+    hala_ctx c = M->ctx;
+    stencil_plan &sp = *M->sp;
+    const int64_t np = sp.L.nprow;
+    const size_t vs = f32 ? sizeof(float) : sizeof(double);
+    dbuf &xp = f32 ? sp.xp32 : sp.xp64, &y = f32 ? sp.y32 : sp.y64, &u = f32 ? sp.u32 : sp.u64;
+    xp.reserve((size_t)np * vs);
+    y.reserve((size_t)np * vs);
+    u.reserve((size_t)np * vs);
+    const int g = grid_for(np, 1024, 8192);
+    if (f32) {
+        fill_sent<float><<<g, 256, 0, c->stream>>>(y.as<float>(), np);
+        fill_sent<float><<<g, 256, 0, c->stream>>>(u.as<float>(), np);
+    } else {
+        fill_sent<double><<<g, 256, 0, c->stream>>>(y.as<double>(), np);
+        fill_sent<double><<<g, 256, 0, c->stream>>>(u.as<double>(), np);
+    }
+    HALA_CHECK_LAUNCH();
+}
 
 bool stencil_build(hala_ilu_s *M) {
     hala_ctx c = M->ctx;
@@ -678,9 +829,7 @@ bool stencil_build(hala_ilu_s *M) {
         M->sp = std::move(sp);
         build_tri(M, M->sp->L, K, false);
         build_tri(M, M->sp->U, K, true);
-        M->sp->y64.reserve((size_t)n * sizeof(double));
-        fill_sent<double><<<grid_for(n, 1024, 4096), 256, 0, c->stream>>>(M->sp->y64.as<double>(), n);
-        HALA_CHECK_LAUNCH();
+        packed_vectors(M, false);
         return true;
     }
     return false;
@@ -699,8 +848,8 @@ void stencil_load(hala_ilu_s *M, bool f32) {
         auto fill = [&](dbuf &rec, auto zero) {
             using V = decltype(zero);
             const int rb = record_bytes(K.NS, up, (int)sizeof(V));
-            // the loader reads up to 8 records past a tile and 1 KiB past a record
-            rec.reserve((size_t)(nrec + 8) * rb + 1024);
+            // the streaming wave reads up to 16 records past a tile and 1 KiB past a record
+            rec.reserve((size_t)(nrec + 16) * rb + 1024);
             stencil_records<V><<<grid_for(nrec * 64, 1024, 8192), 256, 0, c->stream>>>(
                 nrec, K.NS, up ? 1 : 0, rb, T.pos.as<int>(), up ? T.dpos.as<int>() : nullptr,
                 T.mask.as<unsigned short>(), f, rec.as<unsigned char>());
@@ -711,11 +860,7 @@ void stencil_load(hala_ilu_s *M, bool f32) {
     };
     load(sp.L, false);
     load(sp.U, true);
-    if (f32 && !sp.y32.p) {
-        sp.y32.reserve((size_t)M->n * sizeof(float));
-        fill_sent<float><<<grid_for(M->n, 1024, 4096), 256, 0, c->stream>>>(sp.y32.as<float>(), M->n);
-        HALA_CHECK_LAUNCH();
-    }
+    if (f32 && !sp.y32.p) packed_vectors(M, true);
     sp.loaded64 = true;
     sp.loaded32 = f32;
 }

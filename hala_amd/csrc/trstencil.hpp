@@ -40,7 +40,10 @@ struct stencil_plan {
     int nx = 0, ny = 0, nz = 0, P = 0;
     int ntj = 0, ntk = 0;
     stencil_tri L, U;
-    dbuf y64, y32;  // L's natural-order output (U's input); sentinels between applies
+    dbuf prow;  // natural row of each packed row (L order), -1 for padding
+    // packed vectors (FP64 / FP32): the input, L's output (U's input) and U's
+    // output; the outputs hold sentinels between applies
+    dbuf xp64, y64, u64, xp32, y32, u32;
     bool loaded64 = false, loaded32 = false;
 };
 
