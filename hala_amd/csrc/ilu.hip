@@ -316,8 +316,8 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
             if (uerr) std::rethrow_exception(uerr);
             M->factor.reserve((size_t)nnz * sizeof(double));
             M->tmp64.reserve((size_t)n * sizeof(double));
-            // a stencil pattern also gets the pencil schedule of its sweeps
-            if (!stencil_build(M)) M->sp.reset();
+            // with option tri_stencil, a stencil pattern also gets the pencil schedule of its sweeps
+            if (c->opt.tri_stencil && !stencil_build(M)) M->sp.reset();
         } catch (...) {
             delete M;
             throw;

@@ -165,10 +165,12 @@ int hala_ilu_info(hala_ilu ilu, hala_ilu_info_t *info);
  * slab of planes (1: contiguous row blocks), rows per plane (0: not split)}. */
 int hala_ilu_sweep_info(hala_ilu ilu, int upper, int *out);
 /* Whether the sweeps of the ILU(0) apply run on the pencil schedule of a
- * stencil pattern (no reference counterpart).  out[5] = {stencil kind (0: the
- * pattern is not a supported stencil, 1: 2-D 5-point, 2: 2-D 9-point, 3: 3-D
- * 7-point, 4: 3-D 27-point), nx, ny, nz, in use (the context option
- * "tri_stencil" is on)}. */
+ * stencil pattern (no reference counterpart).  The pattern is examined when
+ * the ILU object is created with the context option "tri_stencil" set to 1
+ * (default 0: the general sweeps, no stencil plan).  out[5] = {stencil kind
+ * (0: not examined, or not a supported stencil, 1: 2-D 5-point, 2: 2-D
+ * 9-point, 3: 3-D 7-point, 4: 3-D 27-point), nx, ny, nz, in use (the option
+ * is on now)}. */
 int hala_ilu_stencil_info(hala_ilu ilu, int *out);
 int hala_ilu_factor_f64(hala_ilu ilu, const double *vals, int want_f32);
 int hala_ilu_apply_f64(hala_ilu ilu, const double *x, double *r);
