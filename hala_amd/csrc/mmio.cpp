@@ -63,15 +63,15 @@ int read_header(FILE *f, const char *path, mm_header &h) {
         if (!std::fgets(line, sizeof line, f)) return fail(HALA_ERR_ARG, "Matrix Market size line missing");
     } while (line[0] == '%' || line[strspn(line, " \t\r\n")] == '\0');
     long long r = 0, c = 0, z = 0;
+    const int nread = h.array ? std::sscanf(line, "%lld %lld", &r, &c) : std::sscanf(line, "%lld %lld %lld", &r, &c, &z);
+    if (nread != (h.array ? 2 : 3)) return fail(HALA_ERR_ARG, "bad Matrix Market size line");
+    // the dimensions first: the array format's entry count is formed from them
+    if (r < 0 || c < 0 || r > 0x7fffffff || c > 0x7fffffff) return fail(HALA_ERR_ARG, "Matrix Market sizes out of range");
     if (h.array) {
-        if (std::sscanf(line, "%lld %lld", &r, &c) != 2) return fail(HALA_ERR_ARG, "bad Matrix Market size line");
         if (h.symmetry == HALA_MM_GENERAL) z = r * c;
-        else z = (h.symmetry == HALA_MM_SYMMETRIC) ? r * (r + 1) / 2 : r * (r - 1) / 2;
-    } else if (std::sscanf(line, "%lld %lld %lld", &r, &c, &z) != 3) {
-        return fail(HALA_ERR_ARG, "bad Matrix Market size line");
+        else z = (h.symmetry == HALA_MM_SYMMETRIC) ? r * (r + 1) / 2 : (r > 0 ? r * (r - 1) / 2 : 0);
     }
-    if (r < 0 || c < 0 || z < 0 || r > 0x7fffffff || c > 0x7fffffff || z >= 0x7fffffff)
-        return fail(HALA_ERR_ARG, "Matrix Market sizes out of range");
+    if (z < 0 || z >= 0x7fffffff) return fail(HALA_ERR_ARG, "Matrix Market sizes out of range");
     if (h.symmetry != HALA_MM_GENERAL && r != c) return fail(HALA_ERR_ARG, "symmetric Matrix Market file not square");
     h.nrows = (int)r;
     h.ncols = (int)c;
@@ -150,18 +150,21 @@ int hala_mm_read_coo(const char *path, int expand, int64_t capacity, int *rows, 
                 ai = h.symmetry == HALA_MM_GENERAL ? 0 : (h.symmetry == HALA_MM_SYMMETRIC ? aj : aj + 1);
             }
         } else {
-            i = std::strtoll(p, &e, 10) - 1;
+            // 1-based indices, range-checked before the shift (strtoll saturates)
+            i = std::strtoll(p, &e, 10);
             if (e == p) return fail(HALA_ERR_ARG, "bad Matrix Market entry " + std::to_string(k + 1));
             p = e;
-            j = std::strtoll(p, &e, 10) - 1;
+            j = std::strtoll(p, &e, 10);
             if (e == p) return fail(HALA_ERR_ARG, "bad Matrix Market entry " + std::to_string(k + 1));
             p = e;
             if (h.field != HALA_MM_PATTERN) {
                 v = std::strtod(p, &e);
                 if (e == p) return fail(HALA_ERR_ARG, "bad Matrix Market entry " + std::to_string(k + 1));
             }
-            if (i < 0 || j < 0 || i >= h.nrows || j >= h.ncols)
+            if (i < 1 || j < 1 || i > h.nrows || j > h.ncols)
                 return fail(HALA_ERR_ARG, "Matrix Market entry " + std::to_string(k + 1) + " out of range");
+            i -= 1;
+            j -= 1;
         }
         k++;
         if (!put(i, j, v)) return fail(HALA_ERR_ARG, "hala_mm_read_coo: capacity too small");
@@ -175,7 +178,7 @@ int hala_mm_read_coo(const char *path, int expand, int64_t capacity, int *rows, 
 }
 
 int hala_mm_write_csr(const char *path, int nrows, int ncols, const int *pntr, const int *indx, const double *vals) {
-    if (!path || !pntr || (pntr[nrows] > 0 && (!indx || !vals)) || nrows < 0 || ncols < 0)
+    if (!path || !pntr || nrows < 0 || ncols < 0 || (pntr[nrows] > 0 && (!indx || !vals)))
         return fail(HALA_ERR_ARG, "hala_mm_write_csr: bad argument");
     FILE *f = std::fopen(path, "w");
     if (!f) return fail(HALA_ERR_ARG, std::string("cannot create ") + path);
