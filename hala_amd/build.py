@@ -60,7 +60,9 @@ def build(verbose=True, jobs=None):
     if failed:
         raise RuntimeError("libhala_amd: compilation failed")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-Wl,--no-undefined"]
+    # roctx: one trace range per C-ABI entry point (rocprofv3 --marker-trace)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
+        "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,--no-undefined"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         print(r.stderr, file=sys.stderr)

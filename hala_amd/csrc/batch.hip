@@ -222,6 +222,7 @@ using namespace hala_amd;
 extern "C" {
 
 int hala_batch_create(hala_ctx c, int nsys, const int *row_off, const int *pntr, const int *indx, hala_batch *out) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(c && out && nsys > 0 && row_off && pntr && indx, HALA_ERR_ARG, "hala_batch_create: bad argument");
         *out = nullptr;
@@ -316,6 +317,7 @@ int hala_batch_get_shape(hala_batch B, int *team, int *wg_per_cu) {
 
 int hala_batch_pcg_f64(hala_batch B, const double *vals, const double *dvals, const double *b, double *x,
                        double rtol, int max_iter, int *iters) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(B && vals && dvals && b && x, HALA_ERR_ARG, "hala_batch_pcg_f64: null argument");
         HALA_REQUIRE(max_iter >= 2, HALA_ERR_ARG, "hala_batch_pcg_f64: max_iter must be >= 2");
@@ -332,6 +334,7 @@ int hala_batch_pcg_f64(hala_batch B, const double *vals, const double *dvals, co
 // keeps the fastest.  ms_out (host, nullable) receives its time.
 int hala_batch_tune(hala_batch B, const double *vals, const double *dvals, const double *b, double *x, double rtol,
                     int max_iter, int reps, int *team_out, int *wg_out, double *ms_out) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(B && vals && dvals && b && x && reps >= 1, HALA_ERR_ARG, "hala_batch_tune: bad argument");
         hala_ctx c = B->ctx;

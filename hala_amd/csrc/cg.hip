@@ -424,6 +424,7 @@ extern "C" {
 
 int hala_cg_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag, const double *b,
                 double *x, double tol, int max_iter, int *iterations, double *hist, int max_hist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && b && x, HALA_ERR_ARG, "hala_cg_f64: null argument");
         HALA_REQUIRE(prec >= 0 && prec <= 2, HALA_ERR_ARG, "hala_cg_f64: prec must be 0, 1 or 2");
@@ -442,6 +443,7 @@ __global__ void permute_vals(int64_t nnz, const int *__restrict__ tmap, const do
 
 int hala_cg_t_f64(hala_csr A, const double *vals, hala_ilu ilu, const double *b, double *x, double tol,
                   int max_iter, int *iterations, double *hist, int max_hist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && ilu && b && x, HALA_ERR_ARG, "hala_cg_t_f64: null argument");
         HALA_REQUIRE(ilu->factored && ilu->n == A->nrows, HALA_ERR_ARG, "hala_cg_t_f64: ILU not factored / wrong size");

@@ -309,6 +309,7 @@ extern "C" {
 
 int hala_cg_batch_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, int nrhs, const double *B, double *X,
                       double tol, int max_iter, int *iterations, double *hist, int max_hist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && B && X, HALA_ERR_ARG, "hala_cg_batch_f64: null argument");
         HALA_REQUIRE(prec == 0 || (prec == 1 && ilu), HALA_ERR_ARG,

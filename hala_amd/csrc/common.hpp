@@ -6,6 +6,7 @@
 // per-row sums this makes every FP64 result reproducible run to run.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -44,6 +45,16 @@ struct status_error {
     } while (0)
 
 #define HALA_CHECK_LAUNCH() HALA_HIP(hipGetLastError())
+
+// A roctx range over one entry point of the C ABI (named by the entry point),
+// so a trace groups the kernels of each call: rocprofv3 --marker-trace.
+struct trace_range {
+    explicit trace_range(const char *name) { roctxRangePushA(name); }
+    ~trace_range() { roctxRangePop(); }
+    trace_range(const trace_range &) = delete;
+    trace_range &operator=(const trace_range &) = delete;
+};
+#define HALA_RANGE() ::hala_amd::trace_range hala_range_(__func__)
 
 // Runs body, converting exceptions to status codes and the last-error string.
 template <class F>

@@ -436,6 +436,7 @@ extern "C" {
 
 int hala_gmres_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *b, double *x, double tol,
                    int restart, int max_outer, int *total, double *hist, int max_hist, int *nhist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && b && x, HALA_ERR_ARG, "hala_gmres_f64: null argument");
         HALA_REQUIRE(prec == 0 || (prec == 1 && ilu), HALA_ERR_ARG, "hala_gmres_f64: prec 0 (identity) or 1 (ILU)");
@@ -452,6 +453,7 @@ int hala_gmres_f64(hala_csr A, const double *vals, int prec, hala_ilu ilu, const
 int hala_gmres_ir_f64(hala_csr A, const double *vals, const float *vals32, hala_ilu ilu, const double *b, double *x,
                       double rtol, double inner_rtol, int restart, int max_refine, int *total, double *hist,
                       int max_hist, int *nhist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && ilu && b && x, HALA_ERR_ARG, "hala_gmres_ir_f64: null argument");
         HALA_REQUIRE(ilu->factored && (!vals32 || ilu->has32), HALA_ERR_ARG,

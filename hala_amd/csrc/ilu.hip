@@ -257,6 +257,7 @@ static void ilu_pairs(hala_ctx c, hala_ilu M) {
 }
 
 int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *indx, hala_ilu *out) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(c && out, HALA_ERR_ARG, "hala_ilu_create: null argument");
         *out = nullptr;
@@ -377,6 +378,7 @@ int hala_ilu_sweep_info(hala_ilu M, int upper, int *out) {
 }
 
 int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(M && vals, HALA_ERR_ARG, "hala_ilu_factor_f64: null argument");
         hala_ctx c = M->ctx;
@@ -418,6 +420,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
 }
 
 int hala_ilu_apply_f64(hala_ilu M, const double *x, double *r) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f64: null argument");
         set_device(M->ctx);
@@ -429,6 +432,7 @@ int hala_ilu_apply_f64(hala_ilu M, const double *x, double *r) {
 }
 
 int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f32: null argument");
         HALA_REQUIRE(M->factored && M->has32, HALA_ERR_ARG, "FP32 ILU apply needs want_f32 at factorisation");
@@ -446,6 +450,7 @@ int hala_ilu_apply_f32(hala_ilu M, const float *x, float *r) {
 }
 
 int hala_ilu_apply_f32x64(hala_ilu M, const double *x, double *r) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_f32x64: null argument");
         set_device(M->ctx);
@@ -455,6 +460,7 @@ int hala_ilu_apply_f32x64(hala_ilu M, const double *x, double *r) {
 }
 
 int hala_ilu_apply_t_f64(hala_ilu M, const double *x, double *r) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(M && x && r, HALA_ERR_ARG, "hala_ilu_apply_t_f64: null argument");
         set_device(M->ctx);

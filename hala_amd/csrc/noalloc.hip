@@ -57,6 +57,7 @@ extern "C" {
 int hala_cg_f64_noalloc(hala_csr A, const double *vals, int prec, hala_ilu ilu, const double *pdiag,
                         const double *b, double *x, double tol, int max_iter, int *iterations, double *hist,
                         int max_hist) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A && vals && b && x, HALA_ERR_ARG, "hala_cg_f64_noalloc: null argument");
         HALA_REQUIRE(prec >= 0 && prec <= 2, HALA_ERR_ARG, "hala_cg_f64_noalloc: prec must be 0, 1 or 2");

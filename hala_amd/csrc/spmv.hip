@@ -341,6 +341,7 @@ int hala_csr_validate(hala_csr A, int require_diag, int *bad_row) {
 
 int hala_spmv_f64(hala_csr A, char trans, double alpha, const double *vals, const double *x, double beta,
                   double *y) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A, HALA_ERR_ARG, "spmv: null matrix");
         HALA_REQUIRE((vals || A->nnz == 0) && x && y, HALA_ERR_ARG, "spmv: null pointer");
@@ -353,6 +354,7 @@ int hala_spmv_f64(hala_csr A, char trans, double alpha, const double *vals, cons
 }
 
 int hala_spmv_f32(hala_csr A, char trans, float alpha, const float *vals, const float *x, float beta, float *y) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A, HALA_ERR_ARG, "spmv: null matrix");
         HALA_REQUIRE((vals || A->nnz == 0) && x && y, HALA_ERR_ARG, "spmv: null pointer");
@@ -366,6 +368,7 @@ int hala_spmv_f32(hala_csr A, char trans, float alpha, const float *vals, const 
 
 int hala_spmm_f64(hala_csr A, char transa, char transb, int b_rows, int b_cols, double alpha, const double *vals,
                   const double *B, int ldb, double beta, double *C, int ldc) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A, HALA_ERR_ARG, "spmm: null matrix");
         set_device(A->ctx);
@@ -380,6 +383,7 @@ int hala_spmm_f64(hala_csr A, char transa, char transb, int b_rows, int b_cols, 
 
 int hala_spmm_f32(hala_csr A, char transa, char transb, int b_rows, int b_cols, float alpha, const float *vals,
                   const float *B, int ldb, float beta, float *C, int ldc) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A, HALA_ERR_ARG, "spmm: null matrix");
         set_device(A->ctx);
@@ -393,6 +397,7 @@ int hala_spmm_f32(hala_csr A, char transa, char transb, int b_rows, int b_cols, 
 }
 
 int hala_spmv_f32x64(hala_csr A, double alpha, const float *vals, const double *x, double beta, double *y) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(A, HALA_ERR_ARG, "spmv: null matrix");
         HALA_REQUIRE((vals || A->nnz == 0) && x && y, HALA_ERR_ARG, "spmv: null pointer");

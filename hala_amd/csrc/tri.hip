@@ -90,6 +90,7 @@ extern "C" {
 
 int hala_tri_create(hala_ctx c, char uplo, char diag, int n, int64_t nnz, const int *pntr, const int *indx,
                     hala_tri *out) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(c && out, HALA_ERR_ARG, "hala_tri_create: null argument");
         *out = nullptr;
@@ -137,6 +138,7 @@ int hala_tri_create(hala_ctx c, char uplo, char diag, int n, int64_t nnz, const 
 }
 
 int hala_tri_destroy(hala_tri T) {
+    HALA_RANGE();
     return guard([&] {
         if (!T) return;
         (void)hipSetDevice(T->ctx->device);
@@ -149,6 +151,7 @@ int hala_tri_destroy(hala_tri T) {
 // (alpha*b_i - s) [/ diag]; the transposed sums add the terms in the source-row
 // order of the reference's column sweep.
 int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, const double *b, double *x) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(T && vals && b && x, HALA_ERR_ARG, "hala_trsv_f64: null argument");
         hala_ctx c = T->ctx;
@@ -165,6 +168,7 @@ int hala_trsv_f64(hala_tri T, char trans, double alpha, const double *vals, cons
 
 // FP32 values and vectors, FP32 arithmetic in the same term order
 int hala_trsv_f32(hala_tri T, char trans, float alpha, const float *vals, const float *b, float *x) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(T && vals && b && x, HALA_ERR_ARG, "hala_trsv_f32: null argument");
         hala_ctx c = T->ctx;
@@ -189,6 +193,7 @@ int hala_trsv_f32(hala_tri T, char trans, float alpha, const float *vals, const 
 
 int hala_trsm_f64(hala_tri T, char transa, char transb, int nrhs, double alpha, const double *vals, double *B,
                   int ldb) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(T && vals, HALA_ERR_ARG, "hala_trsm_f64: null argument");
         set_device(T->ctx);
@@ -200,6 +205,7 @@ int hala_trsm_f64(hala_tri T, char transa, char transb, int nrhs, double alpha, 
 }
 
 int hala_trsm_f32(hala_tri T, char transa, char transb, int nrhs, float alpha, const float *vals, float *B, int ldb) {
+    HALA_RANGE();
     return guard([&] {
         HALA_REQUIRE(T && vals, HALA_ERR_ARG, "hala_trsm_f32: null argument");
         set_device(T->ctx);
