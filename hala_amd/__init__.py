@@ -5,7 +5,7 @@ Python mirror of the reference's engine/matrix/ILU/solver interface
 gpu/hala_gpu_ilu.hpp, hex/solvers/*.hpp), used by the tests, the benchmark
 and the autograd harness.  Device memory comes from torch tensors; every
 computation is a call into libhala_amd.so through its C ABI.  The C++ drop-in
-for user code calling hala:: lives in include/hala/.
+for user code calling hala:: is include/hala.hpp and include/hala_solvers.hpp.
 """
 import ctypes as C
 

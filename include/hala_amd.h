@@ -1,6 +1,6 @@
 /*
  * hala_amd.h -- C ABI of libhala_amd.so, the gfx950 (MI355X) sparse iterative
- * solve engine behind the hala:: templates in include/hala/.
+ * solve engine behind the hala:: templates of hala.hpp and hala_solvers.hpp.
  *
  * Plain C: device pointers, sizes and status codes only.  Every entry point
  * returns HALA_OK (0) or a negative status; hala_last_error() gives the
@@ -55,9 +55,18 @@ int hala_ctx_destroy(hala_ctx ctx);
 int hala_ctx_set_stream(hala_ctx ctx, void *stream);
 int hala_ctx_sync(hala_ctx ctx);
 int hala_ctx_device(hala_ctx ctx, int *device);
-/* Launch-shape options (no reference counterpart; the reference's rocSPARSE
- * calls pick their own): "tri_waves_per_cu", "tri_pool_waves", "tri_sleep",
- * "cg_batch_max".  Unknown names return HALA_ERR_ARG. */
+/* Options (no reference counterpart; the reference's rocSPARSE calls pick
+ * their own).  Launch shapes: "tri_waves_per_cu", "tri_pool_waves",
+ * "tri_sleep", "tri_streams", "tri_col_cols", "tri_col_split",
+ * "cg_batch_max", "spmv_blocks_per_cu".  Paths: "tri_col" (1: column-blocked
+ * sweeps where the plan fits, 0: level pool), "tri_stencil" (1: pencil sweeps
+ * for stencil patterns, read when the ILU object is created), "cg_fuse" (1:
+ * the sweeps' permutations fused into the PCG vector kernels).  Methods:
+ * "ilu32_jacobi", "ilu64_jacobi" (k > 0: k Jacobi sweeps per triangle instead
+ * of the exact apply, inside GMRES and the standalone applies).  Errors:
+ * "sync_checks" (1: standalone sweeps wait for the device and report a
+ * dependency wait that gave up; 0: the report is deferred to the next
+ * synchronising call).  Unknown names return HALA_ERR_ARG. */
 int hala_ctx_set_option(hala_ctx ctx, const char *name, int value);
 int hala_ctx_get_option(hala_ctx ctx, const char *name, int *value);
 
