@@ -506,8 +506,8 @@ def main():
         "iters_per_s": round((it - 1) / float(np.mean(solves)), 2),
         "final_relres": relres,
         "roofline": {"kernel": "ILU(0) triangular sweep, one triangle = half an apply: col_gather_in + "
-                               "tri_col_kernel<f64> (L) + col_gather_slots + tri_col_kernel<f64> (U) + "
-                               "col_scatter_out",
+                               "tri_col_kernel<f64> (L, also storing U's slot-order input) + "
+                               "tri_col_kernel<f64> (U) + col_scatter_out",
                      "bound": "hbm", "achieved": round(ach_sweep, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach_sweep / HBM_PEAK_GBS, 4), "traffic": traffic.get("tri_sweep_bytes"),
                      "bytes_per_launch": by_sweep, "avg_launch_ms": round(ms_sweep, 5),

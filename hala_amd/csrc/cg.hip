@@ -113,8 +113,7 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
 // x += a p; r += (-a) Ap; <r,r> (and <r, r./d> for Jacobi).  The last block
 // takes the stop decision and, for identity/Jacobi, the new beta.
 // lslot (ILU on the column layout, may be null): r also goes straight into the
-// L sweep's slot-order input (the sweep's gather pass, fused here; L's polled
-// output was re-armed by the previous apply's hand-off to U).
+// L sweep's slot-order input (the sweep's gather pass, fused here).
 template <int PREC>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__restrict__ p,
                                                         const double *__restrict__ ap, const double *__restrict__ d,
@@ -366,8 +365,8 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
             if (fused) {  // the sweeps' gather and scatter passes fused into the update and r.z
                 tri_plan &L = P.M->L, &U = P.M->U;
                 cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
-                                                                      wait_error_flag(c), st,
-                                                                      L.slot_of_row.as<int>(), L.col.xs64.as<double>());
+                                                                      wait_error_flag(c), st, L.slot_of_row.as<int>(),
+                                                                      L.col.xs64.as<double>());
                 ilu_apply64_mid(P.M, gate);
                 cg_zscatter_rz_kernel<<<ge, 256, 0, c->stream>>>(n, U.slot_of_row.as<int>(), U.out64.as<double>(), r, z,
                                                                  part, wait_error_flag(c), st);

@@ -156,14 +156,15 @@ __global__ void set_int(int *p, int v) { *p = v; }
 static bool use_stencil(hala_ilu M) { return M->sp && M->ctx->opt.tri_stencil; }
 
 bool ilu_fused64(hala_ilu M) {
-    return M->factored && !use_stencil(M) && col_usable(M->ctx, M->L, true) && col_usable(M->ctx, M->U, true);
+    return M->factored && !use_stencil(M) && col_usable(M->ctx, M->L, true) && col_usable(M->ctx, M->U, true) &&
+           M->L.col.hand64;
 }
 
 void ilu_apply64_mid(hala_ilu M, const int *gate) {
     hala_ctx c = M->ctx;
-    col_sweep64(c, M->L, TRI_SUB, 1.0, gate);
-    col_handoff64(c, M->U, M->L, gate);
-    col_sweep64(c, M->U, TRI_SUB, 1.0, gate);
+    // L's helpers re-arm U's polled output, U's helpers L's for the next apply
+    col_sweep64(c, M->L, TRI_SUB, 1.0, gate, &M->U, &M->U);
+    col_sweep64(c, M->U, TRI_SUB, 1.0, gate, nullptr, &M->L);
 }
 
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
@@ -174,9 +175,8 @@ void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate) {
         return;
     }
     if (ilu_fused64(M)) {  // L's output handed to U in slot order
-        tri_col_exec<double, double>(c, M->L, TRI_SUB, 1.0, x, nullptr, gate);
-        col_handoff64(c, M->U, M->L, gate);
-        col_sweep64(c, M->U, TRI_SUB, 1.0, gate);
+        tri_col_exec<double, double>(c, M->L, TRI_SUB, 1.0, x, nullptr, gate, &M->U);
+        col_sweep64(c, M->U, TRI_SUB, 1.0, gate, nullptr, &M->L);
         col_scatter64(c, M->U, r, gate);
         return;
     }
@@ -408,6 +408,7 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
                      "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
         load_values64(c, M->L, f);
         load_values64(c, M->U, f);
+        col_set_handoff(c, M->L, M->U);
         if (M->sp) stencil_load(M, want_f32);
         if (want_f32) {
             load_values32(c, M->L, f);

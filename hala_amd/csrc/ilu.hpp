@@ -56,9 +56,10 @@ void tri_trsm_cols(hala_tri T, char transb, int nrhs, size_t esize, void *B, int
 // gate (nullable, device flag): skip the apply when *gate != 0 (solver stop flag)
 void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 // Both triangles on the column layout: the apply's ends can be fused by the
-// caller.  ilu_apply64_mid runs L from its gathered input (col.xs64 filled and
-// out64 re-armed by the caller), the hand-off and U; the result stays in
-// U.out64 in slot order.
+// caller.  ilu_apply64_mid runs L from its gathered input (L's col.xs64 filled
+// by the caller; the polled outputs were re-armed by the previous apply's
+// sweeps), which hands its results to U's input itself, then U; the result
+// stays in U.out64 in slot order.
 bool ilu_fused64(hala_ilu M);
 void ilu_apply64_mid(hala_ilu M, const int *gate);
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);

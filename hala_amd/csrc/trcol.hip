@@ -576,6 +576,7 @@ void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
     const int kcp = q.K * q.CP, xc = WAVE * q.EL;
     dbuf &dst = f32 ? q.val32 : q.val64;
     dst.reserve((size_t)npad * RB);
+    if (!f32) q.hand64 = false;  // rewritten below: hand-off slots gone
     const int g = (npad + 255) / 256;
     if (f32) {
         col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, kcp, xc, q.code.as<int>(), q.spar.as<int>(),
@@ -611,25 +612,6 @@ __global__ void col_gather_in(int nslots, const int *__restrict__ chunk_row, con
 }
 
 // After a column sweep: the slot-ordered result into natural order.
-// The input of plan `dst` straight from the polled output of plan `src` (the
-// natural-order hand-off vector never materialises), re-arming dst's output
-// and, once read, src's (every row of src is read once here).
-template <class T>
-__global__ void col_gather_slots(int nslots, const int *__restrict__ chunk_row, T *__restrict__ src,
-                                 const int *__restrict__ src_slot, T *__restrict__ xs, T *__restrict__ out,
-                                 const int *gate) {
-    if (gate && *gate) return;
-    const int s = xcd_block((nslots + 255) / 256) * 256 + threadIdx.x;
-    if (s >= nslots) return;
-    out[s] = sentinel<T>();
-    const int r = chunk_row[s];
-    if (r >= 0) {
-        const int q = src_slot[r];
-        xs[s] = src[q];
-        src[q] = sentinel<T>();
-    }
-}
-
 template <class T, class TOUT>
 __global__ void col_scatter_out(int n, const int *__restrict__ slot_of_row, const T *__restrict__ out,
                                 TOUT *__restrict__ out_nat, const int *gate) {
@@ -647,6 +629,10 @@ struct col_args {
     const T *xs;  // input in slot order
     T alpha;
     T *out;
+    T *hand;              // HAND: the next sweep's input, at the slot each record carries (col_set_handoff)
+    int ncol;             // columns; workgroups past them are helpers
+    T *arm;               // helpers: re-arm arm[0 .. narm) with sentinels (another sweep's polled output)
+    int narm;
     unsigned *ticket;  // [0] column ticket, [1] workgroups done
     int *err;
     const int *gate;
@@ -731,8 +717,18 @@ __device__ __forceinline__ void lds_barrier() {
 // the same memory operations (idle lanes read valid padding), and the level
 // count is padded to a multiple of P with empty levels, so both loop bodies
 // are straight-line code.
-template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int PE, int EL, int BT>
+//
+// HAND (unit triangles only: the record's diagonal field holds the row's slot
+// in the next sweep's input instead): each result is also stored into the
+// next sweep's input, so that it needs no gather pass of its own.  (Rows of
+// one cell and level lie close together in that order; a store in natural
+// order would touch a line per lane on the critical path, so the natural-order
+// result stays a pass of its own.)  Workgroups past the columns (their
+// tickets come after every column's) are helpers: they re-arm another sweep's
+// polled output on CUs the columns leave idle.
+template <class T, class TOUT, int MODE, bool UNIT, bool HAND, int MAXD, int P, int PE, int EL, int BT>
 __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
+    static_assert(!HAND || UNIT, "the hand-off slot takes the diagonal's place");
     static_assert(P % PE == 0, "poll ring must divide the record ring");
     constexpr int XC = WAVE * EL;  // external values per level
     if (a.gate && *a.gate) return;
@@ -751,6 +747,19 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     }
     __syncthreads();
     const int col = col_s;
+    if (col >= a.ncol) {  // helper
+        const int nh = (int)gridDim.x - a.ncol;
+        for (int s = (col - a.ncol) * (int)blockDim.x + t; s < a.narm; s += nh * (int)blockDim.x)
+            a.arm[s] = sentinel<T>();
+        if (t == 0) {
+            unsigned d = atomicAdd(a.ticket + 1, 1u);
+            if (d == gridDim.x - 1) {
+                st_agent(a.ticket, 0u);
+                st_agent(a.ticket + 1, 0u);
+            }
+        }
+        return;
+    }
     const int first = a.colrange[2 * col], last = a.colrange[2 * col + 1];
 
     const int nl = last - first + 1;
@@ -903,6 +912,9 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
                 if (valid)
                     st_agent(reinterpret_cast<T *>(reinterpret_cast<char *>(a.out) + (unsigned)r.slot * (unsigned)sizeof(T)),
                              acc);
+                if (HAND && valid)
+                    *reinterpret_cast<T *>(reinterpret_cast<char *>(a.hand) +
+                                           (unsigned)rec_word(r.q, MAXD / 2) * (unsigned)sizeof(T)) = acc;
                 load_rec_at(rec[s], nslot0, nsize);
                 lds_barrier();
             }
@@ -917,23 +929,26 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     }
 }
 
-template <class T, class TOUT, int MODE, bool UNIT, int MAXD, int P, int EL>
+constexpr int ARM_HELPERS = 64;  // helper workgroups of a sweep that re-arms another sweep's output
+
+template <class T, class TOUT, int MODE, bool UNIT, bool HAND, int MAXD, int P, int EL>
 void launch_col_el(hala_ctx c, const tri_colplan &q, col_args<T, TOUT> &a, size_t lds) {
     constexpr int PE = poll_ahead(MAXD);
     static_assert(PE <= PE_MAX, "LDS sized for PE_MAX polls in flight");
     constexpr int BT = (MAXD == 4 ? CPMAX : CPMAX_LONG) + WAVE;
-    tri_col_kernel<T, TOUT, MODE, UNIT, MAXD, P, PE, EL, BT><<<dim3(q.ncol), dim3(q.CP + WAVE), lds, c->stream>>>(a);
+    const int nwg = q.ncol + (a.narm > 0 ? ARM_HELPERS : 0);
+    tri_col_kernel<T, TOUT, MODE, UNIT, HAND, MAXD, P, PE, EL, BT><<<dim3(nwg), dim3(q.CP + WAVE), lds, c->stream>>>(a);
 }
 
-template <class T, class TOUT, int MODE, bool UNIT>
+template <class T, class TOUT, int MODE, bool UNIT, bool HAND = false>
 void launch_col(hala_ctx c, tri_plan &p, col_args<T, TOUT> &a, size_t lds) {
     const tri_colplan &q = p.col;
     if (q.L == 4) {
-        if (q.EL == 2) launch_col_el<T, TOUT, MODE, UNIT, 4, 8, 2>(c, q, a, lds);
-        else launch_col_el<T, TOUT, MODE, UNIT, 4, 8, 4>(c, q, a, lds);
+        if (q.EL == 2) launch_col_el<T, TOUT, MODE, UNIT, HAND, 4, 8, 2>(c, q, a, lds);
+        else launch_col_el<T, TOUT, MODE, UNIT, HAND, 4, 8, 4>(c, q, a, lds);
     } else {
-        if (q.EL == 4) launch_col_el<T, TOUT, MODE, UNIT, 16, 4, 4>(c, q, a, lds);
-        else launch_col_el<T, TOUT, MODE, UNIT, 16, 4, 8>(c, q, a, lds);
+        if (q.EL == 4) launch_col_el<T, TOUT, MODE, UNIT, HAND, 16, 4, 4>(c, q, a, lds);
+        else launch_col_el<T, TOUT, MODE, UNIT, HAND, 16, 4, 8>(c, q, a, lds);
     }
 }
 
@@ -951,9 +966,12 @@ bool col_usable(hala_ctx c, const tri_plan &p, bool is64) {
 
 namespace {
 
-// the column kernel alone: input already in slot order in xs, output left in slot order
+// The column kernel alone: input already in slot order in xs; the result in
+// slot order in the polled output and, with next (FP64, TRI_SUB, unit p with
+// hand-off slots), in next's slot-order input; arm (nullable, narm slots) is
+// re-armed by helper workgroups.
 template <class T, class TOUT>
-void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate) {
+void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate, tri_plan *next, T *arm, int narm) {
     constexpr bool is64 = sizeof(T) == 8;
     tri_colplan &q = p.col;
     col_args<T, TOUT> a;
@@ -967,12 +985,20 @@ void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate) {
     a.xs = is64 ? reinterpret_cast<T *>(q.xs64.p) : reinterpret_cast<T *>(q.xs32.p);
     a.alpha = alpha;
     a.out = is64 ? reinterpret_cast<T *>(p.out64.p) : reinterpret_cast<T *>(p.out32.p);
-
+    a.hand = nullptr;
+    a.ncol = q.ncol;
+    a.arm = arm;
+    a.narm = arm ? narm : 0;
     a.ticket = counters(c) + 104;
     a.err = wait_error_flag(c);
     a.gate = gate;
     const size_t lds = (size_t)lds_bytes(q.K, q.CP, q.EL, q.nlevcol, sizeof(T));
-    if (mode == TRI_SUB) {
+    if (next) {
+        // the records carry next's input slots (FP64 records of a unit triangle)
+        HALA_REQUIRE(is64 && mode == TRI_SUB && p.unit && q.hand64, HALA_ERR_ARG, "column sweep: no hand-off slots");
+        a.hand = reinterpret_cast<T *>(next->col.xs64.p);
+        if constexpr (is64) launch_col<T, TOUT, TRI_SUB, true, true>(c, p, a, lds);
+    } else if (mode == TRI_SUB) {
         if (p.unit) launch_col<T, TOUT, TRI_SUB, true>(c, p, a, lds);
         else launch_col<T, TOUT, TRI_SUB, false>(c, p, a, lds);
     } else {
@@ -985,7 +1011,8 @@ void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate) {
 }  // namespace
 
 template <class T, class TOUT>
-void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate) {
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate,
+                  tri_plan *next) {
     constexpr bool is64 = sizeof(T) == 8;
     tri_colplan &q = p.col;
     const int nslots = p.nchunk * WAVE;
@@ -994,7 +1021,13 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     col_gather_in<T, TOUT><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(nslots, p.chunk_row.as<int>(), in,
                                                                                  xs, out, gate);
     HALA_CHECK_LAUNCH();
-    col_sweep<T, TOUT>(c, p, mode, alpha, gate);
+    T *arm = nullptr;
+    int narm = 0;
+    if (next) {  // next's polled output re-armed by this sweep's helpers
+        arm = is64 ? reinterpret_cast<T *>(next->out64.p) : reinterpret_cast<T *>(next->out32.p);
+        narm = next->nchunk * WAVE;
+    }
+    col_sweep<T, TOUT>(c, p, mode, alpha, gate, next, arm, narm);
     if (out_nat) {
         col_scatter_out<T, TOUT><<<xcd_grid((p.n + 255) / 256), 256, 0, c->stream>>>(p.n, p.slot_of_row.as<int>(), out,
                                                                                     out_nat, gate);
@@ -1002,16 +1035,9 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     }
 }
 
-void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate) {
-    col_sweep<double, double>(c, p, mode, alpha, gate);
-}
-
-void col_handoff64(hala_ctx c, tri_plan &dst, tri_plan &src, const int *gate) {
-    const int nslots = dst.nchunk * WAVE;
-    col_gather_slots<double><<<xcd_grid((nslots + 255) / 256), 256, 0, c->stream>>>(
-        nslots, dst.chunk_row.as<int>(), src.out64.as<double>(), src.slot_of_row.as<int>(), dst.col.xs64.as<double>(),
-        dst.out64.as<double>(), gate);
-    HALA_CHECK_LAUNCH();
+void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate, tri_plan *next, tri_plan *arm) {
+    col_sweep<double, double>(c, p, mode, alpha, gate, next, arm ? arm->out64.as<double>() : nullptr,
+                              arm ? arm->nchunk * WAVE : 0);
 }
 
 void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate) {
@@ -1020,8 +1046,37 @@ void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate) {
     HALA_CHECK_LAUNCH();
 }
 
-template void tri_col_exec<double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, const int *);
-template void tri_col_exec<float, double>(hala_ctx, tri_plan &, int, float, const double *, double *, const int *);
-template void tri_col_exec<float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, const int *);
+// The slot of every row of `src` in the input of `dst` (a sink slot past
+// dst's slots for record lanes without a row), written into the diagonal field
+// of src's FP64 records, which a unit triangle does not read.
+__global__ void col_set_hand(int nslots, const int *__restrict__ chunk_row, const int *__restrict__ dst_slot,
+                             int sink, int RB, int word, unsigned char *__restrict__ rec) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int r = chunk_row[s];
+    int *w = reinterpret_cast<int *>(rec + (size_t)s * RB) + word;
+    w[0] = r >= 0 ? dst_slot[r] : sink;
+    w[1] = 0;
+}
+
+void col_set_handoff(hala_ctx c, tri_plan &src, tri_plan &dst) {
+    tri_colplan &q = src.col;
+    q.hand64 = false;
+    if (!q.ok || !dst.col.ok || !src.unit || !q.val64.p || !dst.col.xs64.p) return;
+    const int nslots = src.nchunk * WAVE;
+    col_set_hand<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, src.chunk_row.as<int>(),
+                                                             dst.slot_of_row.as<int>(), dst.nchunk * WAVE,
+                                                             rec_bytes(q.L, sizeof(double)), q.L / 2,
+                                                             q.val64.as<unsigned char>());
+    HALA_CHECK_LAUNCH();
+    q.hand64 = true;
+}
+
+template void tri_col_exec<double, double>(hala_ctx, tri_plan &, int, double, const double *, double *, const int *,
+                                           tri_plan *);
+template void tri_col_exec<float, double>(hala_ctx, tri_plan &, int, float, const double *, double *, const int *,
+                                          tri_plan *);
+template void tri_col_exec<float, float>(hala_ctx, tri_plan &, int, float, const float *, float *, const int *,
+                                         tri_plan *);
 
 }  // namespace hala_amd

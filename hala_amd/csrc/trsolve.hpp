@@ -49,6 +49,7 @@ struct tri_colplan {
     dbuf spar;                   // per slot: parity of its level within its column (which external buffer it reads)
     dbuf val64, val32;           // packed row records (see col_pack)
     dbuf xs64, xs32;             // input, gathered into slot order per sweep
+    bool hand64 = false;         // val64 carries hand-off slots (col_set_handoff)
 };
 
 struct tri_plan {
@@ -125,16 +126,25 @@ void tri_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TIN *in, TOUT *o
 bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req);
 void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32);
 bool col_usable(hala_ctx c, const tri_plan &p, bool is64);
+// next (nullable; FP64, TRI_SUB, unit p with hand-off slots, see
+// col_set_handoff): the sweep also writes next's input in next's slot order,
+// and its helper workgroups re-arm next's polled output.
 template <class T, class TOUT>
-void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate);
+void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TOUT *out_nat, const int *gate,
+                  tri_plan *next = nullptr);
 
 // Pieces of the FP64 column sweep for callers that fuse its ends: the column
-// kernel alone (input in slot order in col.xs64, output left in slot order in
-// out64); dst's input taken from src's slot-order output (re-arming both
-// outputs); the slot-order output scattered to natural order.
-void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate);
-void col_handoff64(hala_ctx c, tri_plan &dst, tri_plan &src, const int *gate);
+// kernel alone (input in slot order in col.xs64; the result in slot order in
+// out64 and, with next, in next's input as above; arm (nullable): a plan whose
+// polled output the sweep's helper workgroups re-arm meanwhile); the
+// slot-order output scattered to natural order.
+void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate, tri_plan *next = nullptr,
+                 tri_plan *arm = nullptr);
 void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate);
+// Writes into the FP64 records of the unit triangle src the slot of each of
+// its rows in dst's input (sets src.col.hand64; left unset when either plan
+// has no column layout).  Needed after every col_load_values of src.
+void col_set_handoff(hala_ctx c, tri_plan &src, tri_plan &dst);
 
 // Refills the polled output buffer(s) with sentinels (both precisions, or one).
 void tri_rearm(hala_ctx c, tri_plan &p);
