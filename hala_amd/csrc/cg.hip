@@ -113,7 +113,9 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
 // x += a p; r += (-a) Ap; <r,r> (and <r, r./d> for Jacobi).  The last block
 // takes the stop decision and, for identity/Jacobi, the new beta.
 // lslot (ILU on the column layout, may be null): r also goes straight into the
-// L sweep's slot-order input (the sweep's gather pass, fused here).
+// L sweep's slot-order input (the sweep's gather pass, fused here), and x is
+// left alone (null): the L sweep's helper workgroups update it meanwhile, and
+// cg_xfinal does the pass whose sweep the stop gated off.
 template <int PREC>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__restrict__ p,
                                                         const double *__restrict__ ap, const double *__restrict__ d,
@@ -131,8 +133,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
         int64_t i = ((int64_t)blockIdx.x * EK + k) * CB + threadIdx.x;
         double rr = 0.0, rz = 0.0;
         if (i < n) {
-            double pi = p[i];
-            x[i] = x[i] + a * pi;
+            if (x) x[i] = x[i] + a * p[i];
             double ri = r[i] + (-a) * ap[i];
             r[i] = ri;
             if (lslot) lxs[lslot[i]] = ri;
@@ -261,6 +262,14 @@ __global__ void cg_pupdate_kernel(int n, const double *__restrict__ z, const dou
     }
 }
 
+// x += alpha p for the last pass of a fused ILU-PCG solve (whose L sweep, and
+// with it the helpers' update of x, the stop flag gated off)
+__global__ void cg_xfinal(int n, double *__restrict__ x, const double *__restrict__ p, const cg_state *st) {
+    const double a = st->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = x[i] + a * p[i];
+}
+
 __global__ void cg_gate_copy(int n, const int *done, const double *__restrict__ src, double *__restrict__ dst) {
     if (*done) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,10 +373,11 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
         default:
             if (fused) {  // the sweeps' gather and scatter passes fused into the update and r.z
                 tri_plan &L = P.M->L, &U = P.M->U;
-                cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2,
+                cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, nullptr, r, part, part2,
                                                                       wait_error_flag(c), st, L.slot_of_row.as<int>(),
                                                                       L.col.xs64.as<double>());
-                ilu_apply64_mid(P.M, gate);
+                const col_axpy ax{x, p, &st->alpha, n};  // x += alpha p on the L sweep's helper workgroups
+                ilu_apply64_mid(P.M, gate, &ax);
                 cg_zscatter_rz_kernel<<<ge, 256, 0, c->stream>>>(n, U.slot_of_row.as<int>(), U.out64.as<double>(), r, z,
                                                                  part, wait_error_flag(c), st);
             } else {
@@ -387,6 +397,10 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
         for (int k = 0; k < batch; k++) one_pass();
         cg_read(c, &it, &done, &status, &rn);
         if (batch < c->opt.cg_batch_max) batch = std::min(2 * batch, std::max(c->opt.cg_batch_max, 1));
+    }
+    if (fused) {  // the stopping pass's x update (its L sweep was gated off)
+        cg_xfinal<<<gp, 256, 0, c->stream>>>(n, x, p, st);
+        HALA_CHECK_LAUNCH();
     }
     check_wait_error(c);
     if (iterations) *iterations = it;

@@ -160,10 +160,10 @@ bool ilu_fused64(hala_ilu M) {
            M->L.col.hand64;
 }
 
-void ilu_apply64_mid(hala_ilu M, const int *gate) {
+void ilu_apply64_mid(hala_ilu M, const int *gate, const col_axpy *ax) {
     hala_ctx c = M->ctx;
     // L's helpers re-arm U's polled output, U's helpers L's for the next apply
-    col_sweep64(c, M->L, TRI_SUB, 1.0, gate, &M->U, &M->U);
+    col_sweep64(c, M->L, TRI_SUB, 1.0, gate, &M->U, &M->U, ax);
     col_sweep64(c, M->U, TRI_SUB, 1.0, gate, nullptr, &M->L);
 }
 

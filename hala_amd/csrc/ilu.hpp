@@ -59,9 +59,10 @@ void ilu_apply64(hala_ilu M, const double *x, double *r, const int *gate = nullp
 // caller.  ilu_apply64_mid runs L from its gathered input (L's col.xs64 filled
 // by the caller; the polled outputs were re-armed by the previous apply's
 // sweeps), which hands its results to U's input itself, then U; the result
-// stays in U.out64 in slot order.
+// stays in U.out64 in slot order.  ax (nullable): a vector update the L
+// sweep's helper workgroups do meanwhile (see col_sweep64).
 bool ilu_fused64(hala_ilu M);
-void ilu_apply64_mid(hala_ilu M, const int *gate);
+void ilu_apply64_mid(hala_ilu M, const int *gate, const hala_amd::col_axpy *ax = nullptr);
 void ilu_apply32x64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_apply_t64(hala_ilu M, const double *x, double *r, const int *gate = nullptr);
 void ilu_prepare_adjoint(hala_ilu M);

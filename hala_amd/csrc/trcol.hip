@@ -633,6 +633,7 @@ struct col_args {
     int ncol;             // columns; workgroups past them are helpers
     T *arm;               // helpers: re-arm arm[0 .. narm) with sentinels (another sweep's polled output)
     int narm;
+    col_axpy ax;          // helpers: ax.x += *ax.alpha * ax.p over ax.n rows (ax.x null: none)
     unsigned *ticket;  // [0] column ticket, [1] workgroups done
     int *err;
     const int *gate;
@@ -751,6 +752,11 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
         const int nh = (int)gridDim.x - a.ncol;
         for (int s = (col - a.ncol) * (int)blockDim.x + t; s < a.narm; s += nh * (int)blockDim.x)
             a.arm[s] = sentinel<T>();
+        if (a.ax.x) {
+            const double al = *a.ax.alpha;
+            for (int i = (col - a.ncol) * (int)blockDim.x + t; i < a.ax.n; i += nh * (int)blockDim.x)
+                a.ax.x[i] = a.ax.x[i] + al * a.ax.p[i];
+        }
         if (t == 0) {
             unsigned d = atomicAdd(a.ticket + 1, 1u);
             if (d == gridDim.x - 1) {
@@ -929,14 +935,14 @@ __global__ __launch_bounds__(BT) void tri_col_kernel(col_args<T, TOUT> a) {
     }
 }
 
-constexpr int ARM_HELPERS = 64;  // helper workgroups of a sweep that re-arms another sweep's output
+constexpr int ARM_HELPERS = 64;  // helper workgroups of a sweep with side work (re-arming, vector update)
 
 template <class T, class TOUT, int MODE, bool UNIT, bool HAND, int MAXD, int P, int EL>
 void launch_col_el(hala_ctx c, const tri_colplan &q, col_args<T, TOUT> &a, size_t lds) {
     constexpr int PE = poll_ahead(MAXD);
     static_assert(PE <= PE_MAX, "LDS sized for PE_MAX polls in flight");
     constexpr int BT = (MAXD == 4 ? CPMAX : CPMAX_LONG) + WAVE;
-    const int nwg = q.ncol + (a.narm > 0 ? ARM_HELPERS : 0);
+    const int nwg = q.ncol + (a.narm > 0 || a.ax.x ? ARM_HELPERS : 0);
     tri_col_kernel<T, TOUT, MODE, UNIT, HAND, MAXD, P, PE, EL, BT><<<dim3(nwg), dim3(q.CP + WAVE), lds, c->stream>>>(a);
 }
 
@@ -971,7 +977,8 @@ namespace {
 // hand-off slots), in next's slot-order input; arm (nullable, narm slots) is
 // re-armed by helper workgroups.
 template <class T, class TOUT>
-void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate, tri_plan *next, T *arm, int narm) {
+void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate, tri_plan *next, T *arm, int narm,
+               const col_axpy *ax = nullptr) {
     constexpr bool is64 = sizeof(T) == 8;
     tri_colplan &q = p.col;
     col_args<T, TOUT> a;
@@ -989,6 +996,7 @@ void col_sweep(hala_ctx c, tri_plan &p, int mode, T alpha, const int *gate, tri_
     a.ncol = q.ncol;
     a.arm = arm;
     a.narm = arm ? narm : 0;
+    a.ax = ax ? *ax : col_axpy{nullptr, nullptr, nullptr, 0};
     a.ticket = counters(c) + 104;
     a.err = wait_error_flag(c);
     a.gate = gate;
@@ -1035,9 +1043,10 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
     }
 }
 
-void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate, tri_plan *next, tri_plan *arm) {
+void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate, tri_plan *next, tri_plan *arm,
+                 const col_axpy *ax) {
     col_sweep<double, double>(c, p, mode, alpha, gate, next, arm ? arm->out64.as<double>() : nullptr,
-                              arm ? arm->nchunk * WAVE : 0);
+                              arm ? arm->nchunk * WAVE : 0, ax);
 }
 
 void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate) {

@@ -138,8 +138,17 @@ void tri_col_exec(hala_ctx c, tri_plan &p, int mode, T alpha, const TOUT *in, TO
 // out64 and, with next, in next's input as above; arm (nullable): a plan whose
 // polled output the sweep's helper workgroups re-arm meanwhile); the
 // slot-order output scattered to natural order.
+// ax (nullable): x[i] = x[i] + (*alpha) * p[i] over n rows, also done by the
+// helper workgroups (a solver's vector update riding on idle CUs; skipped,
+// like the sweep, when the gate is set).
+struct col_axpy {
+    double *x;
+    const double *p;
+    const double *alpha;
+    int n;
+};
 void col_sweep64(hala_ctx c, tri_plan &p, int mode, double alpha, const int *gate, tri_plan *next = nullptr,
-                 tri_plan *arm = nullptr);
+                 tri_plan *arm = nullptr, const col_axpy *ax = nullptr);
 void col_scatter64(hala_ctx c, tri_plan &p, double *out_nat, const int *gate);
 // Writes into the FP64 records of the unit triangle src the slot of each of
 // its rows in dst's input (sets src.col.hand64; left unset when either plan
