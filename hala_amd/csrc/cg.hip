@@ -17,7 +17,7 @@
 
 namespace hala_amd {
 
-constexpr int EK = 4;  // canonical chunks per block in the elementwise kernels
+constexpr int EK = 2;  // canonical chunks per block in the elementwise kernels (128^3 PCG: 1, 2, 4 -> 79.06, 78.85, 79.27 ms)
 
 enum { PREC_ID = 0, PREC_ILU = 1, PREC_JAC = 2, PREC_ILU_T = 3 };
 
@@ -332,7 +332,9 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     const double *vals = P.transposed ? P.tvals : P.vals;
 
     const int gs = (int)((n + SPMV_ROWS - 1) / SPMV_ROWS);
-    const int gsd = std::min(gs, c->num_cus * c->opt.spmv_blocks_per_cu);  // grid-stride SpMV+dot
+    // SpMV + dot: a workgroup per 256-row block, or grid-stride over them with the option's cap per CU
+    // (128^3 PCG: 8 per CU 80.6 ms, 16 79.1 ms, a workgroup per block 78.7 ms)
+    const int gsd = c->opt.spmv_blocks_per_cu > 0 ? std::min(gs, c->num_cus * c->opt.spmv_blocks_per_cu) : gs;
     const bool nt = stream_nt<double>(P.A->nnz);
     const int ge = (int)((nb + EK - 1) / EK);
     const int gp = grid_for(n, 256 * 4, 16384);

@@ -200,7 +200,7 @@ struct hala_options {
     int tri_pool_waves = 0;    // absolute pool size (overrides waves per CU when > 0)
     int tri_sleep = 1;         // s_sleep slots between polls of an unready dependency
     int cg_batch_max = 32;     // solver passes launched between stop-flag reads
-    int spmv_blocks_per_cu = 8;  // workgroups per CU of the grid-stride fused SpMV + dot
+    int spmv_blocks_per_cu = 0;  // fused SpMV + dot: workgroups per CU of a grid-stride launch (0: one per block)
     int tri_streams = 1;       // ticket counters of the triangular-solve pool (1..TK_MAX)
     int tri_col = 1;           // column-blocked triangular sweeps where the pattern allows (0: level pool only)
     int tri_col_cols = 0;      // columns of the blocked sweep (0: smallest that fits)

@@ -58,7 +58,9 @@ int hala_ctx_device(hala_ctx ctx, int *device);
 /* Options (no reference counterpart; the reference's rocSPARSE calls pick
  * their own).  Launch shapes: "tri_waves_per_cu", "tri_pool_waves",
  * "tri_sleep", "tri_streams", "tri_col_cols", "tri_col_split",
- * "cg_batch_max", "spmv_blocks_per_cu".  Paths: "tri_col" (1: column-blocked
+ * "cg_batch_max", "spmv_blocks_per_cu" (the fused SpMV + dot of CG: 0, the
+ * default, launches a workgroup per 256-row block, k > 0 at most k per CU over
+ * a grid-stride loop).  Paths: "tri_col" (1: column-blocked
  * sweeps where the plan fits, 0: level pool), "tri_stencil" (1: pencil sweeps
  * for stencil patterns, read when the ILU object is created), "cg_fuse" (1:
  * the sweeps' permutations fused into the PCG vector kernels).  Methods:
