@@ -110,6 +110,52 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
     }
 }
 
+// Identity / Jacobi CG on small systems, where every launch costs more than
+// its work: the direction update p = beta p + z of the previous pass is formed
+// inside the product, for each gathered entry and for the block's own rows
+// (z = r or r ./ d on the fly; the same two roundings as cg_pupdate_kernel),
+// from pold into pnew (the next pass swaps them).  Then Ap = A pnew and
+// <pnew, Ap> as in cg_spmv_kernel: each row summed in CSR order, a canonical
+// partial per 256-row block, alpha from the collector.
+constexpr int PZ_MAX_ROWS = 1 << 18;  // above this the separate update's streaming access pays
+template <int PREC>
+__global__ __launch_bounds__(256) void cg_spmv_pz_kernel(int n, const int *__restrict__ pntr,
+                                                         const int *__restrict__ indx, const double *__restrict__ vals,
+                                                         const double *__restrict__ pold, const double *__restrict__ r,
+                                                         const double *__restrict__ d, double *__restrict__ pnew,
+                                                         double *__restrict__ ap, double *part, int *err,
+                                                         cg_state *st) {
+    if (st->done) return;
+    __shared__ double sh[CB];
+    const double b = st->beta;
+    auto pz = [&](int j) {
+        const double zj = (PREC == PREC_ID) ? r[j] : r[j] / d[j];
+        return b * pold[j] + 1.0 * zj;
+    };
+    const int nb = (n + CB - 1) / CB;
+    const int i = blockIdx.x * CB + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        double acc = 0.0;
+        const int e1 = pntr[i + 1];
+        for (int e = pntr[i]; e < e1; e++) acc = acc + vals[e] * pz(indx[e]);
+        const double y = 1.0 * acc + 0.0;
+        const double pi = pz(i);
+        pnew[i] = pi;
+        ap[i] = y;
+        v = pi * y;
+    }
+    v = tree256_1<double>(v, sh);
+    if (threadIdx.x == 0) st_agent(part + blockIdx.x, v);
+    if (is_collector()) {
+        double t = final_pass<double>(part, nb, sh, err);
+        if (threadIdx.x == 0) {
+            st->alpha = st->zr / t;
+            st->it = st->it + 1;
+        }
+    }
+}
+
 // x += a p; r += (-a) Ap; <r,r> (and <r, r./d> for Jacobi).  The last block
 // takes the stop decision and, for identity/Jacobi, the new beta.
 // lslot (ILU on the column layout, may be null): r also goes straight into the
@@ -358,7 +404,31 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     }
     HALA_CHECK_LAUNCH();
 
+    // small identity/Jacobi systems: the direction update rides in the next product (p and z's slot alternate)
+    const bool pz = (P.prec == PREC_ID || P.prec == PREC_JAC) && !P.transposed && n <= PZ_MAX_ROWS;
+    int pass = 0;
     auto one_pass = [&]() {
+        if (pz) {
+            double *pc = (pass & 1) ? z : p, *pn = (pass & 1) ? p : z;  // pass 0 uses p as cg_start left it
+            if (pass == 0)
+                cg_spmv_kernel<false><<<gs, 256, 0, c->stream>>>(n, pntr, indx, vals, pc, ap, part, wait_error_flag(c),
+                                                                 st);
+            else if (P.prec == PREC_ID)
+                cg_spmv_pz_kernel<PREC_ID><<<gs, 256, 0, c->stream>>>(n, pntr, indx, vals, pn, r, nullptr, pc, ap,
+                                                                      part, wait_error_flag(c), st);
+            else
+                cg_spmv_pz_kernel<PREC_JAC><<<gs, 256, 0, c->stream>>>(n, pntr, indx, vals, pn, r, P.pdiag, pc, ap,
+                                                                       part, wait_error_flag(c), st);
+            if (P.prec == PREC_ID)
+                cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, pc, ap, nullptr, x, r, part, part2,
+                                                                     wait_error_flag(c), st);
+            else
+                cg_update_kernel<PREC_JAC><<<ge, 256, 0, c->stream>>>(n, pc, ap, P.pdiag, x, r, part, part2,
+                                                                      wait_error_flag(c), st);
+            HALA_CHECK_LAUNCH();
+            pass++;
+            return;
+        }
         if (nt)
             cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, wait_error_flag(c), st);
         else
