@@ -363,43 +363,51 @@ __device__ __forceinline__ T tree256_1(T v, T *sh) {
 // cannot be mistaken for a later reduction's.
 constexpr unsigned RED_SPIN_LIMIT = 1u << 24;
 
-template <class T>
-__device__ __forceinline__ T poll_slot(T *p, int *err) {
-    T v = ld_agent(p);
-    unsigned spins = 0;
-    while (is_sent(v)) {
-        __builtin_amdgcn_s_sleep(1);
-        v = ld_agent(p);
-        if (++spins > RED_SPIN_LIMIT) {
-            atomicOr(err, 4);
-            return T(0);
-        }
-    }
-    return v;
-}
-
 // Final canonical pass over nb partials by one 256-thread block:
 // s[t] = 0 + p[t] + p[t+256] + ..., then tree256.  Result valid in thread 0.
-// Re-arms every slot it reads.
+// Re-arms every slot it reads.  The slots of a batch still holding the
+// sentinel are re-polled together, so partials that land late cost one round
+// trip after the last of them rather than one each.  A wait is bounded: after
+// RED_SPIN_LIMIT polls it raises bit 4 of err and the partials still missing
+// count as 0.
 template <class T>
 __device__ __forceinline__ T final_pass(T *part, int64_t nb, T *sh, int *err) {
     constexpr int B = 8;  // slots loaded per thread at once
     T s = T(0);
     for (int64_t k0 = threadIdx.x; k0 < nb; k0 += (int64_t)CB * B) {
         T v[B];
+        unsigned pend = 0;
 #pragma unroll
         for (int j = 0; j < B; j++) {
-            int64_t k = k0 + (int64_t)CB * j;
+            const int64_t k = k0 + (int64_t)CB * j;
             v[j] = (k < nb) ? ld_agent(part + k) : T(0);
+            if (k < nb && is_sent(v[j])) pend |= 1u << j;
+        }
+        for (unsigned spins = 0; pend != 0; spins++) {
+            if (spins > RED_SPIN_LIMIT) {
+                atomicOr(err, 4);
+#pragma unroll
+                for (int j = 0; j < B; j++)
+                    if (pend >> j & 1u) v[j] = T(0);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int j = 0; j < B; j++)
+                if (pend >> j & 1u) {
+                    const T x = ld_agent(part + k0 + (int64_t)CB * j);
+                    if (!is_sent(x)) {
+                        v[j] = x;
+                        pend &= ~(1u << j);
+                    }
+                }
         }
 #pragma unroll
         for (int j = 0; j < B; j++) {
-            int64_t k = k0 + (int64_t)CB * j;
+            const int64_t k = k0 + (int64_t)CB * j;
             if (k < nb) {
-                T x = v[j];
-                if (is_sent(x)) x = poll_slot(part + k, err);
                 st_agent(part + k, sentinel<T>());
-                s = s + x;
+                s = s + v[j];
             }
         }
     }
