@@ -530,6 +530,58 @@ def test_cg_jacobi_bitexact(engine):
     assert np.array_equal(bits(host(gx)), bits(x))
 
 
+@pytest.mark.parametrize("prec", ["identity", "jacobi"])
+@pytest.mark.parametrize("side", [64, 520])
+def test_cg_direction_update_paths(engine, prec, side):
+    # up to 2^18 rows the direction update is formed inside the next product,
+    # above it by its own pass: both give the oracle's bits (40 iterations)
+    import torch
+
+    p, i, v = st.laplace2d(side, side, diag=4.3)
+    n = len(p) - 1
+    d = np.full(n, 4.3)
+    b = st.uniform(404 + side, n)
+    kind = oracle.PREC_JACOBI if prec == "jacobi" else oracle.PREC_IDENTITY
+    it, x, hist = oracle.cg(p, i, v, b, tol=0.0, max_iter=40, prec=kind, mdata=d)
+    gp, gi, gv = upload(engine, p, i, v)
+    gx = engine.vector(n)
+    gh = torch.zeros(64, dtype=torch.float64, device=gx.device)
+    pre = ("jacobi", dev(engine, d)) if prec == "jacobi" else None
+    git = H.solve_cg(engine, H.stop_criteria(0.0, 40), gp, gi, gv, pre, dev(engine, b), gx, hist=gh)
+    assert git == it == 40
+    assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist[: it - 1]))
+    assert np.array_equal(bits(host(gx)), bits(x))
+
+
+def test_cg_grid_stride_product(engine):
+    # the fused SpMV + p.Ap launched grid-stride (option spmv_blocks_per_cu > 0) gives the same bits
+    import torch
+
+    p, i, v = st.laplace3d(40, 40, 40)
+    n = len(p) - 1
+    b = st.uniform(505, n)
+    tol = 1e-10 * oracle.cnrm2(b)
+    gp, gi, gv = upload(engine, p, i, v)
+    M = H.make_ilu(engine, gp, gi, gv)
+    out = []
+    saved = engine.get_option("spmv_blocks_per_cu")
+    try:
+        for k in (0, 2):
+            engine.set_option("spmv_blocks_per_cu", k)
+            gx = engine.vector(n)
+            gh = torch.zeros(512, dtype=torch.float64, device=gx.device)
+            it = H.solve_cg_ilu(engine, tol, gp, gi, gv, M, dev(engine, b), gx, hist=gh)
+            out.append((it, bits(host(gx)), bits(host(gh)[: it - 1])))
+    finally:
+        engine.set_option("spmv_blocks_per_cu", saved)
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+    dg, _ = oracle.diag(p, i)
+    f = oracle.ilu0(p, i, v, dg)
+    it, x, _ = oracle.cg(p, i, v, b, tol=tol, prec=oracle.PREC_ILU, mpntr=p, mindx=i, mdiag=dg, mdata=f)
+    assert out[0][0] == it and np.array_equal(out[0][1], bits(x))
+
+
 def test_cg_reference_small(engine):
     # tests/solvers_tests.hpp:12-55: tridiagonal -> ILU exact -> two iterations; cyclic 5x5 to 1e-9
     p = np.array([0, 2, 5, 8, 11, 13], np.int32)
