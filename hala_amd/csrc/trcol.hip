@@ -106,15 +106,12 @@ __global__ void col_count(int n, int ncol, const int *__restrict__ cmap, const i
 }
 
 // per level: start position of every cell inside the level's row list (its
-// rows in (column, row) order; desc: rows ascending with columns descending,
-// the contiguous backward map in the (level, row) layout)
-__global__ void col_starts(int nlev, int ncol, int desc, const int *__restrict__ cellcnt,
-                           int *__restrict__ cstart) {
+// rows in (column, row) order)
+__global__ void col_starts(int nlev, int ncol, const int *__restrict__ cellcnt, int *__restrict__ cstart) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlev) return;
     int run = 0;
-    for (int i = 0; i < ncol; i++) {
-        int c = desc ? ncol - 1 - i : i;
+    for (int c = 0; c < ncol; c++) {
         cstart[(int64_t)l * ncol + c] = run;
         run += cellcnt[(int64_t)l * ncol + c];
     }
@@ -406,7 +403,6 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         }
         if (P <= 0 || n % P != 0) P = 0;
     }
-    bool resorted = false;
     int ncol = ncol_req > 0 ? ncol_req : 4;
     while (ncol_req <= 0 && (int64_t)maxw > (int64_t)ncol * cpmax) ncol *= 2;
     for (; ncol <= 512; ncol *= 2) {
@@ -462,13 +458,10 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
             (int64_t)CP * rec_bytes(L, sizeof(double)) > (B > 1 ? STEP_BYTES_SPLIT : STEP_BYTES))
             continue;
         const int K = std::min(fetch_i32(c, gap.as<int>()) + 1, KMAX);
-        // cells must be contiguous in every level's row list: the (level, row)
-        // layout has them for the contiguous map; otherwise order rows by column
-        if (B > 1 || resorted || !p.perm_sorted) {
-            reorder_levels(c, p, cmap.as<int>(), ncol);
-            resorted = true;
-        }
-        const int desc = (p.backward && !resorted) ? 1 : 0;
+        // cells must be contiguous in every level's row list: rows of a level
+        // ordered by (column, row), which also lays out the slots this layout
+        // indexes (chunk_row, slot_of_row)
+        reorder_levels(c, p, cmap.as<int>(), ncol);
         // external counts, cell starts, cell records
         ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
         epos.reserve((size_t)(nslots + 1) * sizeof(int));
@@ -479,7 +472,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         exclusive_scan_i32(c, ecnt.as<int>(), epos.as<int>(), nslots);
         const int next = fetch_i32(c, epos.as<int>() + nslots);
         cstart.reserve((size_t)ncell * sizeof(int));
-        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, desc, cellcnt.as<int>(), cstart.as<int>());
+        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, cellcnt.as<int>(), cstart.as<int>());
         HALA_CHECK_LAUNCH();
         ci.reserve((size_t)ncell * CI * sizeof(int));
         colrange.reserve((size_t)2 * ncol * sizeof(int));

@@ -313,13 +313,13 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     HALA_CHECK_LAUNCH();
     check_wait_error(c);
     p.nlev = reduce_max_i32(c, p.lev.as<int>(), n) + 1;
-    // rows grouped by level; sorted by row inside a level unless the column
-    // layout (which orders every level its own way) will be tried
+    // rows grouped by level, sorted by row inside a level (the column layout
+    // may reorder every level its own way later)
     p.level_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));
     p.perm.reserve((size_t)n * sizeof(int));
-    p.perm_sorted = !c->opt.tri_col;
+    p.perm_sorted = true;
     p.laid_out = false;
-    bucket_sort(c, n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>(), p.perm_sorted);
+    bucket_sort(c, n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>());
     p.h_level_ptr.resize(p.nlev + 1);
     HALA_HIP(hipMemcpyAsync(p.h_level_ptr.data(), p.level_ptr.p, (size_t)(p.nlev + 1) * sizeof(int),
                             hipMemcpyDeviceToHost, c->stream));
@@ -346,7 +346,7 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
 
 void sort_perm(hala_ctx c, tri_plan &p) {
     if (p.perm_sorted) return;
-    bucket_sort(c, p.n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>(), true);
+    bucket_sort(c, p.n, p.lev.as<int>(), p.nlev, p.level_ptr.as<int>(), p.perm.as<int>());
     p.perm_sorted = true;
 }
 

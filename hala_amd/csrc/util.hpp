@@ -9,9 +9,8 @@ void exclusive_scan_i32(hala_ctx c, const int *in, int *out, int n);
 
 // Groups items 0..nitems-1 by bucket[item] (0 <= bucket < nbuckets):
 // bptr[nbuckets+1] bucket offsets, items[] listed bucket by bucket, ascending
-// inside each bucket (a stable counting sort); sorted = false leaves a
-// bucket's items in arbitrary order.
-void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *bptr, int *items, bool sorted = true);
+// inside each bucket (a stable sort).
+void bucket_sort(hala_ctx c, int nitems, const int *bucket, int nbuckets, int *bptr, int *items);
 
 // Transposed pattern of an nrows x ncols CSR: tpntr[ncols+1]; for each
 // transposed entry e: trow[e] = source row, tmap[e] = source position;
