@@ -90,19 +90,25 @@ __global__ void col_upward(int n, const int *__restrict__ cmap, const int *__res
 }
 
 // cell populations and the largest level gap of a same-column dependency
-__global__ void col_count(int n, int ncol, const int *__restrict__ cmap, const int *__restrict__ lev,
-                          const int *__restrict__ dptr, const int *__restrict__ ddep, int *__restrict__ cellcnt,
-                          int *__restrict__ maxgap) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int c = cmap[r];
-    atomicAdd(cellcnt + (int64_t)lev[r] * ncol + c, 1);
+// (reduced over the wavefront first: one atomic on the single gap counter per
+// wavefront instead of one per row)
+__global__ __launch_bounds__(256) void col_count(int n, int ncol, const int *__restrict__ cmap,
+                                                 const int *__restrict__ lev, const int *__restrict__ dptr,
+                                                 const int *__restrict__ ddep, int *__restrict__ cellcnt,
+                                                 int *__restrict__ maxgap) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
     int g = 0;
-    for (int e = dptr[r]; e < dptr[r + 1]; e++) {
-        int d = ddep[e];
-        if (cmap[d] == c) g = max(g, lev[r] - lev[d]);
+    if (r < n) {
+        const int c = cmap[r];
+        atomicAdd(cellcnt + (int64_t)lev[r] * ncol + c, 1);
+        for (int e = dptr[r]; e < dptr[r + 1]; e++) {
+            int d = ddep[e];
+            if (cmap[d] == c) g = max(g, lev[r] - lev[d]);
+        }
     }
-    if (g > 0) atomicMax(maxgap, g);
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) g = max(g, __shfl_xor(g, o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0 && g > 0) atomicMax(maxgap, g);
 }
 
 // per level: start position of every cell inside the level's row list (its
@@ -121,21 +127,21 @@ __device__ __forceinline__ bool is_local(int r, int d, int c, int cd, const int 
     return cd == c && lev[r] - lev[d] < K;
 }
 
-// external dependencies per slot
-__global__ void col_extcnt(int nslots, const int *__restrict__ cmap, int K, const int *__restrict__ chunk_row,
+// external dependencies per slot (ecnt zeroed beforehand: slots without a
+// row keep 0).  A thread per row, in row order: the row's data and its
+// neighbours' are read nearly contiguously, only the count lands scattered.
+__global__ void col_extcnt(int n, const int *__restrict__ cmap, int K, const int *__restrict__ slot_of_row,
                            const int *__restrict__ lev, const int *__restrict__ dptr, const int *__restrict__ ddep,
                            int *__restrict__ ecnt) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    int r = chunk_row[s], k = 0;
-    if (r >= 0) {
-        const int c = cmap[r];
-        for (int e = dptr[r]; e < dptr[r + 1]; e++) {
-            int d = ddep[e];
-            if (!is_local(r, d, c, cmap[d], lev, K)) k++;
-        }
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int c = cmap[r];
+    int k = 0;
+    for (int e = dptr[r]; e < dptr[r + 1]; e++) {
+        int d = ddep[e];
+        if (!is_local(r, d, c, cmap[d], lev, K)) k++;
     }
-    ecnt[s] = k;
+    ecnt[slot_of_row[r]] = k;
 }
 
 // cell records (column-major) and the level range of every column
@@ -163,21 +169,19 @@ __global__ void col_cells(int nlev, int ncol, const int *__restrict__ lchunk, co
 }
 
 // dependency codes: >= 0 LDS window index, -1 none, <= -2 index into the cell's external list
-__global__ void col_records(int nslots, const int *__restrict__ cmap, int ncol, int K, int CP, int L, int nlev,
-                            const int *__restrict__ chunk_row, const int *__restrict__ lev,
-                            const int *__restrict__ lchunk, const int *__restrict__ cstart,
-                            const int *__restrict__ slot_of_row, const int *__restrict__ dptr,
-                            const int *__restrict__ ddep, const int *__restrict__ dsrc, const int *__restrict__ epos,
-                            const int *__restrict__ colrange, int *__restrict__ code, int *__restrict__ src,
-                            int *__restrict__ ext_slot, int *__restrict__ spar) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const int r = chunk_row[s];
-    if (r < 0) return;
+// (a thread per row, in row order, as col_extcnt; slots without a row keep the
+// prefilled -1 codes)
+__global__ void col_records(int n, const int *__restrict__ cmap, int ncol, int K, int CP, int L, int nlev,
+                            const int *__restrict__ lev, const int *__restrict__ lchunk,
+                            const int *__restrict__ cstart, const int *__restrict__ slot_of_row,
+                            const int *__restrict__ dptr, const int *__restrict__ ddep, const int *__restrict__ dsrc,
+                            const int *__restrict__ epos, const int *__restrict__ colrange, int *__restrict__ code,
+                            int *__restrict__ src, int *__restrict__ ext_slot, int *__restrict__ spar) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int s = slot_of_row[r];
     const int l = lev[r];
     spar[s] = (l - colrange[2 * cmap[r]]) & 1;
-    const int S0 = lchunk[l] * WAVE, Wl = (lchunk[l + 1] - lchunk[l]) * WAVE;
-    const int q = s - S0;
     const int c = cmap[r];
     const int64_t base = (int64_t)s * L;  // thread-major: the row's L entries are contiguous
     const int a = dptr[r], len = dptr[r + 1] - a;
@@ -465,9 +469,9 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         // external counts, cell starts, cell records
         ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
         epos.reserve((size_t)(nslots + 1) * sizeof(int));
-        col_extcnt<<<(nslots + 255) / 256, 256, 0, c->stream>>>(nslots, cmap.as<int>(), K, p.chunk_row.as<int>(),
-                                                               p.lev.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
-                                                               ecnt.as<int>());
+        HALA_HIP(hipMemsetAsync(ecnt.p, 0, (size_t)nslots * sizeof(int), c->stream));
+        col_extcnt<<<g, 256, 0, c->stream>>>(n, cmap.as<int>(), K, p.slot_of_row.as<int>(), p.lev.as<int>(),
+                                             p.dptr.as<int>(), p.ddep.as<int>(), ecnt.as<int>());
         HALA_CHECK_LAUNCH();
         exclusive_scan_i32(c, ecnt.as<int>(), epos.as<int>(), nslots);
         const int next = fetch_i32(c, epos.as<int>() + nslots);
@@ -511,11 +515,11 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         fill_i32_k<<<(nslots + 255) / 256, 256, 0, c->stream>>>(q.spar.as<int>(), nslots, 0);
         q.ext_slot.reserve((size_t)(next + BTMAX) * sizeof(int));
         HALA_HIP(hipMemsetAsync(q.ext_slot.p, 0, q.ext_slot.bytes, c->stream));
-        col_records<<<(nslots + 255) / 256, 256, 0, c->stream>>>(
-            nslots, cmap.as<int>(), ncol, K, CP, q.L, nlev, p.chunk_row.as<int>(), p.lev.as<int>(),
-            p.lvl_chunk.as<int>(), cstart.as<int>(), p.slot_of_row.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
-            p.dsrc.as<int>(), epos.as<int>(), colrange.as<int>(), q.code.as<int>(), q.src.as<int>(),
-            q.ext_slot.as<int>(), q.spar.as<int>());
+        col_records<<<g, 256, 0, c->stream>>>(n, cmap.as<int>(), ncol, K, CP, q.L, nlev, p.lev.as<int>(),
+                                              p.lvl_chunk.as<int>(), cstart.as<int>(), p.slot_of_row.as<int>(),
+                                              p.dptr.as<int>(), p.ddep.as<int>(), p.dsrc.as<int>(), epos.as<int>(),
+                                              colrange.as<int>(), q.code.as<int>(), q.src.as<int>(),
+                                              q.ext_slot.as<int>(), q.spar.as<int>());
         HALA_CHECK_LAUNCH();
         // each external value staged once per cell
         {

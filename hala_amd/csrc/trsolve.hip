@@ -199,14 +199,20 @@ __global__ void place_rows(int n, const int *__restrict__ perm, const int *__res
     slot_of_row[row] = slot;
 }
 
-__global__ void chunk_lengths(int nslots, const int *__restrict__ chunk_row, const int *__restrict__ dptr,
-                              int *__restrict__ chunk_len) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    int row = chunk_row[s];
-    if (row < 0) return;
-    int len = dptr[row + 1] - dptr[row];
-    if (len > 0) atomicMax(chunk_len + s / WAVE, len);
+// Longest dependency list of every chunk.  A chunk's 64 slots are exactly one
+// wavefront of this launch (blocks of 256 slots), so the maximum is a lane
+// reduction and one store, not 64 atomics on one counter.
+__global__ __launch_bounds__(256) void chunk_lengths(int nslots, const int *__restrict__ chunk_row,
+                                                     const int *__restrict__ dptr, int *__restrict__ chunk_len) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int len = 0;
+    if (s < nslots) {
+        const int row = chunk_row[s];
+        if (row >= 0) len = dptr[row + 1] - dptr[row];
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0 && s < nslots) chunk_len[s / WAVE] = len;
 }
 
 __global__ void fill_entries(int nslots, const int *__restrict__ chunk_row, const int *__restrict__ chunk_off,
