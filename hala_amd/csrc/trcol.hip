@@ -211,17 +211,20 @@ __global__ void col_records(int n, const int *__restrict__ cmap, int ncol, int K
 // One workgroup per cell: sort the cell's external references, keep each
 // slot once (written back to the front of the cell's segment) and record for
 // every raw reference its index in that unique list.
-__global__ __launch_bounds__(256) void col_dedup(int nlev, int ncol, int *__restrict__ ci,
+// LDS: 3 arrays of cap ints (cap: a power of two >= the largest cell's raw
+// count, at most DEDUP_MAX), so small lists do not cost occupancy.
+__global__ __launch_bounds__(256) void col_dedup(int nlev, int ncol, int cap, int *__restrict__ ci,
                                                  int *__restrict__ ext_slot, int *__restrict__ uidx,
                                                  int *__restrict__ maxext, int *__restrict__ fail) {
-    __shared__ int key[DEDUP_MAX], orig[DEDUP_MAX], flag[DEDUP_MAX];
+    extern __shared__ int dsh[];
+    int *key = dsh, *orig = dsh + cap, *flag = dsh + 2 * cap;
     __shared__ int nu_s;
     const int cell = blockIdx.x;
     const int l = cell / ncol, c = cell % ncol;
     int *rec = ci + ((int64_t)c * nlev + l) * CI;
     const int e0 = rec[3], en = rec[2] >> 16, size = rec[2] & 0xffff;
     if (en == 0) return;
-    if (en > DEDUP_MAX) {
+    if (en > cap) {
         if (threadIdx.x == 0) atomicOr(fail, 1);
         return;
     }
@@ -528,9 +531,13 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
             uidx.reserve((size_t)(next > 0 ? next : 1) * sizeof(int));
             fail.reserve(sizeof(int));
             HALA_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), c->stream));
+            const int rawmax = fetch_i32(c, maxext.as<int>());  // the largest cell's raw external count
+            int cap = 1;
+            while (cap < rawmax && cap < DEDUP_MAX) cap <<= 1;
             HALA_HIP(hipMemsetAsync(maxext.p, 0, sizeof(int), c->stream));
-            col_dedup<<<(unsigned)ncell, 256, 0, c->stream>>>(nlev, ncol, ci.as<int>(), q.ext_slot.as<int>(),
-                                                            uidx.as<int>(), maxext.as<int>(), fail.as<int>());
+            col_dedup<<<(unsigned)ncell, 256, (size_t)3 * cap * sizeof(int), c->stream>>>(
+                nlev, ncol, cap, ci.as<int>(), q.ext_slot.as<int>(), uidx.as<int>(), maxext.as<int>(),
+                fail.as<int>());
             HALA_CHECK_LAUNCH();
             col_fixcodes<<<(int)((q.nrec + 255) / 256), 256, 0, c->stream>>>(q.nrec, q.code.as<int>(),
                                                                              uidx.as<int>());

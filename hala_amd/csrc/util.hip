@@ -7,92 +7,20 @@
 
 namespace hala_amd {
 
-constexpr int SCAN_B = 1024;  // elements per scan block (256 threads x 4)
-
-__global__ __launch_bounds__(256) void scan_block_sums(const int *__restrict__ in, int n, int *__restrict__ sums) {
-    __shared__ int sh[256];
-    int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
-    int s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (base + k < n) s += in[base + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
-}
-
-// In-place exclusive scan of sums[0..m) by one block; total into sums[m].
-__global__ __launch_bounds__(1024) void scan_sums_single(int *sums, int m) {
-    __shared__ int sh[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < m; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = (i < m) ? sums[i] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive
-            int t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < m) sums[i] = carry + sh[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += sh[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) sums[m] = carry;
-}
-
-__global__ __launch_bounds__(256) void scan_apply(const int *__restrict__ in, int n, const int *__restrict__ sums,
-                                                  int *__restrict__ out) {
-    __shared__ int sh[256];
-    int base = blockIdx.x * SCAN_B + threadIdx.x * 4;
-    int v[4], s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        s += v[k];
-    }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
-    }
-    int run = sums[blockIdx.x] + sh[threadIdx.x] - s;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
-    }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = sums[gridDim.x];
-}
-
+// out[0] = 0 and out[1..n] = the inclusive sums (hipCUB's decoupled look-back
+// scan): the exclusive sums with the total in out[n].  (A three-pass scan
+// with a single-block middle pass ran the 14.6 M pair counts of a 128^3 ILU
+// at 0.35 TB/s.)
 void exclusive_scan_i32(hala_ctx c, const int *in, int *out, int n) {
     HALA_REQUIRE(n >= 0, HALA_ERR_ARG, "scan: negative size");
-    if (n == 0) {
-        HALA_HIP(hipMemsetAsync(out, 0, sizeof(int), c->stream));
-        return;
-    }
-    int nb = (n + SCAN_B - 1) / SCAN_B;
+    HALA_HIP(hipMemsetAsync(out, 0, sizeof(int), c->stream));
+    if (n == 0) return;
     tmp_scope ts(c);
-    dbuf &sums = ts.get();
-    sums.reserve((size_t)(nb + 1) * sizeof(int));
-    scan_block_sums<<<nb, 256, 0, c->stream>>>(in, n, sums.as<int>());
-    HALA_CHECK_LAUNCH();
-    scan_sums_single<<<1, 1024, 0, c->stream>>>(sums.as<int>(), nb);
-    HALA_CHECK_LAUNCH();
-    scan_apply<<<nb, 256, 0, c->stream>>>(in, n, sums.as<int>(), out);
-    HALA_CHECK_LAUNCH();
+    dbuf &tmp = ts.get();
+    size_t tbytes = 0;
+    HALA_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tbytes, in, out + 1, n, c->stream));
+    tmp.reserve(tbytes > 0 ? tbytes : 1);
+    HALA_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, tbytes, in, out + 1, n, c->stream));
 }
 
 // Items 0..n-1 (the sort's values).
