@@ -173,7 +173,7 @@ __global__ void zspmv_t_kernel(int ncols, const int *__restrict__ tpntr, const i
     st(y, k, acc);
 }
 
-// IKJ ILU(0) of the rows of one level (the real ilu_level_kernel's order, complex arithmetic)
+// IKJ ILU(0) of the rows of one level (the operation order of the real factor in ilu.hip, complex arithmetic)
 template <class R>
 __global__ void zilu_level_kernel(const int *__restrict__ rows, int count, const int *__restrict__ pntr,
                                   const int *__restrict__ indx, const int *__restrict__ diag, R *f) {

@@ -69,55 +69,122 @@ __global__ void ilu_pair_fill(int n, const int *__restrict__ pntr, const int *__
     }
 }
 
-// The rows of one level from the symbolic pairs, one wavefront per row: for
-// each strictly lower entry jc = (j,i) in order, l = f[jc] / u_ii, f[jc] = l,
-// then f[jk] -= l * f[ik] over its pairs.  The row is staged in LDS and every
-// operand from earlier rows (pivots u_ii, entries u_ik: final by now) is
-// fetched up front, so a row costs a few memory round trips instead of one
-// chain of dependent loads per entry.  The elimination stays in order; the
-// updates of one entry touch distinct positions, so its lanes apply them at
-// once.  Rows over the staged limits run the same loop on lane 0.
+// Numeric ILU(0) in one launch, sync-free.  Wavefront t factors the t-th row
+// of the L schedule's level order: for each strictly lower entry jc = (j,i)
+// of row j in column order, l = a_ji / u_ii, a_ji = l, then a_jk -= l * u_ik
+// over the (ik, jk) pairs of the symbolic pass -- every entry sees the
+// operation sequence of the reference's KIJ loop
+// (sparse/hala_sparse_utils.hpp:229-254).
+//
+// Operands from earlier rows (the pivots u_ii and the entries u_ik) are read
+// in f, which holds sentinels until the row owning an entry stores its final
+// value with an agent-scope store: each value is its own ready flag, so no
+// fence or per-row flag is needed.  A row depends only on rows of earlier
+// levels, i.e. on lower wavefront indices, and workgroups are dispatched in
+// index order on each XCD (as the reductions' collector also assumes): the
+// lowest unfinished row has every dependency stored and a running wavefront,
+// so every wait ends.  Waits are bounded and raise the wait-error flag (bit
+// 1) if they give up.  (Tickets handed out by an atomic would not need the
+// dispatch order, but one per row cost ~13 ns each, more than the whole
+// factorisation of a 128^3 operator.)
+//
+// A row is staged in LDS and all its operands from earlier rows are polled up
+// front, so a row costs a few memory round trips; the elimination stays in
+// order, and the updates of one entry touch distinct positions, so its lanes
+// apply them at once.  Rows over the staged limits run the same loop on lane
+// 0, in `work` (same indexing as f).
 constexpr int ILU_RB = 128;  // row entries staged
 constexpr int ILU_PS = 4;    // pairs per lane (256 per row)
-__global__ __launch_bounds__(256) void ilu_level_rows(const int4 *__restrict__ frow, const int *__restrict__ fnp,
-                                                      int count, const int *__restrict__ dpos,
-                                                      const int *__restrict__ pptr, const int2 *__restrict__ pairs,
-                                                      double *f) {
+
+__device__ __forceinline__ double poll_factor(const double *p, bool &bad) {
+    double v = ld_agent(p);
+    for (unsigned spins = 0; is_sent(v); spins++) {
+        if (spins > SPIN_LIMIT) {
+            bad = true;
+            return 0.0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = ld_agent(p);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void ilu_rows_syncfree(const int4 *__restrict__ frow, const int *__restrict__ fnp,
+                                                         int n, const int *__restrict__ dpos,
+                                                         const int *__restrict__ pptr, const int2 *__restrict__ pairs,
+                                                         const double *__restrict__ vals, double *f, double *work,
+                                                         int *err) {
     __shared__ double rowbuf[4][ILU_RB];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & (WAVE - 1);
     const int t = blockIdx.x * 4 + w;
-    if (t >= count) return;  // a whole wave
+    if (t >= n) return;  // a whole wavefront
+    double *rb = rowbuf[w];
+    bool bad = false;
     const int4 ri = frow[t];
     const int rs = ri.x, re = ri.y, dj = ri.z, p0 = ri.w, np = fnp[t];
     const int len = re - rs, nl = dj - rs;
-    if (len > ILU_RB || nl >= WAVE || np > ILU_PS * WAVE) {
-        if (lane == 0)
-            for (int jc = rs; jc < dj; jc++) {
-                const double l = f[jc] / f[dpos[jc]];
-                f[jc] = l;
-                for (int p = pptr[jc]; p < pptr[jc + 1]; p++) {
-                    const int2 q = pairs[p];
-                    f[q.y] = f[q.y] - l * f[q.x];
-                }
-            }
+    if (nl == 0) {  // no strictly lower entry: the row is final as given
+        for (int e = lane; e < len; e += WAVE) st_agent(f + rs + e, vals[rs + e]);
         return;
     }
-    double *rb = rowbuf[w];
-    for (int e = lane; e < len; e += WAVE) rb[e] = f[rs + e];
+    if (len > ILU_RB || nl >= WAVE || np > ILU_PS * WAVE) {
+        if (lane == 0) {
+            for (int e = rs; e < re; e++) work[e] = vals[e];
+            for (int jc = rs; jc < dj; jc++) {
+                const double l = work[jc] / poll_factor(f + dpos[jc], bad);
+                work[jc] = l;
+                for (int p = pptr[jc]; p < pptr[jc + 1]; p++) {
+                    const int2 q = pairs[p];
+                    work[q.y] = work[q.y] - l * poll_factor(f + q.x, bad);
+                }
+            }
+            for (int e = rs; e < re; e++) st_agent(f + e, work[e]);
+            if (bad) atomicOr(err, 1);
+        }
+        return;
+    }
+    for (int e = lane; e < len; e += WAVE) rb[e] = vals[rs + e];
     const int pp = lane <= nl ? pptr[rs + lane] : 0;  // pair range of lower entry `lane`
-    const double piv = lane < nl ? f[dpos[rs + lane]] : 1.0;
-    int y[ILU_PS];
+    // operands from earlier rows: bit 0 the pivot of lower entry `lane`, bit k+1 pair k*64+lane
+    const int pi = lane < nl ? dpos[rs + lane] : 0;
+    unsigned pend = lane < nl ? 1u : 0u;
+    int y[ILU_PS], xi[ILU_PS];
     double fx[ILU_PS];
 #pragma unroll
     for (int k = 0; k < ILU_PS; k++) {
         const int idx = k * WAVE + lane;
         y[k] = 0;
+        xi[k] = 0;
         fx[k] = 0.0;
         if (idx < np) {
             const int2 q = pairs[p0 + idx];
             y[k] = q.y - rs;
-            fx[k] = f[q.x];
+            xi[k] = q.x;
+            pend |= 2u << k;
         }
+    }
+    double piv = 1.0;
+    for (unsigned spins = 0;; spins++) {
+        const double v0 = (pend & 1u) ? ld_agent(f + pi) : 1.0;
+        double v[ILU_PS];
+#pragma unroll
+        for (int k = 0; k < ILU_PS; k++) v[k] = (pend >> (k + 1) & 1u) ? ld_agent(f + xi[k]) : 0.0;
+        if ((pend & 1u) && !is_sent(v0)) {
+            piv = v0;
+            pend &= ~1u;
+        }
+#pragma unroll
+        for (int k = 0; k < ILU_PS; k++)
+            if ((pend >> (k + 1) & 1u) && !is_sent(v[k])) {
+                fx[k] = v[k];
+                pend &= ~(2u << k);
+            }
+        if (!__any(pend != 0u)) break;
+        if (spins > SPIN_LIMIT) {
+            bad = bad || pend != 0u;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
     for (int jc = 0; jc < nl; jc++) {
         const int pb = __shfl(pp, jc, WAVE) - p0, pe = __shfl(pp, jc + 1, WAVE) - p0;
@@ -129,7 +196,14 @@ __global__ __launch_bounds__(256) void ilu_level_rows(const int4 *__restrict__ f
             if (idx >= pb && idx < pe) rb[y[k]] = rb[y[k]] - l * fx[k];
         }
     }
-    for (int e = lane; e < len; e += WAVE) f[rs + e] = rb[e];
+    for (int e = lane; e < len; e += WAVE) st_agent(f + rs + e, rb[e]);
+    if (__any(bad) && lane == 0) atomicOr(err, 1);
+}
+
+// every factor entry to the sentinel (not yet stored by its row)
+__global__ void factor_arm(double *__restrict__ f, int64_t nnz) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+        f[e] = sentinel64();
 }
 
 // the factorisation's row records, in the level order of the L schedule
@@ -385,16 +459,14 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         set_device(c);
         alloc_scope as(c->cache.get());
         double *f = M->factor.as<double>();
-        HALA_HIP(hipMemcpyAsync(f, vals, (size_t)M->nnz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        const std::vector<int> &lp = M->L.h_level_ptr;
-        for (int l = 1; l < M->L.nlev; l++) {  // level 0 rows have no strictly lower entries
-            int cnt = lp[l + 1] - lp[l];
-            if (cnt <= 0) continue;
-            ilu_level_rows<<<(cnt + 3) / 4, 256, 0, c->stream>>>(M->frow.as<int4>() + lp[l], M->fnp.as<int>() + lp[l],
-                                                                cnt, M->dpos.as<int>(), M->pptr.as<int>(),
-                                                                M->pairs.as<int2>(), f);
-            HALA_CHECK_LAUNCH();
-        }
+        M->fwork.reserve((size_t)M->nnz * sizeof(double));
+        factor_arm<<<grid_for(M->nnz, 256 * 4, 8192), 256, 0, c->stream>>>(f, M->nnz);
+        ilu_rows_syncfree<<<(M->n + 3) / 4, 256, 0, c->stream>>>(M->frow.as<int4>(), M->fnp.as<int>(), M->n,
+                                                                M->dpos.as<int>(), M->pptr.as<int>(),
+                                                                M->pairs.as<int2>(), vals, f, M->fwork.as<double>(),
+                                                                wait_error_flag(c));
+        HALA_CHECK_LAUNCH();
+        check_wait_error(c);
         int *bad = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 6);
         set_int<<<1, 1, 0, c->stream>>>(bad, INT_MAX);
         pivot_check<<<(M->n + 255) / 256, 256, 0, c->stream>>>(M->n, M->diag.as<int>(), f, bad);

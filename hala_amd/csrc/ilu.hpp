@@ -16,6 +16,7 @@ struct hala_ilu_s {
     hala_amd::dbuf factor;     // combined L\U values (CSR order), FP64
     hala_amd::dbuf dpos, pptr, pairs;  // elimination: u_ii position and (ik, jk) pairs per lower entry
     hala_amd::dbuf frow, fnp;          // rows in level order: (start, end, diagonal, first pair), pair count
+    hala_amd::dbuf fwork;              // in-progress values of rows too long to stage in LDS (factor indexing)
     hala_amd::tri_plan L, U;   // forward (unit L) and backward (U) solves
     hala_amd::tri_plan Ut, Lt; // adjoint: U^T forward, L^T backward (built on demand)
     hala_amd::csr_transpose tp;
