@@ -104,15 +104,17 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
 // A lane keeps LDEP dependencies of its row in registers and polls all of
 // those still pending at once, so a row waits one memory round trip per
 // attempt rather than one per dependency.
+// Wavefront w takes rows w*64 .. w*64+63 (in sweep order) by its index:
+// a row depends only on rows of lower wavefronts (or its own), and
+// workgroups are dispatched in index order on each XCD, so the lowest
+// unfinished wavefront has what it waits for stored, or in its own lanes.
+// (An atomic ticket per wavefront would not need the dispatch order, but
+// queues every wavefront's start on one counter.)
 constexpr int LDEP = 8;
 __global__ __launch_bounds__(256) void level_kernel(int n, int backward, const int *__restrict__ dptr,
-                                                    const int *__restrict__ ddep, int *lev, unsigned *ticket,
-                                                    int *err, int nwaves) {
+                                                    const int *__restrict__ ddep, int *lev, int *err) {
     const int lane = threadIdx.x & (WAVE - 1);
-    int w = 0;
-    if (lane == 0) w = (int)atomicAdd(ticket, 1u);
-    w = __shfl(w, 0, WAVE);
-    if (w == nwaves - 1 && lane == 0) st_agent(ticket, 0u);  // last ticket re-arms the counter
+    const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     int idx = w * WAVE + lane;
     int row = backward ? n - 1 - idx : idx;
     bool done = !(idx < n);
@@ -313,9 +315,8 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     // levels
     int nwaves = (n + WAVE - 1) / WAVE;
     int lblocks = (nwaves + 3) / 4;
-    unsigned *ticket = counters(c) + 100;
     level_kernel<<<lblocks, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, p.dptr.as<int>(), p.ddep.as<int>(),
-                                                 p.lev.as<int>(), ticket, wait_error_flag(c), lblocks * 4);
+                                                 p.lev.as<int>(), wait_error_flag(c));
     HALA_CHECK_LAUNCH();
     check_wait_error(c);
     p.nlev = reduce_max_i32(c, p.lev.as<int>(), n) + 1;
