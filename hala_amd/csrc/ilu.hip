@@ -322,6 +322,11 @@ static void ilu_pairs(hala_ctx c, hala_ilu M) {
     ilu_pair_fill<<<g, 256, 0, c->stream>>>(n, M->pntr, M->indx, M->diag.as<int>(), M->pptr.as<int>(),
                                             M->dpos.as<int>(), M->pairs.as<int2>());
     HALA_CHECK_LAUNCH();
+}
+
+// rows in L's level order with their ranges (after ilu_pairs and L's schedule)
+static void ilu_rows(hala_ctx c, hala_ilu M) {
+    const int n = M->n, g = (n + 255) / 256;
     // a copy of the level order: the plan's own may be re-sorted later
     M->frow.reserve((size_t)n * sizeof(int4));
     M->fnp.reserve((size_t)n * sizeof(int));
@@ -367,6 +372,9 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                     try {
                         set_device(a);
                         alloc_scope aas(a->cache.get());  // blocks reused on this stream only
+                        // the elimination pairs first: U's level-set kernel queues behind L's
+                        // (each takes the whole chip), so this thread has time to spare
+                        ilu_pairs(a, M);
                         build_deps(a, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
                         build_schedule(a, M->U, false);
                         HALA_HIP(hipStreamSynchronize(a->stream));
@@ -377,12 +385,12 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                 try {
                     build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
                     build_schedule(c, M->L, false);
-                    ilu_pairs(c, M);
                 } catch (...) {
                     ut.join();
                     throw;
                 }
                 ut.join();
+                if (!uerr) ilu_rows(c, M);  // needs the pair pointers (the other stream is idle after the join)
             } catch (...) {
                 (void)hipEventDestroy(ready);
                 throw;

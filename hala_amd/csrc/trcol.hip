@@ -89,18 +89,15 @@ __global__ void col_upward(int n, const int *__restrict__ cmap, const int *__res
     if (up) *flag = 1;
 }
 
-// cell populations and the largest level gap of a same-column dependency
-// (reduced over the wavefront first: one atomic on the single gap counter per
-// wavefront instead of one per row)
-__global__ __launch_bounds__(256) void col_count(int n, int ncol, const int *__restrict__ cmap,
-                                                 const int *__restrict__ lev, const int *__restrict__ dptr,
-                                                 const int *__restrict__ ddep, int *__restrict__ cellcnt,
-                                                 int *__restrict__ maxgap) {
+// the largest level gap of a same-column dependency (reduced over the
+// wavefront first: one atomic on the single counter per wavefront)
+__global__ __launch_bounds__(256) void col_gap(int n, const int *__restrict__ cmap, const int *__restrict__ lev,
+                                               const int *__restrict__ dptr, const int *__restrict__ ddep,
+                                               int *__restrict__ maxgap) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     int g = 0;
     if (r < n) {
         const int c = cmap[r];
-        atomicAdd(cellcnt + (int64_t)lev[r] * ncol + c, 1);
         for (int e = dptr[r]; e < dptr[r + 1]; e++) {
             int d = ddep[e];
             if (cmap[d] == c) g = max(g, lev[r] - lev[d]);
@@ -111,16 +108,14 @@ __global__ __launch_bounds__(256) void col_count(int n, int ncol, const int *__r
     if ((threadIdx.x & (WAVE - 1)) == 0 && g > 0) atomicMax(maxgap, g);
 }
 
-// per level: start position of every cell inside the level's row list (its
-// rows in (column, row) order)
-__global__ void col_starts(int nlev, int ncol, const int *__restrict__ cellcnt, int *__restrict__ cstart) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlev) return;
-    int run = 0;
-    for (int c = 0; c < ncol; c++) {
-        cstart[(int64_t)l * ncol + c] = run;
-        run += cellcnt[(int64_t)l * ncol + c];
-    }
+// cell populations and the start of every cell inside its level's row list,
+// from the bucket starts of the rows sorted by (level, column, row)
+__global__ void col_cellcnt(int64_t ncell, int ncol, const int *__restrict__ bptr, int *__restrict__ cellcnt,
+                            int *__restrict__ cstart) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncell) return;
+    cellcnt[k] = bptr[k + 1] - bptr[k];
+    cstart[k] = bptr[k] - bptr[k / ncol * ncol];
 }
 
 __device__ __forceinline__ bool is_local(int r, int d, int c, int cd, const int *lev, int K) {
@@ -388,7 +383,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
     if (maxlen > MAXD || maxlen < 1) return false;
     const int g = (n + 255) / 256;
     dbuf &cellcnt = ts.get(), &gap = ts.get(), &cstart = ts.get(), &ecnt = ts.get(), &epos = ts.get();
-    dbuf &maxext = ts.get(), &cmap = ts.get();
+    dbuf &maxext = ts.get(), &cmap = ts.get(), &cbptr = ts.get(), &corder = ts.get();
     dbuf ci, colrange;  // handed to the plan
     gap.reserve(sizeof(int));
     maxext.reserve(sizeof(int));
@@ -448,11 +443,21 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
             HALA_CHECK_LAUNCH();
         }
         const int64_t ncell = (int64_t)nlev * ncol;
+        HALA_REQUIRE(ncell < INT_MAX, HALA_ERR_ARG, "column plan: too many cells");
         cellcnt.reserve((size_t)ncell * sizeof(int));
-        HALA_HIP(hipMemsetAsync(cellcnt.p, 0, (size_t)ncell * sizeof(int), c->stream));
+        cstart.reserve((size_t)ncell * sizeof(int));
+        cbptr.reserve((size_t)(ncell + 1) * sizeof(int));
+        corder.reserve((size_t)n * sizeof(int));
         HALA_HIP(hipMemsetAsync(gap.p, 0, sizeof(int), c->stream));
-        col_count<<<g, 256, 0, c->stream>>>(n, ncol, cmap.as<int>(), p.lev.as<int>(), p.dptr.as<int>(),
-                                           p.ddep.as<int>(), cellcnt.as<int>(), gap.as<int>());
+        col_gap<<<g, 256, 0, c->stream>>>(n, cmap.as<int>(), p.lev.as<int>(), p.dptr.as<int>(), p.ddep.as<int>(),
+                                         gap.as<int>());
+        HALA_CHECK_LAUNCH();
+        // rows by (level, column, row): the cells' populations and starts come
+        // from the bucket starts, and the order is the layout's if this column
+        // count is kept
+        sort_level_keys(c, p, cmap.as<int>(), ncol, cbptr.as<int>(), corder.as<int>());
+        col_cellcnt<<<(int)((ncell + 255) / 256), 256, 0, c->stream>>>(ncell, ncol, cbptr.as<int>(),
+                                                                      cellcnt.as<int>(), cstart.as<int>());
         HALA_CHECK_LAUNCH();
         const int maxcell = reduce_max_i32(c, cellcnt.as<int>(), (int)ncell);
         const int CP = (maxcell + WAVE - 1) / WAVE * WAVE;
@@ -468,7 +473,7 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         // cells must be contiguous in every level's row list: rows of a level
         // ordered by (column, row), which also lays out the slots this layout
         // indexes (chunk_row, slot_of_row)
-        reorder_levels(c, p, cmap.as<int>(), ncol);
+        apply_level_order(c, p, corder.as<int>());
         // external counts, cell starts, cell records
         ecnt.reserve((size_t)(nslots + 1) * sizeof(int));
         epos.reserve((size_t)(nslots + 1) * sizeof(int));
@@ -478,9 +483,6 @@ bool build_col_plan(hala_ctx c, tri_plan &p, int ncol_req) {
         HALA_CHECK_LAUNCH();
         exclusive_scan_i32(c, ecnt.as<int>(), epos.as<int>(), nslots);
         const int next = fetch_i32(c, epos.as<int>() + nslots);
-        cstart.reserve((size_t)ncell * sizeof(int));
-        col_starts<<<(nlev + 255) / 256, 256, 0, c->stream>>>(nlev, ncol, cellcnt.as<int>(), cstart.as<int>());
-        HALA_CHECK_LAUNCH();
         ci.reserve((size_t)ncell * CI * sizeof(int));
         colrange.reserve((size_t)2 * ncol * sizeof(int));
         init_range<<<(ncol + 63) / 64, 64, 0, c->stream>>>(ncol, nlev, colrange.as<int>());

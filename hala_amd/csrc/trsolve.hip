@@ -287,19 +287,20 @@ __global__ void level_key(int n, int nkey, const int *__restrict__ lev, const in
 
 }  // namespace
 
-void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey) {
+void sort_level_keys(hala_ctx c, const tri_plan &p, const int *key, int nkey, int *bptr, int *order) {
     const int n = p.n;
     HALA_REQUIRE((int64_t)p.nlev * nkey < INT_MAX, HALA_ERR_ARG, "triangular plan: too many level keys");
     tmp_scope ts(c);
-    dbuf &bk = ts.get(), &bptr = ts.get(), &order = ts.get();
+    dbuf &bk = ts.get();
     bk.reserve((size_t)n * sizeof(int));
-    bptr.reserve((size_t)((int64_t)p.nlev * nkey + 1) * sizeof(int));
-    order.reserve((size_t)n * sizeof(int));
     level_key<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nkey, p.lev.as<int>(), key, bk.as<int>());
     HALA_CHECK_LAUNCH();
-    bucket_sort(c, n, bk.as<int>(), p.nlev * nkey, bptr.as<int>(), order.as<int>());
-    // the new order is also the plan's level order (rows of a cell stay together)
-    HALA_HIP(hipMemcpyAsync(p.perm.p, order.p, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    bucket_sort(c, n, bk.as<int>(), p.nlev * nkey, bptr, order);
+}
+
+void apply_level_order(hala_ctx c, tri_plan &p, const int *order) {
+    // the new order is also the plan's level order (rows of a bucket stay together)
+    HALA_HIP(hipMemcpyAsync(p.perm.p, order, (size_t)p.n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     p.perm_sorted = false;
     lay_out_slots(c, p, p.perm.as<int>());
     HALA_HIP(hipStreamSynchronize(c->stream));

@@ -102,11 +102,14 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32);
 // perm in (level, row) order (sorts it if the analysis left a level's rows unsorted).
 void sort_perm(hala_ctx c, tri_plan &p);
 
-// Re-orders the rows inside every level by (key[r], r), key in [0, nkey), and
-// rebuilds the slot layout; perm takes the new order (sort_perm restores
-// (level, row)), levels and level_ptr are unchanged.  Called before any values
-// are loaded.
-void reorder_levels(hala_ctx c, tri_plan &p, const int *key, int nkey);
+// Sorts the rows by (level, key[r], r), key in [0, nkey): order receives the
+// rows, bptr[l * nkey + k] the start of bucket (l, k) in it (nlev * nkey + 1
+// entries).  The plan is not changed.
+void sort_level_keys(hala_ctx c, const tri_plan &p, const int *key, int nkey, int *bptr, int *order);
+// Makes a row order from sort_level_keys the plan's level order and rebuilds
+// the slot layout; perm takes the new order (sort_perm restores (level, row)),
+// levels and level_ptr are unchanged.  Called before any values are loaded.
+void apply_level_order(hala_ctx c, tri_plan &p, const int *order);
 
 // Gathers plan values from a factor value array (positions ent_src / diag_src).
 void load_values64(hala_ctx c, tri_plan &p, const double *factor);
