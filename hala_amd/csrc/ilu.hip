@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <exception>
+#include <future>
 #include <thread>
 
 #include "ilu.hpp"
@@ -361,9 +362,19 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
             HALA_CHECK_LAUNCH();
             // The two analyses are independent and mostly latency-bound: U's runs
             // on the auxiliary context's stream, from a second host thread.
+            // U's levels are L's mirrored where that passes U's own level
+            // recurrence (one pass instead of a second level-set kernel, which
+            // would have to wait for L's: each takes the whole chip).
             hala_ctx a = aux_ctx(c);
-            hipEvent_t ready = nullptr;
+            hipEvent_t ready = nullptr, lev_ready = nullptr;
             HALA_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            if (hipEventCreateWithFlags(&lev_ready, hipEventDisableTiming) != hipSuccess) {
+                (void)hipEventDestroy(ready);
+                throw status_error{HALA_ERR_HIP, "hala_ilu_create: event creation failed"};
+            }
+            std::promise<int> lnlev;  // L's level count once its levels are on the stream (0: none)
+            std::future<int> lnlev_f = lnlev.get_future();
+            bool lnlev_set = false;
             std::exception_ptr uerr;
             try {
                 HALA_HIP(hipEventRecord(ready, c->stream));
@@ -372,11 +383,17 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                     try {
                         set_device(a);
                         alloc_scope aas(a->cache.get());  // blocks reused on this stream only
-                        // the elimination pairs first: U's level-set kernel queues behind L's
-                        // (each takes the whole chip), so this thread has time to spare
+                        // the elimination pairs first: this thread waits for L's levels anyway
                         ilu_pairs(a, M);
                         build_deps(a, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
-                        build_schedule(a, M->U, false);
+                        const int nl = lnlev_f.get();
+                        bool mirrored = false;
+                        if (nl > 0) {
+                            HALA_HIP(hipStreamWaitEvent(a->stream, lev_ready, 0));
+                            mirrored = mirror_levels(a, M->U, M->L.lev.as<int>(), nl);
+                        }
+                        if (!mirrored) compute_levels(a, M->U);
+                        build_schedule(a, M->U, false, true);
                         HALA_HIP(hipStreamSynchronize(a->stream));
                     } catch (...) {
                         uerr = std::current_exception();
@@ -384,8 +401,13 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                 });
                 try {
                     build_deps(c, M->L, n, DEP_LOWER, pntr, indx, M->diag.as<int>(), nullptr, true, false);
-                    build_schedule(c, M->L, false);
+                    compute_levels(c, M->L);
+                    HALA_HIP(hipEventRecord(lev_ready, c->stream));
+                    lnlev.set_value(M->L.nlev);
+                    lnlev_set = true;
+                    build_schedule(c, M->L, false, true);
                 } catch (...) {
+                    if (!lnlev_set) lnlev.set_value(0);
                     ut.join();
                     throw;
                 }
@@ -393,9 +415,11 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                 if (!uerr) ilu_rows(c, M);  // needs the pair pointers (the other stream is idle after the join)
             } catch (...) {
                 (void)hipEventDestroy(ready);
+                (void)hipEventDestroy(lev_ready);
                 throw;
             }
             (void)hipEventDestroy(ready);
+            (void)hipEventDestroy(lev_ready);
             if (uerr) std::rethrow_exception(uerr);
             M->factor.reserve((size_t)nnz * sizeof(double));
             M->tmp64.reserve((size_t)n * sizeof(double));

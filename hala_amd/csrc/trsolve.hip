@@ -175,6 +175,22 @@ __global__ void fill_i32_k(int *__restrict__ x, int64_t n, int v) {
     if (i < n) x[i] = v;
 }
 
+// Candidate levels from another triangle's (top = its level count - 1):
+// lev[r] = top - lo[r], checked against this triangle's own level recurrence
+// (0 for a row without dependencies, else 1 + the largest level among them).
+// A row that breaks it sets *bad.  When none does, the candidate is the
+// recurrence's unique solution, which level_kernel would compute.
+__global__ void level_mirror(int n, int top, const int *__restrict__ lo, const int *__restrict__ dptr,
+                             const int *__restrict__ ddep, int *__restrict__ lev, int *__restrict__ bad) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int cand = top - lo[r];
+    int m = 0;
+    for (int e = dptr[r]; e < dptr[r + 1]; e++) m = max(m, top - lo[ddep[e]] + 1);
+    lev[r] = cand;
+    if (m != cand) *bad = 1;
+}
+
 template <class T>
 __global__ void fill_sent_k(T *__restrict__ x, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -306,14 +322,13 @@ void apply_level_order(hala_ctx c, tri_plan &p, const int *order) {
     HALA_HIP(hipStreamSynchronize(c->stream));
 }
 
-void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
+void compute_levels(hala_ctx c, tri_plan &p) {
     const int n = p.n;
     HALA_REQUIRE(n > 0, HALA_ERR_ARG, "triangular plan: empty matrix");
     int g = (n + 255) / 256;
     p.lev.reserve((size_t)n * sizeof(int));
     fill_i32_k<<<g, 256, 0, c->stream>>>(p.lev.as<int>(), n, -1);
     HALA_CHECK_LAUNCH();
-    // levels
     int nwaves = (n + WAVE - 1) / WAVE;
     int lblocks = (nwaves + 3) / 4;
     level_kernel<<<lblocks, 256, 0, c->stream>>>(n, p.backward ? 1 : 0, p.dptr.as<int>(), p.ddep.as<int>(),
@@ -321,6 +336,28 @@ void build_schedule(hala_ctx c, tri_plan &p, bool want32) {
     HALA_CHECK_LAUNCH();
     check_wait_error(c);
     p.nlev = reduce_max_i32(c, p.lev.as<int>(), n) + 1;
+}
+
+bool mirror_levels(hala_ctx c, tri_plan &p, const int *lev_other, int nlev_other) {
+    const int n = p.n;
+    HALA_REQUIRE(n > 0 && nlev_other > 0, HALA_ERR_ARG, "triangular plan: empty matrix");
+    tmp_scope ts(c);
+    dbuf &bad = ts.get();
+    bad.reserve(sizeof(int));
+    HALA_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), c->stream));
+    p.lev.reserve((size_t)n * sizeof(int));
+    level_mirror<<<(n + 255) / 256, 256, 0, c->stream>>>(n, nlev_other - 1, lev_other, p.dptr.as<int>(),
+                                                         p.ddep.as<int>(), p.lev.as<int>(), bad.as<int>());
+    HALA_CHECK_LAUNCH();
+    if (fetch_i32(c, bad.as<int>()) != 0) return false;
+    p.nlev = nlev_other;  // the candidate's largest value: the other triangle has rows at level 0
+    return true;
+}
+
+void build_schedule(hala_ctx c, tri_plan &p, bool want32, bool have_levels) {
+    const int n = p.n;
+    HALA_REQUIRE(n > 0, HALA_ERR_ARG, "triangular plan: empty matrix");
+    if (!have_levels) compute_levels(c, p);
     // rows grouped by level, sorted by row inside a level (the column layout
     // may reorder every level its own way later)
     p.level_ptr.reserve((size_t)(p.nlev + 1) * sizeof(int));

@@ -97,7 +97,16 @@ void build_deps(hala_ctx c, tri_plan &p, int n, dep_sel sel, const int *pntr, co
                 const int *diagpos, const csr_transpose *tp, bool unit, bool backward);
 
 // Computes levels, permutation, chunks and entry layout; fills out64 with sentinels.
-void build_schedule(hala_ctx c, tri_plan &p, bool want32);
+void build_schedule(hala_ctx c, tri_plan &p, bool want32, bool have_levels = false);
+// The level of every row (p.lev, p.nlev) by the sync-free level-set kernel;
+// build_schedule calls it unless told the levels are there.
+void compute_levels(hala_ctx c, tri_plan &p);
+// The levels of p as nlev_other - 1 - lev_other (another triangle's levels
+// mirrored), if they satisfy p's own level recurrence, which makes them equal
+// to compute_levels' result; false (p.lev unspecified) otherwise.  For an ILU
+// factor's U, with L's levels, this holds on stencil patterns whose rows all
+// lie on a longest dependency chain.
+bool mirror_levels(hala_ctx c, tri_plan &p, const int *lev_other, int nlev_other);
 
 // perm in (level, row) order (sorts it if the analysis left a level's rows unsorted).
 void sort_perm(hala_ctx c, tri_plan &p);
