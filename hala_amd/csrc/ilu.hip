@@ -510,15 +510,36 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         M->jac_values64 = false;
         HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
                      "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
-        load_values64(c, M->L, f);
-        load_values64(c, M->U, f);
+        // the two triangles' values are packed side by side: U's on the
+        // auxiliary context's stream, joined back before anything reads them
+        // (scattered gathers from the factor, mostly latency)
+        hala_ctx a = aux_ctx(c);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        HALA_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        if (hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(ev[0]);
+            throw status_error{HALA_ERR_HIP, "hala_ilu_factor_f64: event creation failed"};
+        }
+        try {
+            HALA_HIP(hipEventRecord(ev[0], c->stream));
+            HALA_HIP(hipStreamWaitEvent(a->stream, ev[0], 0));
+            load_values64(a, M->U, f);
+            if (want_f32) load_values32(a, M->U, f);
+            HALA_HIP(hipEventRecord(ev[1], a->stream));
+            load_values64(c, M->L, f);
+            if (want_f32) load_values32(c, M->L, f);
+            HALA_HIP(hipStreamWaitEvent(c->stream, ev[1], 0));
+        } catch (...) {
+            (void)hipStreamSynchronize(a->stream);
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            throw;
+        }
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
         col_set_handoff(c, M->L, M->U);
         if (M->sp) stencil_load(M, want_f32);
-        if (want_f32) {
-            load_values32(c, M->L, f);
-            load_values32(c, M->U, f);
-            M->has32 = true;
-        }
+        if (want_f32) M->has32 = true;
         M->factored = true;
         HALA_HIP(hipStreamSynchronize(c->stream));
     });

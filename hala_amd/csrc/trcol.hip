@@ -295,30 +295,63 @@ __global__ void init_range(int ncol, int nlev, int *__restrict__ colrange) {
 }
 
 // Packed row records, one per slot, RB bytes each (a multiple of 16, read
-// with 16-byte loads): int16 dependency codes [LR] | diagonal T | values T[LR].
+// with 16-byte loads): int16 dependency codes [LR] | diagonal T | values T[LR]
+// (zero padding after).  A thread per 16-byte quad of a record, so that each
+// lane stores one whole quad next to its neighbours'.
 template <class T>
 __global__ void col_pack(int nslots, int nslots_pad, int LR, int RB, int kcp, int xc, const int *__restrict__ code,
                          const int *__restrict__ spar, const int *__restrict__ src, const int *__restrict__ dg_src,
                          const double *__restrict__ f, unsigned char *__restrict__ rec) {
-    const int zidx = kcp + 2 * xc;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots_pad) return;
-    unsigned char *o = rec + (size_t)s * RB;
-    short *oc = reinterpret_cast<short *>(o);
-    T *od = reinterpret_cast<T *>(o + 2 * LR);
-    T *ov = od + 1;
+    constexpr int SZ = (int)sizeof(T);
+    const int nq = RB / 16;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nslots_pad * nq) return;
+    const int s = (int)(t / nq), qd = (int)(t % nq);
     const bool in = s < nslots;
-    for (int k = 0; k < LR; k++) {
-        // kernel-side codes are LDS element indices: a window entry, external e
-        // of the slot's level in the buffer of that level's parity, or the zero
-        // element zidx for no dependency (value 0)
-        const int c = in ? code[(int64_t)s * LR + k] : -1;
-        const int e = (in && c != -1) ? src[(int64_t)s * LR + k] : -1;
-        oc[k] = (short)(c == -1 ? zidx : (c >= 0 ? c : kcp + spar[s] * xc + (-c - 2)));
-        ov[k] = e >= 0 ? (T)f[e] : T(0);
+    const int zidx = kcp + 2 * xc;
+    const int cb = 2 * LR, vb = cb + SZ, ve = vb + LR * SZ;  // ends of the codes, the diagonal, the values
+    unsigned w4[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int b = 16 * qd + 4 * j;  // byte offset of the word in the record
+        unsigned w = 0;
+        if (b < cb) {
+            // kernel-side codes are LDS element indices: a window entry, external e
+            // of the slot's level in the buffer of that level's parity, or the zero
+            // element zidx for no dependency (value 0)
+            unsigned h[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int k = b / 2 + u;
+                const int c = in ? code[(int64_t)s * LR + k] : -1;
+                const int v = c == -1 ? zidx : (c >= 0 ? c : kcp + spar[s] * xc + (-c - 2));
+                h[u] = (unsigned)(unsigned short)(short)v;
+            }
+            w = h[0] | (h[1] << 16);
+        } else if (b < ve) {
+            T val;
+            int off;
+            if (b < vb) {
+                const int d = in ? dg_src[s] : -1;
+                val = d >= 0 ? (T)f[d] : T(1);
+                off = b - cb;
+            } else {
+                const int k = (b - vb) / SZ;
+                const int c = in ? code[(int64_t)s * LR + k] : -1;
+                const int e = (in && c != -1) ? src[(int64_t)s * LR + k] : -1;
+                val = e >= 0 ? (T)f[e] : T(0);
+                off = (b - vb) % SZ;
+            }
+            if (SZ == 8) {
+                const unsigned long long bits = (unsigned long long)__double_as_longlong((double)val);
+                w = off == 0 ? (unsigned)bits : (unsigned)(bits >> 32);
+            } else {
+                w = __float_as_uint((float)val);
+            }
+        }
+        w4[j] = w;
     }
-    const int d = in ? dg_src[s] : -1;
-    *od = d >= 0 ? (T)f[d] : T(1);
+    reinterpret_cast<uint4 *>(rec)[t] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
 }
 
 // window + external double buffer + cell records (padded to whole pipeline blocks plus the read-ahead)
@@ -583,7 +616,7 @@ void col_load_values(hala_ctx c, tri_plan &p, const double *factor, bool f32) {
     dbuf &dst = f32 ? q.val32 : q.val64;
     dst.reserve((size_t)npad * RB);
     if (!f32) q.hand64 = false;  // rewritten below: hand-off slots gone
-    const int g = (npad + 255) / 256;
+    const int g = (int)(((int64_t)npad * (RB / 16) + 255) / 256);  // a thread per 16-byte quad
     if (f32) {
         col_pack<float><<<g, 256, 0, c->stream>>>(nslots, npad, q.L, RB, kcp, xc, q.code.as<int>(), q.spar.as<int>(),
                                                  q.src.as<int>(),
