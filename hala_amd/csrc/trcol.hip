@@ -86,11 +86,16 @@ __global__ void col_upward(int n, const int *__restrict__ cmap, const int *__res
     const int c = cmap[r];
     bool up = false;
     for (int e = dptr[r]; e < dptr[r + 1]; e++) up |= cmap[ddep[e]] > c;
-    if (up) *flag = 1;
+    // one store per wavefront, and none once the flag is seen set (stores of
+    // every row to one word queue on its memory channel, and other kernels'
+    // accesses behind them)
+    if (__any(up) && (threadIdx.x & (WAVE - 1)) == 0 && ld_agent(flag) == 0) *flag = 1;
 }
 
 // the largest level gap of a same-column dependency (reduced over the
-// wavefront first: one atomic on the single counter per wavefront)
+// wavefront first, then an atomic only where it would raise the counter's
+// current value: the counter settles within the first wavefronts, and
+// atomics of every wavefront on one word would queue on its memory channel)
 __global__ __launch_bounds__(256) void col_gap(int n, const int *__restrict__ cmap, const int *__restrict__ lev,
                                                const int *__restrict__ dptr, const int *__restrict__ ddep,
                                                int *__restrict__ maxgap) {
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(256) void col_gap(int n, const int *__restrict__ cm
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) g = max(g, __shfl_xor(g, o, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0 && g > 0) atomicMax(maxgap, g);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && g > 0 && g > ld_agent(maxgap)) atomicMax(maxgap, g);
 }
 
 // cell populations and the start of every cell inside its level's row list,
@@ -160,7 +165,7 @@ __global__ void col_cells(int nlev, int ncol, const int *__restrict__ lchunk, co
         atomicMin(colrange + 2 * c, l);
         atomicMax(colrange + 2 * c + 1, l);
     }
-    if (en > 0) atomicMax(maxext, en);
+    if (en > 0 && en > ld_agent(maxext)) atomicMax(maxext, en);  // (the atomic only where it raises the value)
 }
 
 // dependency codes: >= 0 LDS window index, -1 none, <= -2 index into the cell's external list
@@ -269,7 +274,7 @@ __global__ __launch_bounds__(256) void col_dedup(int nlev, int ncol, int cap, in
     }
     if (threadIdx.x == 0) {
         rec[2] = size | (nu << 16);
-        atomicMax(maxext, nu);
+        if (nu > ld_agent(maxext)) atomicMax(maxext, nu);
     }
 }
 

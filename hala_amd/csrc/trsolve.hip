@@ -183,12 +183,14 @@ __global__ void fill_i32_k(int *__restrict__ x, int64_t n, int v) {
 __global__ void level_mirror(int n, int top, const int *__restrict__ lo, const int *__restrict__ dptr,
                              const int *__restrict__ ddep, int *__restrict__ lev, int *__restrict__ bad) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int cand = top - lo[r];
-    int m = 0;
-    for (int e = dptr[r]; e < dptr[r + 1]; e++) m = max(m, top - lo[ddep[e]] + 1);
-    lev[r] = cand;
-    if (m != cand) *bad = 1;
+    int cand = 0, m = 0;
+    if (r < n) {
+        cand = top - lo[r];
+        for (int e = dptr[r]; e < dptr[r + 1]; e++) m = max(m, top - lo[ddep[e]] + 1);
+        lev[r] = cand;
+    }
+    // one store per wavefront, none once the flag is seen set
+    if (__any(m != cand) && (threadIdx.x & (WAVE - 1)) == 0 && ld_agent(bad) == 0) *bad = 1;
 }
 
 template <class T>
