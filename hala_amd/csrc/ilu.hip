@@ -383,9 +383,10 @@ int hala_ilu_create(hala_ctx c, int n, int64_t nnz, const int *pntr, const int *
                     try {
                         set_device(a);
                         alloc_scope aas(a->cache.get());  // blocks reused on this stream only
-                        // the elimination pairs first: this thread waits for L's levels anyway
-                        ilu_pairs(a, M);
+                        // U's dependencies before L's level-set kernel takes the chip, then the
+                        // elimination pairs beside it: this thread waits for L's levels anyway
                         build_deps(a, M->U, n, DEP_UPPER, pntr, indx, M->diag.as<int>(), nullptr, false, true);
+                        ilu_pairs(a, M);
                         const int nl = lnlev_f.get();
                         bool mirrored = false;
                         if (nl > 0) {
