@@ -77,6 +77,17 @@ __global__ __launch_bounds__(256) void cg_start_kernel(int n, const double *__re
     }
 }
 
+// The 256-row block group a workgroup handles: with xcd, the XCD-aware
+// virtual block (every XCD takes a contiguous share of the rows, so that the
+// scattered stores of neighbouring rows into one line of a slot order merge
+// in one L2 instead of leaving the line partially written by several; the
+// grid is then xcd_grid(nvirt) and workgroups past nvirt have nothing to do),
+// else its own index.  Workgroup 0 handles block 0 either way.  (128^3
+// ILU-PCG: the update with its scatter into L's slot order 78.6 -> 77.6 ms
+// per solve; the z gather from U's slot order and the fused product were
+// slower in this order.)
+__device__ __forceinline__ int cg_block(int xcd, int nvirt) { return xcd ? xcd_block(nvirt) : (int)blockIdx.x; }
+
 // Ap = A p (beta 0) fused with <p, Ap>; the last block forms alpha = zr / <p,Ap>.
 // One block = 256 rows = one canonical partial.
 template <bool NT>
@@ -168,15 +179,17 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
                                                         double *__restrict__ x, double *__restrict__ r, double *part,
                                                         double *part2, int *err, cg_state *st,
                                                         const int *__restrict__ lslot = nullptr,
-                                                        double *__restrict__ lxs = nullptr) {
+                                                        double *__restrict__ lxs = nullptr, int xcd = 0) {
     if (st->done) return;
     __shared__ double sh[CB * EK * 2];
     const int64_t nb = ((int64_t)n + CB - 1) / CB;
+    const int vb = cg_block(xcd, (int)((nb + EK - 1) / EK));
+    if ((int64_t)vb * EK >= nb) return;
     const double a = st->alpha;
     double v[2 * EK];
 #pragma unroll
     for (int k = 0; k < EK; k++) {
-        int64_t i = ((int64_t)blockIdx.x * EK + k) * CB + threadIdx.x;
+        int64_t i = ((int64_t)vb * EK + k) * CB + threadIdx.x;
         double rr = 0.0, rz = 0.0;
         if (i < n) {
             if (x) x[i] = x[i] + a * p[i];
@@ -201,7 +214,7 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, const double *__r
     }
     if (threadIdx.x == 0)
         for (int k = 0; k < EK; k++) {
-            int64_t b = (int64_t)blockIdx.x * EK + k;
+            int64_t b = (int64_t)vb * EK + k;
             if (b < nb) {
                 st_agent(part + b, v[k]);
                 if (PREC == PREC_JAC) st_agent(part2 + b, v[EK + k]);
@@ -445,9 +458,10 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
         default:
             if (fused) {  // the sweeps' gather and scatter passes fused into the update and r.z
                 tri_plan &L = P.M->L, &U = P.M->U;
-                cg_update_kernel<PREC_ILU><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, nullptr, r, part, part2,
-                                                                      wait_error_flag(c), st, L.slot_of_row.as<int>(),
-                                                                      L.col.xs64.as<double>());
+                const int xu = c->opt.cg_xcd ? 1 : 0;
+                cg_update_kernel<PREC_ILU><<<xu ? xcd_grid(ge) : ge, 256, 0, c->stream>>>(
+                    n, p, ap, nullptr, nullptr, r, part, part2, wait_error_flag(c), st, L.slot_of_row.as<int>(),
+                    L.col.xs64.as<double>(), xu);
                 const col_axpy ax{x, p, &st->alpha, n};  // x += alpha p on the L sweep's helper workgroups
                 ilu_apply64_mid(P.M, gate, &ax);
                 cg_zscatter_rz_kernel<<<ge, 256, 0, c->stream>>>(n, U.slot_of_row.as<int>(), U.out64.as<double>(), r, z,

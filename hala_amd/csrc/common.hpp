@@ -206,6 +206,7 @@ struct hala_options {
     int tri_col_cols = 0;      // columns of the blocked sweep (0: smallest that fits)
     int tri_col_split = 0;     // columns per slab of planes (0: automatic, 1: contiguous row blocks)
     int cg_fuse = 1;           // ILU-PCG: the sweeps' gather/scatter fused into the update and r.z kernels
+    int cg_xcd = 1;            // fused ILU-PCG: the update (scatter into L's slot order) in XCD-aware block order
     int ilu32_jacobi = 0;      // FP32 ILU applies between FP64 vectors: k > 0 Jacobi sweeps per triangle (0: exact)
     int ilu64_jacobi = 0;      // FP64 ILU applies inside GMRES and hala_ilu_apply_f64: k > 0 Jacobi sweeps per
                                // triangle (0: exact); CG keeps the exact sweeps

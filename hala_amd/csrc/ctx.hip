@@ -213,6 +213,7 @@ static int *option_slot(hala_ctx c, const char *name) {
     if (k == "tri_col_cols") return &c->opt.tri_col_cols;
     if (k == "tri_col_split") return &c->opt.tri_col_split;
     if (k == "cg_fuse") return &c->opt.cg_fuse;
+    if (k == "cg_xcd") return &c->opt.cg_xcd;
     if (k == "cg_batch_max") return &c->opt.cg_batch_max;
     if (k == "spmv_blocks_per_cu") return &c->opt.spmv_blocks_per_cu;
     if (k == "ilu32_jacobi") return &c->opt.ilu32_jacobi;

@@ -63,7 +63,9 @@ int hala_ctx_device(hala_ctx ctx, int *device);
  * a grid-stride loop).  Paths: "tri_col" (1: column-blocked
  * sweeps where the plan fits, 0: level pool), "tri_stencil" (1: pencil sweeps
  * for stencil patterns, read when the ILU object is created), "cg_fuse" (1:
- * the sweeps' permutations fused into the PCG vector kernels).  Methods:
+ * the sweeps' permutations fused into the PCG vector kernels), "cg_xcd" (1:
+ * with cg_fuse, the r update and its scatter into the L sweep's input in
+ * XCD-aware block order; 0: dispatch order).  Methods:
  * "ilu32_jacobi", "ilu64_jacobi" (k > 0: k Jacobi sweeps per triangle instead
  * of the exact apply, inside GMRES and the standalone applies).  Errors:
  * "sync_checks" (1: standalone sweeps wait for the device and report a

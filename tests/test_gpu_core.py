@@ -486,13 +486,17 @@ def test_cg_ilu_published_table(engine):
         assert np.array_equal(bits(u), bits(xo))
 
 
-@pytest.mark.parametrize("fuse", [1, 0], ids=["fused", "separate"])
-def test_cg_ilu_3d_bitexact(engine, fuse):
+@pytest.mark.parametrize("fuse,xcd,shape", [(1, 1, (32, 32, 32)), (1, 1, (30, 31, 29)), (1, 0, (30, 31, 29)),
+                                             (0, 1, (32, 32, 32))],
+                         ids=["fused", "fused-ragged", "fused-ragged-dispatch-order", "separate"])
+def test_cg_ilu_3d_bitexact(engine, fuse, xcd, shape):
     # fused: the triangular sweeps' gather and scatter passes run inside the
-    # CG update and r.z kernels; separate: the full ILU apply each pass
+    # CG update and r.z kernels (xcd: the update in XCD-aware block order, 0:
+    # dispatch order); separate: the full ILU apply each pass.  Ragged: a row
+    # count that is no multiple of the 256-row blocks.
     import torch
 
-    p, i, v = st.laplace3d(32, 32, 32)
+    p, i, v = st.laplace3d(*shape)
     n = len(p) - 1
     b = st.uniform(202, n)
     tol = 1e-10 * oracle.cnrm2(b)
@@ -503,12 +507,15 @@ def test_cg_ilu_3d_bitexact(engine, fuse):
     ilu = H.make_ilu(engine, gp, gi, gv)
     gx = engine.vector(n)
     gh = torch.zeros(2000, dtype=torch.float64, device=gx.device)
-    saved = engine.get_option("cg_fuse")
-    engine.set_option("cg_fuse", fuse)
+    names = ("cg_fuse", "cg_xcd")
+    saved = [engine.get_option(k) for k in names]
+    for k, val in zip(names, (fuse, xcd)):
+        engine.set_option(k, val)
     try:
         git = H.solve_cg_ilu(engine, tol, gp, gi, gv, ilu, dev(engine, b), gx, hist=gh)
     finally:
-        engine.set_option("cg_fuse", saved)
+        for k, val in zip(names, saved):
+            engine.set_option(k, val)
     assert git == it
     assert np.array_equal(bits(host(gh)[: it - 1]), bits(hist))
     assert np.array_equal(bits(host(gx)), bits(x))
