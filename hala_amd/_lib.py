@@ -26,7 +26,7 @@ pvp = C.POINTER(C.c_void_p)
 
 class CsrInfo(C.Structure):
     _fields_ = [("nrows", i32), ("ncols", i32), ("nnz", i64), ("max_row", i32), ("min_row", i32),
-                ("mean_row", f64), ("kernel", i32)]
+                ("mean_row", f64), ("kernel", i32), ("index_bytes", i32)]
 
 
 class IluInfo(C.Structure):

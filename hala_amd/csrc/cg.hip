@@ -90,9 +90,9 @@ __device__ __forceinline__ int cg_block(int xcd, int nvirt) { return xcd ? xcd_b
 
 // Ap = A p (beta 0) fused with <p, Ap>; the last block forms alpha = zr / <p,Ap>.
 // One block = 256 rows = one canonical partial.
-template <bool NT>
+template <bool NT, class TI = int>
 __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restrict__ pntr,
-                                                      const int *__restrict__ indx, const double *__restrict__ vals,
+                                                      const TI *__restrict__ indx, const double *__restrict__ vals,
                                                       const double *__restrict__ p, double *__restrict__ ap,
                                                       double *part, int *err, cg_state *st) {
     if (st->done) return;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int *__restri
     const int nb = (n + SPMV_ROWS - 1) / SPMV_ROWS;
     // grid-stride over 256-row blocks: one arrival (atomic) per workgroup, not per row block
     for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-        double acc = csr_block_rowsum<double, double, double, false, NT>(n, pntr, indx, vals, p, prod, blk);
+        double acc = csr_block_rowsum<double, double, double, false, NT, TI>(n, pntr, indx, vals, p, prod, blk);
         int i = blk * SPMV_ROWS + threadIdx.x;
         double v = 0.0;
         if (i < n) {
@@ -389,6 +389,7 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
     const int *pntr = P.transposed ? P.A->tp.tpntr.as<int>() : P.A->pntr;
     const int *indx = P.transposed ? P.A->tp.trow.as<int>() : P.A->indx;
     const double *vals = P.transposed ? P.tvals : P.vals;
+    const short *boff = P.transposed ? nullptr : P.A->boff.as<short>();
 
     const int gs = (int)((n + SPMV_ROWS - 1) / SPMV_ROWS);
     // SpMV + dot: a workgroup per 256-row block, or grid-stride over them with the option's cap per CU
@@ -442,10 +443,18 @@ static void run_cg(hala_ctx c, const cg_problem &P, const double *b, double *x, 
             pass++;
             return;
         }
-        if (nt)
+        if (boff) {  // columns as int16 offsets from the block's first row
+            if (nt)
+                cg_spmv_kernel<true, short><<<gsd, 256, 0, c->stream>>>(n, pntr, boff, vals, p, ap, part,
+                                                                        wait_error_flag(c), st);
+            else
+                cg_spmv_kernel<false, short><<<gsd, 256, 0, c->stream>>>(n, pntr, boff, vals, p, ap, part,
+                                                                         wait_error_flag(c), st);
+        } else if (nt) {
             cg_spmv_kernel<true><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, wait_error_flag(c), st);
-        else
+        } else {
             cg_spmv_kernel<false><<<gsd, 256, 0, c->stream>>>(n, pntr, indx, vals, p, ap, part, wait_error_flag(c), st);
+        }
         switch (P.prec) {
         case PREC_ID:
             cg_update_kernel<PREC_ID><<<ge, 256, 0, c->stream>>>(n, p, ap, nullptr, x, r, part, part2, wait_error_flag(c), st);

@@ -118,15 +118,16 @@ __global__ __launch_bounds__(256) void gm_residual_kernel(int n, const int *__re
     t[i] = (-1.0) * s + 1.0 * b[i];
 }
 
-// t = A r (beta 0), gated on the inner loop
-template <class TV, bool NT>
+// t = A r (beta 0), gated on the inner loop (TI = short: the matrix's int16
+// block offsets instead of its columns)
+template <class TV, bool NT, class TI = int>
 __global__ __launch_bounds__(256) void gm_spmv_kernel(int n, const int *__restrict__ pntr,
-                                                      const int *__restrict__ indx, const TV *__restrict__ vals,
+                                                      const TI *__restrict__ indx, const TV *__restrict__ vals,
                                                       const double *__restrict__ r, double *__restrict__ t,
                                                       const gm_state *st) {
     if (st->all_done || !st->in_cycle) return;
     __shared__ TV prod[spmv_tile<TV>::value];
-    TV acc = csr_block_rowsum<TV, double, TV, false, NT>(n, pntr, indx, vals, r, prod, blockIdx.x);
+    TV acc = csr_block_rowsum<TV, double, TV, false, NT, TI>(n, pntr, indx, vals, r, prod, blockIdx.x);
     int i = blockIdx.x * SPMV_ROWS + threadIdx.x;
     if (i >= n) return;
     double s = (sizeof(TV) == 4) ? (double)((TV)1 * acc + (TV)0) : (double)acc;
@@ -366,6 +367,7 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
     ga.cf = ga.Z + GM_MAX_RESTART + 1;
     double *part = partials64(c, (size_t)(restart + 1) * nb);
     const int *pntr = P.A->pntr, *indx = P.A->indx;
+    const short *boff = P.A->boff.as<short>();  // int16 block offsets of the columns (null: none)
     const int64_t nnz = P.A->nnz;
     const int gs = (int)((n + SPMV_ROWS - 1) / SPMV_ROWS);
     const int ge = (int)((nb + GK - 1) / GK);
@@ -390,10 +392,18 @@ static void run_gmres(hala_ctx c, const gm_problem &P, const double *b, double *
         // ---- inner steps (gated on the device once the cycle stops)
         for (int s = 0; s < restart; s++) {
             if (P.vals32) {
-                if (stream_nt<float>(nnz)) gm_spmv_kernel<float, true><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals32, r, t, st);
+                if (boff && stream_nt<float>(nnz))
+                    gm_spmv_kernel<float, true, short><<<gs, 256, 0, c->stream>>>(n, pntr, boff, P.vals32, r, t, st);
+                else if (boff)
+                    gm_spmv_kernel<float, false, short><<<gs, 256, 0, c->stream>>>(n, pntr, boff, P.vals32, r, t, st);
+                else if (stream_nt<float>(nnz)) gm_spmv_kernel<float, true><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals32, r, t, st);
                 else gm_spmv_kernel<float, false><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals32, r, t, st);
             } else {
-                if (stream_nt<double>(nnz)) gm_spmv_kernel<double, true><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals64, r, t, st);
+                if (boff && stream_nt<double>(nnz))
+                    gm_spmv_kernel<double, true, short><<<gs, 256, 0, c->stream>>>(n, pntr, boff, P.vals64, r, t, st);
+                else if (boff)
+                    gm_spmv_kernel<double, false, short><<<gs, 256, 0, c->stream>>>(n, pntr, boff, P.vals64, r, t, st);
+                else if (stream_nt<double>(nnz)) gm_spmv_kernel<double, true><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals64, r, t, st);
                 else gm_spmv_kernel<double, false><<<gs, 256, 0, c->stream>>>(n, pntr, indx, P.vals64, r, t, st);
             }
             if (!P.M) gm_copy_kernel<<<gp, 256, 0, c->stream>>>(n, t, r, st, 1);

@@ -38,6 +38,41 @@ __global__ void init_stats(int *out) {
     out[0] = 0;
     out[1] = INT_MAX;
     out[2] = INT_MAX;
+    out[3] = INT_MIN;
+    out[4] = INT_MIN;
+}
+
+// out[3] = max (column - r0), out[4] = max (r0 - column) over the entries,
+// r0 the first row of the entry's 256-row block (reduced over the wavefront,
+// then an atomic only where it raises the stored value)
+__global__ __launch_bounds__(256) void block_offset_kernel(int nrows, const int *__restrict__ pntr,
+                                                           const int *__restrict__ indx, int *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int hi = INT_MIN, lo = INT_MIN;
+    if (i < nrows) {
+        const int r0 = i / SPMV_ROWS * SPMV_ROWS;
+        for (int j = pntr[i]; j < pntr[i + 1]; j++) {
+            hi = max(hi, indx[j] - r0);
+            lo = max(lo, r0 - indx[j]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        hi = max(hi, __shfl_xor(hi, o, 64));
+        lo = max(lo, __shfl_xor(lo, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (hi > ld_agent(out + 3)) atomicMax(out + 3, hi);
+        if (lo > ld_agent(out + 4)) atomicMax(out + 4, lo);
+    }
+}
+
+__global__ void block_offset_fill(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
+                                  short *__restrict__ boff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int r0 = i / SPMV_ROWS * SPMV_ROWS;
+    for (int j = pntr[i]; j < pntr[i + 1]; j++) boff[j] = (short)(indx[j] - r0);
 }
 
 // out[2] = first bad row (INT_MAX if none): decreasing pointers, column out
@@ -76,13 +111,13 @@ int validate_pattern(hala_ctx c, int nrows, int ncols, int64_t nnz, const int *p
 // ---------------------------------------------------------------------------
 // y = alpha*sum + beta*y (beta == 0: y not read).  f32_inner: the FP32 sum is
 // first normalised as 1*sum + 0 in FP32, then combined in the output type.
-template <class TV, class TX, class TY, class TA, bool NT>
+template <class TV, class TX, class TY, class TA, bool NT, class TI = int>
 __global__ __launch_bounds__(256) void spmv_rows_kernel(int nrows, const int *__restrict__ pntr,
-                                                        const int *__restrict__ indx, const TV *__restrict__ vals,
+                                                        const TI *__restrict__ indx, const TV *__restrict__ vals,
                                                         const TX *__restrict__ x, TY alpha, TY beta,
                                                         TY *__restrict__ y, int f32_inner) {
     __shared__ TA prod[spmv_tile<TA>::value];
-    TA acc = csr_block_rowsum<TV, TX, TA, false, NT>(nrows, pntr, indx, vals, x, prod, blockIdx.x);
+    TA acc = csr_block_rowsum<TV, TX, TA, false, NT, TI>(nrows, pntr, indx, vals, x, prod, blockIdx.x);
     const int row = blockIdx.x * SPMV_ROWS + threadIdx.x;
     if (row >= nrows) return;
     TY s;
@@ -161,12 +196,23 @@ void spmv_axpby(hala_csr A, const TV *vals, const TX *x, TY alpha, TY beta, TY *
                                                                   y, f32_inner ? 1 : 0);
     } else {
         int g = (A->nrows + SPMV_ROWS - 1) / SPMV_ROWS;
-        if (stream_nt<TV>(A->nnz))
-            spmv_rows_kernel<TV, TX, TY, TA, true><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals, x,
-                                                                                 alpha, beta, y, f32_inner ? 1 : 0);
-        else
-            spmv_rows_kernel<TV, TX, TY, TA, false><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals,
-                                                                                  x, alpha, beta, y, f32_inner ? 1 : 0);
+        const int fi = f32_inner ? 1 : 0;
+        const short *bo = A->boff.as<short>();
+        if (stream_nt<TV>(A->nnz)) {
+            if (bo)
+                spmv_rows_kernel<TV, TX, TY, TA, true, short><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, bo, vals,
+                                                                                            x, alpha, beta, y, fi);
+            else
+                spmv_rows_kernel<TV, TX, TY, TA, true><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx, vals,
+                                                                                     x, alpha, beta, y, fi);
+        } else {
+            if (bo)
+                spmv_rows_kernel<TV, TX, TY, TA, false, short><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, bo,
+                                                                                             vals, x, alpha, beta, y, fi);
+            else
+                spmv_rows_kernel<TV, TX, TY, TA, false><<<g, SPMV_ROWS, 0, c->stream>>>(A->nrows, A->pntr, A->indx,
+                                                                                      vals, x, alpha, beta, y, fi);
+        }
     }
     HALA_CHECK_LAUNCH();
 }
@@ -271,9 +317,10 @@ int hala_csr_create(hala_ctx c, int nrows, int ncols, int64_t nnz, const int *pn
             int *st = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 2);
             init_stats<<<1, 1, 0, c->stream>>>(st);
             row_stats_kernel<<<grid_for(nrows, 256 * 8, 1024), 256, 0, c->stream>>>(nrows, pntr, st);
-            int h[2];
+            block_offset_kernel<<<(nrows + 255) / 256, 256, 0, c->stream>>>(nrows, pntr, indx, st);
+            int h[5];
             hipError_t e1 = hipGetLastError();
-            if (e1 == hipSuccess) e1 = hipMemcpyAsync(c->host_pinned, st, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+            if (e1 == hipSuccess) e1 = hipMemcpyAsync(c->host_pinned, st, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
             if (e1 == hipSuccess) e1 = hipStreamSynchronize(c->stream);
             if (e1 != hipSuccess) {
                 delete A;
@@ -288,6 +335,22 @@ int hala_csr_create(hala_ctx c, int nrows, int ncols, int64_t nnz, const int *pn
             // order.  Long rows waste the per-row thread there: one
             // wavefront per row instead.
             A->kernel = (A->mean_row > 48.0 || A->max_row > 4 * spmv_tile<double>::value) ? 1 : 0;
+            // banded patterns: columns as int16 offsets from the block's first row
+            if (A->kernel == 0 && nnz > 0 && h[3] <= SHRT_MAX && h[4] <= -(int)SHRT_MIN) {
+                try {
+                    A->boff.reserve((size_t)nnz * sizeof(short));
+                } catch (...) {
+                    delete A;
+                    throw;
+                }
+                block_offset_fill<<<(nrows + 255) / 256, 256, 0, c->stream>>>(nrows, pntr, indx, A->boff.as<short>());
+                hipError_t e2 = hipGetLastError();
+                if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+                if (e2 != hipSuccess) {
+                    delete A;
+                    throw status_error{HALA_ERR_HIP, std::string("csr offsets: ") + hipGetErrorString(e2)};
+                }
+            }
         }
         *out = A;
     });
@@ -312,6 +375,7 @@ int hala_csr_info(hala_csr A, hala_csr_info_t *info) {
         info->min_row = A->min_row;
         info->mean_row = A->mean_row;
         info->kernel = A->kernel;
+        info->index_bytes = A->boff.p ? 2 : 4;
     });
 }
 

@@ -13,6 +13,10 @@ struct hala_csr_s {
     double mean_row = 0.0;
     int kernel = 0;  // 0: LDS-staged block of rows, 1: wavefront per row
     hala_amd::csr_transpose tp;  // built on first transposed product
+    // column of every entry as an int16 offset from its 256-row block's first
+    // row, when every offset fits (banded patterns: every stencil here); null
+    // otherwise.  The LDS-staged products stream it instead of indx.
+    hala_amd::dbuf boff;
 };
 
 namespace hala_amd {
@@ -44,13 +48,17 @@ struct spmv_tile {
 // row in CSR order.  Returns the row sum (TA(0) for rows past the end).
 // SUB: start from `init` and subtract the products one at a time instead.
 // NT: values and indices by non-temporal loads.
+// TI = short: indx holds each entry's column minus the block's first row
+// (hala_csr_s::boff), two bytes instead of four per entry.
 // Must be called by all 256 threads of the block.
-template <class TV, class TX, class TA, bool SUB = false, bool NT = false>
-__device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const int *__restrict__ indx,
+template <class TV, class TX, class TA, bool SUB = false, bool NT = false, class TI = int>
+__device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict__ pntr, const TI *__restrict__ indx,
                                                const TV *__restrict__ vals, const TX *__restrict__ x, TA *prod,
                                                int blk, TA init = TA(0)) {
+    static_assert(sizeof(TI) == 4 || sizeof(TI) == 2, "int or short column indices");
     constexpr int TILE = spmv_tile<TA>::value;
     const int r0 = blk * SPMV_ROWS;
+    const int cb = sizeof(TI) == 2 ? r0 : 0;  // column base of the stored indices
     const int row = r0 + threadIdx.x;
     const int rlast = min(r0 + SPMV_ROWS, nrows);
     const int bstart = pntr[r0];
@@ -66,8 +74,9 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
         int e = threadIdx.x;
         for (; e + 3 * SPMV_ROWS < cnt; e += 4 * SPMV_ROWS) {
             const int j = base + e;
-            int c0 = ld_stream<NT>(indx + j), c1 = ld_stream<NT>(indx + j + SPMV_ROWS),
-                c2 = ld_stream<NT>(indx + j + 2 * SPMV_ROWS), c3 = ld_stream<NT>(indx + j + 3 * SPMV_ROWS);
+            int c0 = cb + (int)ld_stream<NT>(indx + j), c1 = cb + (int)ld_stream<NT>(indx + j + SPMV_ROWS),
+                c2 = cb + (int)ld_stream<NT>(indx + j + 2 * SPMV_ROWS),
+                c3 = cb + (int)ld_stream<NT>(indx + j + 3 * SPMV_ROWS);
             TV v0 = ld_stream<NT>(vals + j), v1 = ld_stream<NT>(vals + j + SPMV_ROWS),
                v2 = ld_stream<NT>(vals + j + 2 * SPMV_ROWS), v3 = ld_stream<NT>(vals + j + 3 * SPMV_ROWS);
             TA x0 = (TA)x[c0], x1 = (TA)x[c1], x2 = (TA)x[c2], x3 = (TA)x[c3];
@@ -78,7 +87,7 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
         }
         for (; e < cnt; e += SPMV_ROWS) {
             const int j = base + e;
-            prod[e] = (TA)ld_stream<NT>(vals + j) * (TA)x[ld_stream<NT>(indx + j)];
+            prod[e] = (TA)ld_stream<NT>(vals + j) * (TA)x[cb + (int)ld_stream<NT>(indx + j)];
         }
         __syncthreads();
         const int a = max(rs, base), b = min(re, base + cnt);

@@ -121,6 +121,9 @@ typedef struct {
     int max_row, min_row;
     double mean_row;
     int kernel; /* 0: LDS-staged block-of-rows, 1: wavefront per row */
+    int index_bytes; /* 2: the block-of-rows products stream each column as an
+                        int16 offset from its 256-row block's first row (every
+                        offset fits: banded patterns); 4: the caller's indx */
 } hala_csr_info_t;
 int hala_csr_create(hala_ctx ctx, int nrows, int ncols, int64_t nnz, const int *pntr, const int *indx,
                     hala_csr *csr);

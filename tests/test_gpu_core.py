@@ -88,6 +88,53 @@ def test_spmv_f64_bitexact(engine, name):
     assert np.array_equal(bits(host(gy)), bits(ref))
 
 
+@pytest.mark.parametrize("hi,lo,short", [(32767, 32768, True), (32768, 32768, False), (32767, 32769, False)],
+                         ids=["int16-limits", "past-high", "past-low"])
+def test_spmv_block_offsets(engine, hi, lo, short):
+    # columns streamed as int16 offsets from the 256-row block's first row when
+    # every offset fits (-32768 .. 32767), the caller's int32 columns otherwise;
+    # both match the oracle bit for bit, also inside CG
+    n = 70000
+    rows = []
+    for r in range(n):
+        r0 = r // 256 * 256
+        c = {r}
+        if r0 + hi < n and r % 7 == 0:
+            c.add(r0 + hi)  # the largest offset above the block start
+        if r0 - lo >= 0 and r % 5 == 0:
+            c.add(r0 - lo)  # the largest below
+        if r > 0:
+            c.add(r - 1)
+        rows.append(sorted(c))
+    p = np.zeros(n + 1, np.int32)
+    p[1:] = np.cumsum([len(c) for c in rows])
+    i = np.concatenate([np.array(c, np.int32) for c in rows])
+    v = np.random.default_rng(12).uniform(-1, 1, len(i))
+    gp, gi, gv = upload(engine, p, i, v)
+    A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+    assert A.info()["kernel"] == 0
+    assert A.info()["index_bytes"] == (2 if short else 4)
+    x, y0 = st.uniform(31, n), st.uniform(32, n)
+    for alpha, beta in [(1.0, 0.0), (2.5, -0.5)]:
+        gx, gy = dev(engine, x), dev(engine, y0)
+        A.gemv("N", alpha, gx, beta, gy)
+        ref = oracle.spmv("N", n, n, alpha, p, i, v, x, beta, y0)
+        assert np.array_equal(bits(host(gy)), bits(ref)), (alpha, beta)
+
+
+def test_spmv_block_offsets_stencils(engine):
+    # the stencil operators (C2's 7-point 128^3 included) take the int16 offsets
+    for p, i, v in (st.laplace3d(128, 128, 128), st.convdiff27(24, seed=2), st.laplace2d(300, 200)):
+        n = len(p) - 1
+        gp, gi, gv = upload(engine, p, i, v)
+        A = H.make_sparse_matrix(engine, n, gp, gi, gv)
+        assert A.info()["index_bytes"] == 2
+        x = st.uniform(41, n)
+        gx, gy = dev(engine, x), engine.vector(n)
+        A.gemv("N", 1.0, gx, 0.0, gy)
+        assert np.array_equal(bits(host(gy)), bits(oracle.spmv("N", n, n, 1.0, p, i, v, x, 0.0, np.zeros(n))))
+
+
 def _long_row_pattern(n, per_row, seed, one_long=0):
     rng = np.random.default_rng(seed)
     rows = [np.unique(np.concatenate([[r], rng.choice(n, per_row, replace=False)])) for r in range(n)]
