@@ -26,6 +26,7 @@ struct hala_ilu_s {
     // FP32 (and, for FP64 applies, FP64) values and diagonal, four iterates of
     // each precision (built on first use)
     hala_amd::dbuf jl_ptr, jl_idx, jl_val, ju_ptr, ju_idx, ju_val, jd_val, jwork;
+    hala_amd::dbuf jl_off, ju_off;  // the strict parts' columns as int16 block offsets (empty: not banded)
     hala_amd::dbuf jl_val64, ju_val64, jd_val64, jwork64;
     bool jac_pattern = false, jac_values = false, jac_values64 = false;
     // complex factors (cplx.hip): interleaved real/imaginary, CSR order

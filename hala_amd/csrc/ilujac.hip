@@ -58,8 +58,8 @@ __global__ void jac_fill_val(int n, const int *__restrict__ pntr, const int *__r
 
 // One sweep over a packed triangle: xn_i = (b_i - v_ij xo_j - ...) [/ d_i], in V.
 // ZINIT: also z0_i = xn_i / d_i (the last L sweep starts the U sweeps; d is U's diagonal).
-template <class V, class TB, class TX, class TO, bool UNIT, bool ZINIT, bool NT>
-__global__ __launch_bounds__(256) void jac_sweep(int n, const int *__restrict__ ptr, const int *__restrict__ idx,
+template <class V, class TB, class TX, class TO, bool UNIT, bool ZINIT, bool NT, class TI>
+__global__ __launch_bounds__(256) void jac_sweep(int n, const int *__restrict__ ptr, const TI *__restrict__ idx,
                                                  const V *__restrict__ val, const V *__restrict__ d,
                                                  const TB *__restrict__ b, const TX *__restrict__ xo,
                                                  TO *__restrict__ xn, V *__restrict__ z0, const int *gate) {
@@ -67,20 +67,26 @@ __global__ __launch_bounds__(256) void jac_sweep(int n, const int *__restrict__ 
     __shared__ V prod[spmv_tile<V>::value];
     const int i = blockIdx.x * SPMV_ROWS + threadIdx.x;
     const V init = i < n ? (V)b[i] : V(0);
-    V acc = csr_block_rowsum<V, TX, V, true, NT>(n, ptr, idx, val, xo, prod, blockIdx.x, init);
+    V acc = csr_block_rowsum<V, TX, V, true, NT, TI>(n, ptr, idx, val, xo, prod, blockIdx.x, init);
     if (i >= n) return;
     if (!UNIT) acc = acc / d[i];
     xn[i] = (TO)acc;
     if (ZINIT) z0[i] = acc / d[i];
 }
 
+// off (nullable): the triangle's columns as int16 block offsets, read instead of idx
 template <class V, class TB, class TX, class TO, bool UNIT, bool ZINIT>
-void sweep(hala_ctx c, int n, const int *ptr, const int *idx, const V *val, const V *d, const TB *b, const TX *xo,
-           TO *xn, V *z0, const int *gate) {
+void sweep(hala_ctx c, int n, const int *ptr, const int *idx, const short *off, const V *val, const V *d,
+           const TB *b, const TX *xo, TO *xn, V *z0, const int *gate) {
     // cached loads: the k sweeps of a triangle re-read it back to back, from the MALL
     // (non-temporal streams measured 20 % slower here)
     const int g = (n + SPMV_ROWS - 1) / SPMV_ROWS;
-    jac_sweep<V, TB, TX, TO, UNIT, ZINIT, false><<<g, 256, 0, c->stream>>>(n, ptr, idx, val, d, b, xo, xn, z0, gate);
+    if (off)
+        jac_sweep<V, TB, TX, TO, UNIT, ZINIT, false, short><<<g, 256, 0, c->stream>>>(n, ptr, off, val, d, b, xo, xn,
+                                                                                     z0, gate);
+    else
+        jac_sweep<V, TB, TX, TO, UNIT, ZINIT, false, int><<<g, 256, 0, c->stream>>>(n, ptr, idx, val, d, b, xo, xn, z0,
+                                                                                   gate);
     HALA_CHECK_LAUNCH();
 }
 
@@ -104,6 +110,9 @@ void jac_pattern(hala_ilu M) {
     jac_fill_idx<<<g, 256, 0, c->stream>>>(n, M->pntr, M->indx, M->diag.as<int>(), M->jl_ptr.as<int>(),
                                            M->ju_ptr.as<int>(), M->jl_idx.as<int>(), M->ju_idx.as<int>());
     HALA_CHECK_LAUNCH();
+    // banded triangles: the sweeps stream int16 block offsets instead (6 instead of 8 bytes per FP32 entry)
+    if (!build_block_offsets(c, n, nl, M->jl_ptr.as<int>(), M->jl_idx.as<int>(), M->jl_off)) M->jl_off.release();
+    if (!build_block_offsets(c, n, nu, M->ju_ptr.as<int>(), M->ju_idx.as<int>(), M->ju_off)) M->ju_off.release();
     M->jac_pattern = true;
 }
 
@@ -133,25 +142,26 @@ void jac_apply(hala_ilu M, const double *x, double *r, const int *gate, int k, c
     V *ybuf[2] = {w, w + n}, *zbuf[2] = {w + 2 * (size_t)n, w + 3 * (size_t)n};
     const int *lp = M->jl_ptr.as<int>(), *li = M->jl_idx.as<int>();
     const int *up = M->ju_ptr.as<int>(), *ui = M->ju_idx.as<int>();
+    const short *lo = M->jl_off.as<short>(), *uo = M->ju_off.as<short>();
     // L sweeps: y(m+1) = x - (L - I) y(m), y0 = x; the last one also writes z0 = y / d
     for (int m = 0; m < k; m++) {
         V *yn = ybuf[m & 1];
         const bool last = m == k - 1;
         if (m == 0) {
-            if (last) sweep<V, double, double, V, true, true>(c, n, lp, li, lv, dv, x, x, yn, zbuf[0], gate);
-            else sweep<V, double, double, V, true, false>(c, n, lp, li, lv, dv, x, x, yn, nullptr, gate);
+            if (last) sweep<V, double, double, V, true, true>(c, n, lp, li, lo, lv, dv, x, x, yn, zbuf[0], gate);
+            else sweep<V, double, double, V, true, false>(c, n, lp, li, lo, lv, dv, x, x, yn, nullptr, gate);
         } else {
             const V *yo = ybuf[(m - 1) & 1];
-            if (last) sweep<V, double, V, V, true, true>(c, n, lp, li, lv, dv, x, yo, yn, zbuf[0], gate);
-            else sweep<V, double, V, V, true, false>(c, n, lp, li, lv, dv, x, yo, yn, nullptr, gate);
+            if (last) sweep<V, double, V, V, true, true>(c, n, lp, li, lo, lv, dv, x, yo, yn, zbuf[0], gate);
+            else sweep<V, double, V, V, true, false>(c, n, lp, li, lo, lv, dv, x, yo, yn, nullptr, gate);
         }
     }
     // U sweeps: z(m+1) = (y - (U - D) z(m)) / d; the last writes the FP64 result
     const V *y = ybuf[(k - 1) & 1];
     for (int m = 0; m < k; m++) {
         const V *zo = zbuf[m & 1];
-        if (m == k - 1) sweep<V, V, V, double, false, false>(c, n, up, ui, uv, dv, y, zo, r, nullptr, gate);
-        else sweep<V, V, V, V, false, false>(c, n, up, ui, uv, dv, y, zo, zbuf[(m + 1) & 1], nullptr, gate);
+        if (m == k - 1) sweep<V, V, V, double, false, false>(c, n, up, ui, uo, uv, dv, y, zo, r, nullptr, gate);
+        else sweep<V, V, V, V, false, false>(c, n, up, ui, uo, uv, dv, y, zo, zbuf[(m + 1) & 1], nullptr, gate);
     }
 }
 

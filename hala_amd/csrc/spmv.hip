@@ -75,6 +75,25 @@ __global__ void block_offset_fill(int nrows, const int *__restrict__ pntr, const
     for (int j = pntr[i]; j < pntr[i + 1]; j++) boff[j] = (short)(indx[j] - r0);
 }
 
+bool build_block_offsets(hala_ctx c, int nrows, int64_t nnz, const int *ptr, const int *idx, dbuf &out) {
+    if (nrows <= 0 || nnz <= 0) return false;
+    tmp_scope ts(c);
+    dbuf &st = ts.get();
+    st.reserve(5 * sizeof(int));
+    init_stats<<<1, 1, 0, c->stream>>>(st.as<int>());
+    block_offset_kernel<<<(nrows + 255) / 256, 256, 0, c->stream>>>(nrows, ptr, idx, st.as<int>());
+    HALA_CHECK_LAUNCH();
+    int h[5];
+    HALA_HIP(hipMemcpyAsync(c->host_pinned, st.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HALA_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(h, c->host_pinned, sizeof(h));
+    if (h[3] > SHRT_MAX || h[4] > -(int)SHRT_MIN) return false;
+    out.reserve((size_t)nnz * sizeof(short));
+    block_offset_fill<<<(nrows + 255) / 256, 256, 0, c->stream>>>(nrows, ptr, idx, out.as<short>());
+    HALA_CHECK_LAUNCH();
+    return true;
+}
+
 // out[2] = first bad row (INT_MAX if none): decreasing pointers, column out
 // of range, unsorted/duplicate columns, or (require_diag) no diagonal entry.
 __global__ void validate_kernel(int nrows, int ncols, int64_t nnz, const int *__restrict__ pntr,

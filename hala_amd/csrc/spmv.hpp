@@ -103,6 +103,12 @@ __device__ __forceinline__ TA csr_block_rowsum(int nrows, const int *__restrict_
     return csr_block_rowsum<TV, TX, TA>(nrows, pntr, indx, vals, x, prod, blockIdx.x);
 }
 
+// The int16 copy of a pattern's columns as offsets from the first row of each
+// entry's 256-row block (what csr_block_rowsum<..., short> reads), built into
+// out when every offset fits; false (out untouched) otherwise.  Reads two
+// counters back (synchronises the stream).
+bool build_block_offsets(hala_ctx c, int nrows, int64_t nnz, const int *ptr, const int *idx, dbuf &out);
+
 // First row of an invalid pattern (-1 when valid): pointers must start at 0,
 // end at nnz and not decrease; columns sorted, unique, inside [0, ncols);
 // require_diag: every row holds its diagonal.

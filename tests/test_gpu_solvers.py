@@ -151,6 +151,50 @@ def test_ilu_jacobi_apply_bitexact(engine, k, prec):
         engine.set_option(opt, saved)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ilu_jacobi_apply_wide_band(engine, prec):
+    """Jacobi-applied ILU on a pattern whose strict parts reach more than 32767 rows from
+    their block's first row: the sweeps read the int32 columns (banded operators stream int16
+    offsets instead); bit for bit with the oracle either way."""
+    n, far = 40000, 33000
+    rows = []
+    for r in range(n):
+        c = {r}
+        if r > 0:
+            c.add(r - 1)
+        if r + 1 < n:
+            c.add(r + 1)
+        if r % 97 == 0 and r + far < n:
+            c.add(r + far)
+        if (r - far) % 97 == 0 and r - far >= 0:
+            c.add(r - far)  # the transposed entry: a symmetric pattern
+        rows.append(sorted(c))
+    p = np.zeros(n + 1, np.int32)
+    p[1:] = np.cumsum([len(c) for c in rows])
+    i = np.concatenate([np.array(c, np.int32) for c in rows])
+    rr = np.repeat(np.arange(n), np.diff(p))
+    v = np.where(i == rr, 8.0, -1.0) + np.random.default_rng(5).uniform(-0.1, 0.1, len(i))
+    opt = "ilu32_jacobi" if prec == "f32" else "ilu64_jacobi"
+    saved = engine.get_option(opt)
+    try:
+        dg, _ = oracle.diag(p, i)
+        f = oracle.ilu0(p, i, v, dg)
+        M = H.make_ilu(engine, *upload(engine, p, i, v), want_f32=True)
+        x = st.uniform(62, n)
+        gr = engine.vector(n)
+        engine.set_option(opt, 2)
+        if prec == "f32":
+            M.apply_f32x64(dev(engine, x), gr)
+            ref = oracle.ilu_apply_f32_jacobi(p, i, dg, f.astype(np.float32), x.astype(np.float32),
+                                              2).astype(np.float64)
+        else:
+            M.apply(dev(engine, x), gr)
+            ref = oracle.ilu_apply_jacobi(p, i, dg, f, x, 2)
+        assert np.array_equal(bits(host(gr)), bits(ref))
+    finally:
+        engine.set_option(opt, saved)
+
+
 @pytest.mark.parametrize("fp32", [True, False], ids=["fp32_inner", "fp64_inner"])
 def test_gmres_ir_jacobi_inner_bitexact(engine, fp32):
     import torch
