@@ -499,18 +499,18 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
                                                                 M->pairs.as<int2>(), vals, f, M->fwork.as<double>(),
                                                                 wait_error_flag(c));
         HALA_CHECK_LAUNCH();
-        check_wait_error(c);
         int *bad = reinterpret_cast<int *>(c->scal.as<double>() + S_DOT + 6);
         set_int<<<1, 1, 0, c->stream>>>(bad, INT_MAX);
         pivot_check<<<(M->n + 255) / 256, 256, 0, c->stream>>>(M->n, M->diag.as<int>(), f, bad);
         HALA_CHECK_LAUNCH();
-        int b = fetch_i32(c, bad);
         M->factored = false;
         M->adjoint_loaded = false;
         M->jac_values = false;
         M->jac_values64 = false;
-        HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
-                     "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
+        // The values are packed without waiting for the host; the
+        // factorisation's dependency waits and its pivots are checked once
+        // everything has run (packing a factor that fails is harmless: it
+        // stays unfactored).
         // the two triangles' values are packed side by side: U's on the
         // auxiliary context's stream, joined back before anything reads them
         // (scattered gathers from the factor, mostly latency)
@@ -540,9 +540,12 @@ int hala_ilu_factor_f64(hala_ilu M, const double *vals, int want_f32) {
         (void)hipEventDestroy(ev[1]);
         col_set_handoff(c, M->L, M->U);
         if (M->sp) stencil_load(M, want_f32);
+        check_wait_error(c);  // synchronises; a dependency wait that gave up throws here
+        const int b = fetch_i32(c, bad);
+        HALA_REQUIRE(b == INT_MAX, HALA_ERR_PIVOT,
+                     "hala_ilu_factor_f64: zero or non-finite pivot in row " + std::to_string(b));
         if (want_f32) M->has32 = true;
         M->factored = true;
-        HALA_HIP(hipStreamSynchronize(c->stream));
     });
 }
 
