@@ -302,6 +302,35 @@ def test_tridiagonal_ilu_is_exact_lu():
     np.testing.assert_allclose(L @ U, dense(p, i, v, 5), atol=1e-15)
 
 
+def _mirror_certificate(p, i):
+    """(U's level recurrence holds for the candidate nlev_L - 1 - lev_L, candidate == U's levels)."""
+    n = len(p) - 1
+    dg, _ = oracle.diag(p, i)
+    lo, _, _ = oracle.levels(p, i, dg, False)
+    up, _, _ = oracle.levels(p, i, dg, True)
+    cand = lo.max() - lo
+    ok = all((0 if dg[r] + 1 == p[r + 1] else 1 + int(cand[i[dg[r] + 1:p[r + 1]]].max())) == cand[r]
+             for r in range(n))
+    return ok, bool(np.array_equal(up, cand))
+
+
+def test_level_mirror_certificate():
+    # U's levels are taken from L's (nlev_L - 1 - lev_L) when that candidate satisfies U's own
+    # recurrence (0 without upper entries, else 1 + the largest over them); the recurrence has one
+    # solution, so the check passes exactly when the candidate is U's level set.  It holds on the
+    # stencils (every row lies on a longest chain) and fails on rows off it.
+    for p, i, v in (st.laplace2d(9, 7), st.laplace3d(6, 5, 4), st.convdiff27(5, seed=1)):
+        assert _mirror_certificate(p, i) == (True, True)
+    # a 2-D grid beside a disconnected 3-row chain: the chain's rows are off the longest chain
+    p, i, v = st.laplace2d(8, 8)
+    n0 = len(p) - 1
+    rows = [list(i[p[r]:p[r + 1]]) for r in range(n0)] + [[n0, n0 + 1], [n0, n0 + 1, n0 + 2], [n0 + 1, n0 + 2]]
+    P = np.zeros(len(rows) + 1, np.int32)
+    P[1:] = np.cumsum([len(c) for c in rows])
+    I = np.concatenate([np.array(c, np.int32) for c in rows])
+    assert _mirror_certificate(P, I) == (False, False)
+
+
 def test_level_set_closed_forms():
     # natural x-fastest ordering: 5-pt level = i + j; 7-pt = i + j + k; 27-pt = i + 2j + 4k
     for (nx, ny), (p, i, v) in [((7, 5), st.laplace2d(7, 5))]:
